@@ -1,0 +1,130 @@
+/*
+ * moraine_hip.h -- C ABI of the MI355X (gfx950) implementation of moraine's
+ * distributed-scatterer hot path:  ks_test -> select_shp -> emperical_co[_pc]
+ * -> emi -> temporal coherence.
+ *
+ * Conventions (all entry points):
+ *   - every array pointer is DEVICE memory, C-contiguous, caller-owned; the
+ *     library never allocates device memory except where a workspace query
+ *     says so (the caller passes that workspace in);
+ *   - complex64 arrays are interleaved (re, im) float pairs, passed as float*;
+ *   - `stream` is a hipStream_t (NULL = legacy default stream); every call is
+ *     asynchronous on that stream;
+ *   - return value: MR_OK (0) or an MR_ERR_* code; mr_last_error_string()
+ *     gives a message (thread-local);
+ *   - kernels never fail silently: invalid cells are written as NaN exactly
+ *     where the reference writes NaN.
+ * Reference citations are paths in moraine v0.8.5.
+ */
+#ifndef MORAINE_HIP_H
+#define MORAINE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MR_OK 0
+#define MR_ERR_ARG 1         /* bad shape / argument */
+#define MR_ERR_HIP 2         /* HIP runtime error */
+#define MR_ERR_UNSUPPORTED 3 /* size outside what this build supports */
+
+typedef void* mr_stream_t; /* hipStream_t */
+
+const char* mr_last_error_string(void);
+int mr_abi_version(void);
+
+/* ---------------------------------------------------------------------------
+ * KS test + fused SHP selection.
+ * Replaces moraine.ks_test (moraine/shp.py:269-305: _sort_numba/cp.sort :117-125,
+ * :290; _ks_test[_no_dist]_kernel :130-266) and moraine.select_shp
+ * (moraine/shp.py:308-323).
+ *   rmli      (nl, w, n) float32 | float64 amplitude/intensity stack (unsorted)
+ *   p_out     (nl, w, 2az+1, 2r+1) same dtype, or NULL        (ks_test p)
+ *   dist_out  same shape/dtype, or NULL                          (return_dist)
+ *   mask_out  same shape, uint8 (0/1), or NULL                   (is_shp)
+ *   shp_num   (nl, w) int32, or NULL                             (shp_num)
+ *   p_max     SHP threshold used for mask_out/shp_num (is_shp = p < p_max)
+ *   workspace >= mr_ks_workspace_bytes(nl, w, n) bytes of device memory
+ * n must be 1..MR_KS_MAX_N.  p and dist are bit-exact with the reference CPU
+ * path: p = dtype(ks_p_f64(lambda(k))) with k = N*D the integer merge-walk step
+ * count.
+ * ------------------------------------------------------------------------- */
+#define MR_KS_MAX_N 256
+size_t mr_ks_workspace_bytes(int nl, int w, int n);
+int mr_ks_test_f32(const float* rmli, int nl, int w, int n, int az, int r, double p_max,
+                   float* p_out, float* dist_out, uint8_t* mask_out, int32_t* shp_num,
+                   void* workspace, size_t workspace_bytes, mr_stream_t stream);
+int mr_ks_test_f64(const double* rmli, int nl, int w, int n, int az, int r, double p_max,
+                   double* p_out, double* dist_out, uint8_t* mask_out, int32_t* shp_num,
+                   void* workspace, size_t workspace_bytes, mr_stream_t stream);
+
+/* Host-only (no GPU): the k -> (p, dist, is_shp) tables ks_test uses, for
+ * k = 0..n (n+1 entries each; shp as 0/1 bytes).  Exposed so the tables can be
+ * checked against the reference series without a device. */
+int mr_ks_tables_f32(int n, double p_max, float* p, float* dist, uint8_t* shp);
+
+/* select_shp on an existing p array (moraine/shp.py:308-323): is_shp = p < p_max
+ * (NaN -> 0), shp_num = count per pixel.  p: (npix, win) float32. */
+int mr_select_shp_f32(const float* p, long long npix, int win, double p_max, uint8_t* is_shp,
+                      int32_t* shp_num, mr_stream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * Coherence estimation.
+ * mr_emperical_co replaces moraine.emperical_co (moraine/co.py:117-137,
+ * kernel :61-114): per pixel of rows [row0, row0+nrows) and all (m, j):
+ *   cov[m,j] = S[m,j] / n_shp,  coh[m,j] = S[m,j] / sqrt(P[m] P[j]),
+ *   S = sum over in-bounds SHPs of z_m conj(z_j), P[m] = sum |z_m|^2.
+ *   rslc (nl, w, n) complex64;  is_shp (nl, w, aw, rw) uint8 (full raster);
+ *   cov, coh: (nrows, w, n, n) complex64 (cov may be NULL).  Half windows are
+ *   (aw-1)/2, (rw-1)/2 as in the reference.
+ * mr_emperical_co_pc replaces moraine.emperical_co_pc (moraine/co.py:230-266,
+ * kernel :141-190, numba :193-227):
+ *   idx (n_pc, 2) int32 (az, r); pc_is_shp (n_pc, aw, rw) uint8;
+ *   pairs (n_pairs, 2) int32 (ref < sec); coh (n_pc, n_pairs) complex64.
+ * ------------------------------------------------------------------------- */
+int mr_emperical_co(const float* rslc, const uint8_t* is_shp, int nl, int w, int n, int aw, int rw,
+                    int row0, int nrows, float* cov, float* coh, mr_stream_t stream);
+int mr_emperical_co_pc(const float* rslc, int nl, int w, int n, const int32_t* idx,
+                       const uint8_t* pc_is_shp, int n_pc, int aw, int rw, const int32_t* pairs,
+                       int n_pairs, float* coh, mr_stream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * EMI phase linking.  Replaces moraine.emi (moraine/pl.py:68-77; _emi_cp
+ * :18-42; _emi_numba :45-65).
+ *   coh (n_pts, n(n-1)/2) complex64 in triu order (all pairs);
+ *   ph (n_pts, n) complex64; quality (n_pts,) float32 (smallest eigenvalue).
+ * Gamma = uncompress(coh); M = inv(|Gamma|) o Gamma; v = eigenvector of the
+ * smallest eigenvalue of M; ph = v*conj(v[ref]) / |v*conj(v[ref])|.
+ * inv(|Gamma|) is formed in FP64; the eigen-solve precision is stated in the
+ * implementation notes of emi.hip.  Singular |Gamma| gives NaN for that point.
+ * workspace >= mr_emi_workspace_bytes(n_pts, n).
+ * ------------------------------------------------------------------------- */
+size_t mr_emi_workspace_bytes(long long n_pts, int n);
+int mr_emi(const float* coh, long long n_pts, int n, int ref, float* ph, float* quality,
+           void* workspace, size_t workspace_bytes, mr_stream_t stream);
+
+/* uncompress_coh (moraine/co.py:281-299): (n_pts, n_pairs) -> (n_pts, n, n),
+ * [a,b] = c, [b,a] = conj(c), diagonal 1, other entries 0. */
+int mr_uncompress_coh(const float* coh, long long n_pts, int n, const int32_t* pairs, int n_pairs,
+                      float* out, mr_stream_t stream);
+
+/* ---------------------------------------------------------------------------
+ * Temporal coherence.
+ * mr_ds_temp_coh replaces moraine.ds_temp_coh (moraine/pl.py:126-145, numba
+ * :80-100, kernel :103-123): t = |sum_p coh_p/|coh_p| conj(ph_a) ph_b| / n_pairs.
+ * mr_temp_coh replaces moraine.temp_coh (moraine/pqm.py:124-159, numba :46-71,
+ * kernel :99-121): t = |sum_p intf_p conj(r_a) r_b / (|intf_p||r_a||r_b|)| / n_pairs.
+ * Inputs are never modified.
+ * ------------------------------------------------------------------------- */
+int mr_ds_temp_coh(const float* coh, const float* ph, long long n_pts, int n, const int32_t* pairs,
+                   int n_pairs, float* t_coh, mr_stream_t stream);
+int mr_temp_coh(const float* intf, const float* rslc, long long n_pts, int n, const int32_t* pairs,
+                int n_pairs, float* t_coh, mr_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MORAINE_HIP_H */

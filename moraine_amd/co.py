@@ -1,0 +1,77 @@
+"""Covariance / coherence matrix estimation on the GPU.
+
+Drop-in for moraine.emperical_co (moraine/co.py:117-137),
+moraine.emperical_co_pc (moraine/co.py:230-266) and moraine.uncompress_coh
+(moraine/co.py:281-299).  Kernels: csrc/co.hip.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _dev, _lib
+from .tnet import TempNet, nimage_from_npair
+
+
+def emperical_co(rslc, is_shp, block_size: int = 128):
+    """Full-raster (cov, coh), each (nlines, width, nimages, nimages) complex64.
+
+    cov = sum/n_shp, coh = sum/sqrt(P_m P_j) over the in-bounds SHPs of each
+    pixel's window; half windows are (az_win-1)//2, (r_win-1)//2.
+    """
+    as_torch = _dev.is_torch(rslc)
+    z = _dev.to_dev(rslc, torch.complex64)
+    m = _dev.to_dev(is_shp, torch.bool)
+    nl, w, n = z.shape
+    aw, rw = m.shape[-2:]
+    if tuple(m.shape[:2]) != (nl, w):
+        raise ValueError("emperical_co: is_shp must be (nlines, width, az_win, r_win)")
+    cov = torch.empty((nl, w, n, n), dtype=torch.complex64, device=z.device)
+    coh = torch.empty_like(cov)
+    _lib.call("mr_emperical_co", _dev.ptr(z), _dev.ptr(m), nl, w, n, aw, rw, 0, nl, _dev.ptr(cov), _dev.ptr(coh),
+              _dev.stream())
+    return _dev.back(cov, as_torch), _dev.back(coh, as_torch)
+
+
+def _pairs(image_pairs, n):
+    if image_pairs is None:
+        image_pairs = TempNet.from_bandwidth(n).image_pairs
+    if _dev.is_torch(image_pairs):
+        image_pairs = image_pairs.cpu().numpy()
+    return np.ascontiguousarray(np.asarray(image_pairs, dtype=np.int32))
+
+
+def emperical_co_pc(rslc, idx, pc_is_shp, block_size: int = 128, image_pairs=None):
+    """Coherence of the selected image pairs for a point cloud: (n_pc, n_pairs) complex64."""
+    as_torch = _dev.is_torch(rslc)
+    z = _dev.to_dev(rslc, torch.complex64)
+    nl, w, n = z.shape
+    ix = _dev.to_dev(idx, torch.int32)
+    m = _dev.to_dev(pc_is_shp, torch.bool)
+    aw, rw = m.shape[-2:]
+    pairs_np = _pairs(image_pairs, n)
+    pairs = _dev.to_dev(pairs_np)
+    n_pc = ix.shape[0]
+    coh = torch.empty((n_pc, pairs_np.shape[0]), dtype=torch.complex64, device=z.device)
+    _lib.call("mr_emperical_co_pc", _dev.ptr(z), nl, w, n, _dev.ptr(ix), _dev.ptr(m), n_pc, aw, rw, _dev.ptr(pairs),
+              pairs_np.shape[0], _dev.ptr(coh), _dev.stream())
+    return _dev.back(coh, as_torch)
+
+
+def uncompress_coh(coh, image_pairs=None):
+    """(..., n_pairs) compressed -> (..., n, n) Hermitian with unit diagonal."""
+    as_torch = _dev.is_torch(coh)
+    c = _dev.to_dev(coh, torch.complex64)
+    if image_pairs is None:
+        n = nimage_from_npair(c.shape[-1])
+        pairs_np = TempNet.from_bandwidth(n).image_pairs
+    else:
+        pairs_np = np.asarray(image_pairs.cpu() if _dev.is_torch(image_pairs) else image_pairs, np.int32)
+        n = int(pairs_np[-1, -1]) + 1
+    lead = tuple(c.shape[:-1])
+    npts = int(np.prod(lead)) if lead else 1
+    out = torch.empty(lead + (n, n), dtype=torch.complex64, device=c.device)
+    pairs = _dev.to_dev(np.ascontiguousarray(pairs_np))
+    _lib.call("mr_uncompress_coh", _dev.ptr(c), npts, n, _dev.ptr(pairs), pairs_np.shape[0], _dev.ptr(out),
+              _dev.stream())
+    return _dev.back(out, as_torch)

@@ -1,0 +1,766 @@
+// EMI phase linking for gfx950.  Replaces moraine.emi (moraine/pl.py:68-77;
+// _emi_cp :18-42 = cupy inv + eigh; _emi_numba :45-65 = LAPACK inv + eigh).
+//
+// Per point:  Gamma = uncompress(coh);  A = |Gamma|;  M = inv(A) o Gamma;
+// (lambda_min, v) = smallest eigenpair of M;  ph = v conj(v_ref)/|.|;
+// quality = lambda_min.
+//
+// Algorithm (same in both paths):
+//   1. inv(A) in FP64 by the symmetric sweep operator with largest-|diagonal|
+//      pivoting.  A is symmetric but indefinite whenever the SHP count is below
+//      N (|Gamma| of a rank-deficient coherence matrix), so Cholesky cannot be
+//      used; the sweep keeps symmetry, needs no row exchanges and its result
+//      does not depend on the pivot order.
+//   2. M = inv(A) o Gamma.
+//   3. Householder reduction of the Hermitian M to real symmetric tridiagonal
+//      form T (unitary diagonal scaling absorbs the sub-diagonal phases).
+//   4. Smallest eigenvalue of T by multisection on Sturm counts (FP64, 64
+//      shifts per round, one per lane), eigenvector of T by inverse iteration
+//      with a pivoted tridiagonal LU (FP64), back-transformation through the
+//      stored reflectors.
+// Fast path (n <= 64): one wave per point, lane i owns row i of the matrix in
+// VGPRs (FP64 for step 1, FP32 complex for steps 2-4); a uniform runtime
+// column index is served by a switch (scalar branch tree + v_mov), never by
+// scratch memory.  General path (n > 64): one 256-thread block per point,
+// everything FP64 in a per-block global workspace.
+#include "common.hpp"
+
+namespace mr {
+
+#define MR_C8(B) C(B + 0) C(B + 1) C(B + 2) C(B + 3) C(B + 4) C(B + 5) C(B + 6) C(B + 7)
+#define MR_C64 MR_C8(0) MR_C8(8) MR_C8(16) MR_C8(24) MR_C8(32) MR_C8(40) MR_C8(48) MR_C8(56)
+
+template <typename V, int NM>
+__device__ __forceinline__ V reg_get(const V (&a)[NM], int k) {
+  V v = V(0);
+  switch (k) {
+#define C(i) \
+  case i:    \
+    if constexpr ((i) < NM) v = a[i]; \
+    break;
+    MR_C64
+#undef C
+    default:
+      break;
+  }
+  return v;
+}
+
+template <typename V, int NM>
+__device__ __forceinline__ void reg_set(V (&a)[NM], int k, V v) {
+  switch (k) {
+#define C(i) \
+  case i:    \
+    if constexpr ((i) < NM) a[i] = v; \
+    break;
+    MR_C64
+#undef C
+    default:
+      break;
+  }
+}
+
+__device__ __forceinline__ int wave_argmax(double key, int lane) {
+  int idx = lane;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double ok = __shfl_xor(key, o, 64);
+    const int oi = __shfl_xor(idx, o, 64);
+    if (ok > key || (ok == key && oi < idx)) {
+      key = ok;
+      idx = oi;
+    }
+  }
+  return __builtin_amdgcn_readfirstlane(idx);
+}
+
+// index of pair (a, b), a < b, in triu order (moraine/tnet.py:15-20)
+__device__ __forceinline__ int pair_index(int a, int b, int n) { return a * (2 * n - a - 1) / 2 + (b - a - 1); }
+
+// Sturm count: number of eigenvalues of the real symmetric tridiagonal (d, e2=|e|^2) below x.
+__device__ __forceinline__ int sturm_count(const double* d, const double* e2, int n, double x, double pivmin) {
+  double q = d[0] - x;
+  int c = q < 0.0;
+  for (int i = 1; i < n; ++i) {
+    if (fabs(q) < pivmin) q = -pivmin;
+    q = (d[i] - x) - e2[i - 1] / q;
+    c += q < 0.0;
+  }
+  return c;
+}
+
+// Smallest eigenpair of the real symmetric tridiagonal (d, ae) -> lambda, y (written to y[]).
+// Executed by one lane; all arrays are in this wave's LDS.
+__device__ void tridiag_min_eigpair(const double* d, const double* ae, const double* e2, double* y, double* wk,
+                                    int n, double lambda, double scale) {
+  // inverse iteration with a partially pivoted LU of (T - lambda I), two sweeps from y = 1.
+  double* u0 = wk;          // U diagonal
+  double* u1 = wk + n;      // U first super-diagonal
+  double* u2 = wk + 2 * n;  // U second super-diagonal (fill-in from pivoting)
+  double* lm = wk + 3 * n;  // multipliers
+  int* sw = reinterpret_cast<int*>(wk + 4 * n);  // row swapped at step i
+  const double tiny = scale * 2.2e-16 + 1e-300;
+  // factor
+  double a = d[0] - lambda, b = n > 1 ? ae[0] : 0.0, cc = 0.0;  // current row i: [a b cc]
+  for (int i = 0; i < n - 1; ++i) {
+    const double sub = ae[i];                 // T[i+1][i]
+    const double nd = d[i + 1] - lambda;      // T[i+1][i+1]
+    const double nsup = i + 1 < n - 1 ? ae[i + 1] : 0.0;
+    if (fabs(a) >= fabs(sub)) {
+      const double l = (a != 0.0) ? sub / a : 0.0;
+      u0[i] = a;
+      u1[i] = b;
+      u2[i] = cc;
+      lm[i] = l;
+      sw[i] = 0;
+      a = nd - l * b;
+      b = nsup - l * cc;
+      cc = 0.0;
+    } else {
+      const double l = a / sub;
+      u0[i] = sub;
+      u1[i] = nd;
+      u2[i] = nsup;
+      lm[i] = l;
+      sw[i] = 1;
+      const double na = b - l * nd;
+      const double nb = cc - l * nsup;
+      a = na;
+      b = nb;
+      cc = 0.0;
+    }
+  }
+  u0[n - 1] = a;
+  u1[n - 1] = 0.0;
+  u2[n - 1] = 0.0;
+  for (int i = 0; i < n; ++i) {
+    if (fabs(u0[i]) < tiny) u0[i] = copysign(tiny, u0[i] == 0.0 ? 1.0 : u0[i]);
+    y[i] = 1.0;
+  }
+  for (int it = 0; it < 3; ++it) {
+    // forward: apply row swaps and multipliers
+    for (int i = 0; i < n - 1; ++i) {
+      if (sw[i]) {
+        const double t = y[i];
+        y[i] = y[i + 1];
+        y[i + 1] = t;
+      }
+      y[i + 1] -= lm[i] * y[i];
+    }
+    // backward
+    for (int i = n - 1; i >= 0; --i) {
+      double s = y[i];
+      if (i + 1 < n) s -= u1[i] * y[i + 1];
+      if (i + 2 < n) s -= u2[i] * y[i + 2];
+      y[i] = s / u0[i];
+    }
+    double nrm = 0.0;
+    for (int i = 0; i < n; ++i) nrm += y[i] * y[i];
+    nrm = 1.0 / sqrt(nrm);
+    for (int i = 0; i < n; ++i) y[i] *= nrm;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// fast path: one wave per point, NM >= n, n <= 64
+// ----------------------------------------------------------------------------
+constexpr int EMI_FAST_THREADS = 64;
+
+template <int NM>
+__global__ __launch_bounds__(EMI_FAST_THREADS) void emi_fast_kernel(const float2* __restrict__ coh, long long npts,
+                                                                    int n, int ref, float2* __restrict__ ph,
+                                                                    float* __restrict__ quality) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int P = n * (n - 1) / 2;
+  const int lane0 = threadIdx.x;
+  float2* cs = reinterpret_cast<float2*>(smem);                          // staged coh, P entries
+  // per-lane broadcast buffers hold 64 entries (every lane writes its slot)
+  double* colb = reinterpret_cast<double*>(smem + ((P * 8 + 15) & ~15));  // 64 doubles (sweep broadcast)
+  float2* ub = reinterpret_cast<float2*>(colb + 64);                      // 64 (reflector broadcast)
+  float2* qb = ub + 64;                                                   // 64
+  double* td = reinterpret_cast<double*>(qb + 64);                        // NM  T diagonal
+  double* tae = td + NM;                                                  // NM  |sub-diagonal|
+  double* te2 = tae + NM;                                                 // NM  |sub-diagonal|^2
+  float2* dph = reinterpret_cast<float2*>(te2 + NM);                      // NM  phase scaling D
+  double* ty = reinterpret_cast<double*>(dph + NM);                       // NM  eigenvector of T
+  double* wk = ty + NM;                                                   // 5*NM work (LU)
+  float2* hal = reinterpret_cast<float2*>(wk + 5 * NM);                   // NM  Householder alpha_k
+  float* hinv = reinterpret_cast<float*>(hal + NM);                       // NM  Householder 1/den_k
+
+  for (long long pt = blockIdx.x; pt < npts; pt += gridDim.x) {
+    // Opaque per-point copy of the lane id: keeps the compiler from hoisting
+    // NM per-lane gather addresses out of the point loop (they would stay live
+    // in VGPRs for the whole kernel).
+    int lane = lane0;
+    __asm__ volatile("" : "+v"(lane));
+    const float2* src = coh + pt * (long long)P;
+    for (int t = lane; t < P; t += 64) cs[t] = src[t];
+    __builtin_amdgcn_wave_barrier();
+
+    // ---- step 1: A = |Gamma| (row per lane, FP64), sweep inverse ----------
+    double a[NM];
+#pragma unroll
+    for (int j = 0; j < NM; ++j) {
+      double v = 0.0;
+      if (lane < n && j < n) {
+        if (j == lane) {
+          v = 1.0;
+        } else {
+          const int idx = j > lane ? pair_index(lane, j, n) : pair_index(j, lane, n);
+          const float2 g = cs[idx];
+          v = sqrt((double)g.x * g.x + (double)g.y * g.y);
+        }
+      }
+      a[j] = v;
+      if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+    }
+    double dg = lane < n ? 1.0 : 0.0;  // own diagonal entry, tracked through the sweep
+    bool swept = lane >= n;
+    for (int s = 0; s < n; ++s) {
+      const int k = wave_argmax(swept ? -1.0 : fabs(dg), lane);
+      const double c = reg_get<double, NM>(a, k);
+      // Broadcasting d-1 in place of the pivot d lets the rank-1 update itself
+      // write column k:  c - (c/d)(d-1) = c/d.  The pivot lane's own diagonal
+      // comes out as 2 - 1/d instead of -1/d; that entry is never read again
+      // by the sweep, so the +2 offset is removed once, when M is formed.
+      colb[lane] = (lane == k) ? c - 1.0 : c;
+      __builtin_amdgcn_wave_barrier();
+      const double dk = __shfl(c, k, 64);
+      const double rd = 1.0 / dk;
+      const double f = (lane == k) ? (1.0 - rd) : c * rd;
+#pragma unroll
+      for (int j = 0; j < NM; ++j) {
+        a[j] = fma(-f, colb[j], a[j]);
+        if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);  // bound live broadcast registers
+      }
+      dg = fma(-f, c, dg);
+      swept = swept || (lane == k);
+      __builtin_amdgcn_wave_barrier();
+    }
+    // a now holds -inv(A) row `lane` (diagonal entry offset by +2)
+
+    // ---- step 2: M = inv(A) o Gamma (FP32 complex row per lane) ------------
+    float mr_[NM], mi_[NM];
+#pragma unroll
+    for (int j = 0; j < NM; ++j) {
+      float re = 0.f, im = 0.f;
+      if (lane < n && j < n) {
+        const double ai = (j == lane) ? 2.0 - a[j] : -a[j];  // undo the sweep's +2 on the diagonal
+        if (j == lane) {
+          re = (float)ai;
+        } else if (j > lane) {
+          const float2 g = cs[pair_index(lane, j, n)];
+          re = (float)(ai * g.x);
+          im = (float)(ai * g.y);
+        } else {
+          const float2 g = cs[pair_index(j, lane, n)];
+          re = (float)(ai * g.x);
+          im = (float)(-ai * g.y);
+        }
+      }
+      mr_[j] = re;
+      mi_[j] = im;
+      if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+    }
+
+    // ---- step 3: Householder tridiagonalisation -----------------------------
+    float2 dcur = make_float2(1.f, 0.f);  // running phase product D_k
+    if (lane == 0) dph[0] = dcur;
+    for (int k = 0; k + 2 < n; ++k) {
+      const float xr_raw = reg_get<float, NM>(mr_, k);
+      const float xi_raw = reg_get<float, NM>(mi_, k);
+      const bool below = lane > k && lane < n;
+      const float xr = below ? xr_raw : 0.f, xi = below ? xi_raw : 0.f;
+      const float nrm2 = wave_sum(xr * xr + xi * xi);
+      const float x0r = __shfl(xr, k + 1, 64), x0i = __shfl(xi, k + 1, 64);
+      const float dkk = __shfl(xr_raw, k, 64);
+      const float nx = sqrtf(nrm2);
+      const float ax0 = sqrtf(x0r * x0r + x0i * x0i);
+      const float phr = ax0 > 0.f ? x0r / ax0 : 1.f, phi = ax0 > 0.f ? x0i / ax0 : 0.f;
+      const float alr = -phr * nx, ali = -phi * nx;  // alpha = -phase * ||x||
+      const float den = sqrtf(2.f * nx * (nx + ax0));
+      const float inv = den > 0.f ? 1.f / den : 0.f;
+      float ur = (lane == k + 1) ? (x0r - alr) : xr;
+      float ui = (lane == k + 1) ? (x0i - ali) : xi;
+      ur *= inv;
+      ui *= inv;
+      // column k keeps x (rows > k are never updated again); the reflector is
+      // rebuilt from x, alpha and 1/den during the back-transformation
+      if (lane == 0) {
+        hal[k] = make_float2(alr, ali);
+        hinv[k] = inv;
+        td[k] = dkk;
+        tae[k] = nx;
+        te2[k] = (double)nx * nx;
+        // D_{k+1} = D_k * alpha/|alpha|
+        const float sr = nx > 0.f ? alr / nx : 1.f, si = nx > 0.f ? ali / nx : 0.f;
+        dcur = make_float2(dcur.x * sr - dcur.y * si, dcur.x * si + dcur.y * sr);
+        dph[k + 1] = dcur;
+      }
+      ub[lane] = make_float2(ur, ui);
+      __builtin_amdgcn_wave_barrier();
+      // p = M22 u
+      float pr = 0.f, pi = 0.f;
+#pragma unroll
+      for (int j = 0; j < NM; ++j) {
+        if ((j | 7) > k) {  // skip 8-column groups left of the active block (uniform)
+          const float2 uj = ub[j];
+          pr = fmaf(mr_[j], uj.x, pr);
+          pr = fmaf(-mi_[j], uj.y, pr);
+          pi = fmaf(mr_[j], uj.y, pi);
+          pi = fmaf(mi_[j], uj.x, pi);
+        }
+        if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+      }
+      const float cval = wave_sum(ur * pr + ui * pi);  // real(u^H p)
+      const float qr = below ? 2.f * (pr - cval * ur) : 0.f;
+      const float qi = below ? 2.f * (pi - cval * ui) : 0.f;
+      qb[lane] = make_float2(qr, qi);
+      __builtin_amdgcn_wave_barrier();
+      // M22 -= u q^H + q u^H
+#pragma unroll
+      for (int j = 0; j < NM; ++j) {
+        if ((j | 7) > k) {
+          const float2 uj = ub[j];
+          const float2 qj = qb[j];
+          float r = mr_[j], i = mi_[j];
+          r = fmaf(-ur, qj.x, r);
+          r = fmaf(-ui, qj.y, r);
+          i = fmaf(ur, qj.y, i);
+          i = fmaf(-ui, qj.x, i);
+          r = fmaf(-qr, uj.x, r);
+          r = fmaf(-qi, uj.y, r);
+          i = fmaf(qr, uj.y, i);
+          i = fmaf(-qi, uj.x, i);
+          mr_[j] = r;
+          mi_[j] = i;
+        }
+        if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+    {
+      // trailing 2x2 block
+      const float r0 = reg_get<float, NM>(mr_, n - 2);
+      const float i0 = reg_get<float, NM>(mi_, n - 2);
+      const float r1 = reg_get<float, NM>(mr_, n - 1);
+      const float dm2 = __shfl(r0, n - 2, 64);
+      const float dm1 = __shfl(r1, n - 1, 64);
+      const float er = __shfl(r0, n - 1, 64), ei = __shfl(i0, n - 1, 64);
+      if (lane == 0) {
+        const float ae = sqrtf(er * er + ei * ei);
+        td[n - 2] = dm2;
+        td[n - 1] = dm1;
+        tae[n - 2] = ae;
+        te2[n - 2] = (double)ae * ae;
+        const float sr = ae > 0.f ? er / ae : 1.f, si = ae > 0.f ? ei / ae : 0.f;
+        dph[n - 1] = make_float2(dcur.x * sr - dcur.y * si, dcur.x * si + dcur.y * sr);
+        tae[n - 1] = 0.0;
+        te2[n - 1] = 0.0;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+
+    // ---- step 4a: smallest eigenvalue of T by multisection ------------------
+    double lo, hi, scale;
+    {
+      double gl = 0.0, gh = 0.0, sc = 0.0;
+      if (lane < n) {
+        const double off = (lane > 0 ? tae[lane - 1] : 0.0) + tae[lane];
+        gl = td[lane] - off;
+        gh = td[lane] + off;
+        sc = fabs(td[lane]) + off;
+      } else {
+        gl = 1e308;
+        gh = -1e308;
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        gl = fmin(gl, __shfl_xor(gl, o, 64));
+        gh = fmax(gh, __shfl_xor(gh, o, 64));
+        sc = fmax(sc, __shfl_xor(sc, o, 64));
+      }
+      scale = sc > 0.0 ? sc : 1.0;
+      lo = gl - 1e-14 * scale;
+      hi = gh + 1e-14 * scale;
+    }
+    const double pivmin = 1e-300 + scale * scale * 1e-290;
+    for (int round = 0; round < 8; ++round) {
+      if (hi - lo <= 2e-15 * scale) break;
+      const double x = lo + (hi - lo) * (double)(lane + 1) / 65.0;
+      const int cnt = sturm_count(td, te2, n, x, pivmin);
+      const unsigned long long m = __ballot(cnt >= 1);
+      if (m == 0ull) {
+        lo = __shfl(x, 63, 64);
+      } else {
+        const int L = __ffsll((long long)m) - 1;
+        const double xl = __shfl(x, L, 64);
+        const double xp = __shfl(x, L > 0 ? L - 1 : 0, 64);
+        hi = xl;
+        if (L > 0) lo = xp;
+      }
+    }
+    const double lam = 0.5 * (lo + hi);
+
+    // ---- step 4b: eigenvector of T, back-transformation ----------------------
+    if (lane == 0) tridiag_min_eigpair(td, tae, te2, ty, wk, n, lam, scale);
+    __builtin_amdgcn_wave_barrier();
+    // Rayleigh quotient of T for the reported eigenvalue
+    double rq;
+    {
+      double t = 0.0;
+      if (lane < n) {
+        const double yi = ty[lane];
+        double ty_ = td[lane] * yi;
+        if (lane > 0) ty_ += tae[lane - 1] * ty[lane - 1];
+        if (lane + 1 < n) ty_ += tae[lane] * ty[lane + 1];
+        t = yi * ty_;
+      }
+      rq = wave_sum(t);
+    }
+    float vr = 0.f, vi = 0.f;
+    if (lane < n) {
+      const float2 dd = dph[lane];
+      const float yi = (float)ty[lane];
+      vr = dd.x * yi;
+      vi = dd.y * yi;
+    }
+    for (int k = n - 3; k >= 0; --k) {
+      const float xr = reg_get<float, NM>(mr_, k);
+      const float xi = reg_get<float, NM>(mi_, k);
+      const bool below = lane > k && lane < n;
+      const float2 al = hal[k];
+      const float hv = hinv[k];
+      const float ar = below ? ((lane == k + 1) ? xr - al.x : xr) * hv : 0.f;
+      const float ai = below ? ((lane == k + 1) ? xi - al.y : xi) * hv : 0.f;
+      // s = u^H v
+      const float sr = wave_sum(ar * vr + ai * vi);
+      const float si = wave_sum(ar * vi - ai * vr);
+      vr -= 2.f * (ar * sr - ai * si);
+      vi -= 2.f * (ar * si + ai * sr);
+    }
+    // ---- output: ph = v conj(v_ref) / |.|, quality --------------------------
+    const float rr = __shfl(vr, ref, 64), ri = __shfl(vi, ref, 64);
+    if (lane < n) {
+      const float pr = vr * rr + vi * ri;
+      const float pim = vi * rr - vr * ri;
+      const float am = sqrtf(pr * pr + pim * pim);
+      ph[pt * n + lane] = make_float2(pr / am, pim / am);
+    }
+    if (lane == 0) quality[pt] = (float)rq;
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// ----------------------------------------------------------------------------
+// general path: one 256-thread block per point, FP64 in a global workspace
+// ----------------------------------------------------------------------------
+constexpr int EMI_GEN_THREADS = 256;
+constexpr int EMI_GEN_BLOCKS = 1024;
+
+__host__ __device__ inline size_t emi_gen_slice_bytes(int n) {
+  // A (n*n f64) + M (n*n c128) + vectors (col, u, q, p: n c128 each) + 12 n-double arrays
+  // (dg, td, tae, te2, ty, dphr, dphi, 5n LU work) + n ints (swept)
+  return ((size_t)n * n * 8 + (size_t)n * n * 16 + (size_t)n * 16 * 4 + (size_t)n * 8 * 13 + 255) & ~(size_t)255;
+}
+
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  v = wave_sum(v);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) red[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += red[i];
+  return s;
+}
+
+__global__ __launch_bounds__(EMI_GEN_THREADS) void emi_general_kernel(const float2* __restrict__ coh,
+                                                                      long long npts, int n, int ref,
+                                                                      float2* __restrict__ ph,
+                                                                      float* __restrict__ quality,
+                                                                      unsigned char* __restrict__ ws) {
+  __shared__ double red[EMI_GEN_THREADS / 64];
+  __shared__ double sh_key[EMI_GEN_THREADS];
+  __shared__ int sh_idx[EMI_GEN_THREADS];
+  const int P = n * (n - 1) / 2;
+  const int tid = threadIdx.x;
+  unsigned char* base = ws + (size_t)blockIdx.x * emi_gen_slice_bytes(n);
+  double* A = reinterpret_cast<double*>(base);
+  c128* M = reinterpret_cast<c128*>(A + (size_t)n * n);
+  c128* col = M + (size_t)n * n;  // n
+  c128* u = col + n;
+  c128* q = u + n;
+  c128* pv = q + n;
+  double* dg = reinterpret_cast<double*>(pv + n);
+  double* td = dg + n;
+  double* tae = td + n;
+  double* te2 = tae + n;
+  double* ty = te2 + n;
+  double* dphr = ty + n;
+  double* dphi = dphr + n;
+  double* wk = dphi + n;  // 5n
+  int* swept = reinterpret_cast<int*>(wk + 5 * n);
+
+  for (long long pt = blockIdx.x; pt < npts; pt += gridDim.x) {
+    const float2* src = coh + pt * (long long)P;
+    for (int t = tid; t < n * n; t += EMI_GEN_THREADS) {
+      const int i = t / n, j = t - (t / n) * n;
+      c128 g;
+      if (i == j) {
+        g = {1.0, 0.0};
+      } else if (j > i) {
+        const float2 c = src[pair_index(i, j, n)];
+        g = {c.x, c.y};
+      } else {
+        const float2 c = src[pair_index(j, i, n)];
+        g = {c.x, -c.y};
+      }
+      M[t] = g;
+      A[t] = sqrt(g.re * g.re + g.im * g.im);
+    }
+    for (int i = tid; i < n; i += EMI_GEN_THREADS) {
+      dg[i] = 1.0;
+      swept[i] = 0;
+    }
+    __syncthreads();
+    // sweep inverse
+    for (int s = 0; s < n; ++s) {
+      double key = -1.0;
+      int idx = 0x7fffffff;
+      for (int i = tid; i < n; i += EMI_GEN_THREADS)
+        if (!swept[i] && (fabs(dg[i]) > key || (fabs(dg[i]) == key && i < idx))) {
+          key = fabs(dg[i]);
+          idx = i;
+        }
+      sh_key[tid] = key;
+      sh_idx[tid] = idx;
+      __syncthreads();
+      for (int st = EMI_GEN_THREADS / 2; st > 0; st >>= 1) {
+        if (tid < st) {
+          const double ok = sh_key[tid + st];
+          const int oi = sh_idx[tid + st];
+          if (ok > sh_key[tid] || (ok == sh_key[tid] && oi < sh_idx[tid])) {
+            sh_key[tid] = ok;
+            sh_idx[tid] = oi;
+          }
+        }
+        __syncthreads();
+      }
+      const int k = sh_idx[0];
+      for (int i = tid; i < n; i += EMI_GEN_THREADS) col[i].re = A[(size_t)i * n + k];
+      __syncthreads();
+      const double rd = 1.0 / col[k].re;
+      for (int t = tid; t < n * n; t += EMI_GEN_THREADS) {
+        const int i = t / n, j = t - (t / n) * n;
+        const double f = (i == k) ? (1.0 - rd) : col[i].re * rd;
+        A[t] = fma(-f, col[j].re, A[t]);
+      }
+      __syncthreads();
+      for (int i = tid; i < n; i += EMI_GEN_THREADS) {
+        A[(size_t)i * n + k] = (i == k) ? -rd : col[i].re * rd;
+        const double f = (i == k) ? (1.0 - rd) : col[i].re * rd;
+        dg[i] = fma(-f, col[i].re, dg[i]);
+        if (i == k) swept[i] = 1;
+      }
+      __syncthreads();
+    }
+    // M = inv(A) o Gamma  (A holds -inv)
+    for (int t = tid; t < n * n; t += EMI_GEN_THREADS) {
+      const double ai = -A[t];
+      M[t] = {ai * M[t].re, ai * M[t].im};
+    }
+    __syncthreads();
+    // Householder tridiagonalisation (FP64); reflector k kept in column k below the diagonal
+    double dcr = 1.0, dci = 0.0;
+    if (tid == 0) {
+      dphr[0] = 1.0;
+      dphi[0] = 0.0;
+    }
+    for (int k = 0; k + 2 < n; ++k) {
+      double part = 0.0;
+      for (int i = k + 1 + tid; i < n; i += EMI_GEN_THREADS) {
+        const c128 x = M[(size_t)i * n + k];
+        part += x.re * x.re + x.im * x.im;
+      }
+      const double nrm2 = block_sum(part, red);
+      const c128 x0 = M[(size_t)(k + 1) * n + k];
+      const double nx = sqrt(nrm2);
+      const double ax0 = sqrt(x0.re * x0.re + x0.im * x0.im);
+      const double phr = ax0 > 0 ? x0.re / ax0 : 1.0, phi = ax0 > 0 ? x0.im / ax0 : 0.0;
+      const double alr = -phr * nx, ali = -phi * nx;
+      const double den = sqrt(2.0 * nx * (nx + ax0));
+      const double inv = den > 0 ? 1.0 / den : 0.0;
+      const double dkk = M[(size_t)k * n + k].re;
+      __syncthreads();
+      for (int i = tid; i < n; i += EMI_GEN_THREADS) {
+        c128 v = {0.0, 0.0};
+        if (i > k) {
+          const c128 x = M[(size_t)i * n + k];
+          v = (i == k + 1) ? c128{(x.re - alr) * inv, (x.im - ali) * inv} : c128{x.re * inv, x.im * inv};
+        }
+        u[i] = v;
+      }
+      __syncthreads();
+      for (int i = tid; i < n; i += EMI_GEN_THREADS) {
+        c128 p = {0.0, 0.0};
+        if (i > k)
+          for (int j = k + 1; j < n; ++j) {
+            const c128 m = M[(size_t)i * n + j];
+            p.re += m.re * u[j].re - m.im * u[j].im;
+            p.im += m.re * u[j].im + m.im * u[j].re;
+          }
+        pv[i] = p;
+      }
+      __syncthreads();
+      double cp = 0.0;
+      for (int i = tid; i < n; i += EMI_GEN_THREADS) cp += u[i].re * pv[i].re + u[i].im * pv[i].im;
+      const double cval = block_sum(cp, red);
+      for (int i = tid; i < n; i += EMI_GEN_THREADS)
+        q[i] = (i > k) ? c128{2.0 * (pv[i].re - cval * u[i].re), 2.0 * (pv[i].im - cval * u[i].im)} : c128{0.0, 0.0};
+      __syncthreads();
+      const int m = n - k - 1;
+      for (int t = tid; t < m * m; t += EMI_GEN_THREADS) {
+        const int i = k + 1 + t / m, j = k + 1 + (t - (t / m) * m);
+        c128 v = M[(size_t)i * n + j];
+        // -= u_i conj(q_j) + q_i conj(u_j)
+        v.re -= u[i].re * q[j].re + u[i].im * q[j].im + q[i].re * u[j].re + q[i].im * u[j].im;
+        v.im -= u[i].im * q[j].re - u[i].re * q[j].im + q[i].im * u[j].re - q[i].re * u[j].im;
+        M[(size_t)i * n + j] = v;
+      }
+      for (int i = tid; i < n; i += EMI_GEN_THREADS)
+        if (i > k) M[(size_t)i * n + k] = u[i];
+      if (tid == 0) {
+        td[k] = dkk;
+        tae[k] = nx;
+        te2[k] = nx * nx;
+        const double sr = nx > 0 ? alr / nx : 1.0, si = nx > 0 ? ali / nx : 0.0;
+        const double r = dcr * sr - dci * si, im = dcr * si + dci * sr;
+        dcr = r;
+        dci = im;
+        dphr[k + 1] = dcr;
+        dphi[k + 1] = dci;
+      }
+      __syncthreads();
+    }
+    if (tid == 0) {
+      const c128 e = M[(size_t)(n - 1) * n + (n - 2)];
+      const double ae = sqrt(e.re * e.re + e.im * e.im);
+      td[n - 2] = M[(size_t)(n - 2) * n + (n - 2)].re;
+      td[n - 1] = M[(size_t)(n - 1) * n + (n - 1)].re;
+      tae[n - 2] = ae;
+      te2[n - 2] = ae * ae;
+      tae[n - 1] = 0.0;
+      te2[n - 1] = 0.0;
+      const double sr = ae > 0 ? e.re / ae : 1.0, si = ae > 0 ? e.im / ae : 0.0;
+      dphr[n - 1] = dcr * sr - dci * si;
+      dphi[n - 1] = dcr * si + dci * sr;
+      // bisection (serial, FP64) + inverse iteration
+      double lo = 1e308, hi = -1e308, sc = 0.0;
+      for (int i = 0; i < n; ++i) {
+        const double off = (i > 0 ? tae[i - 1] : 0.0) + tae[i];
+        lo = fmin(lo, td[i] - off);
+        hi = fmax(hi, td[i] + off);
+        sc = fmax(sc, fabs(td[i]) + off);
+      }
+      const double scale = sc > 0 ? sc : 1.0;
+      lo -= 1e-14 * scale;
+      hi += 1e-14 * scale;
+      const double pivmin = 1e-300 + scale * scale * 1e-290;
+      for (int it = 0; it < 200 && hi - lo > 2e-15 * scale; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (mid <= lo || mid >= hi) break;
+        if (sturm_count(td, te2, n, mid, pivmin) >= 1)
+          hi = mid;
+        else
+          lo = mid;
+      }
+      const double lam = 0.5 * (lo + hi);
+      tridiag_min_eigpair(td, tae, te2, ty, wk, n, lam, scale);
+      double rq = 0.0;
+      for (int i = 0; i < n; ++i) {
+        double t = td[i] * ty[i];
+        if (i > 0) t += tae[i - 1] * ty[i - 1];
+        if (i + 1 < n) t += tae[i] * ty[i + 1];
+        rq += ty[i] * t;
+      }
+      quality[pt] = (float)rq;
+    }
+    __syncthreads();
+    // back-transformation v = H_0 ... H_{n-3} D y
+    for (int i = tid; i < n; i += EMI_GEN_THREADS) pv[i] = {dphr[i] * ty[i], dphi[i] * ty[i]};
+    __syncthreads();
+    for (int k = n - 3; k >= 0; --k) {
+      double sr = 0.0, si = 0.0;
+      for (int i = k + 1 + tid; i < n; i += EMI_GEN_THREADS) {
+        const c128 uu = M[(size_t)i * n + k];
+        sr += uu.re * pv[i].re + uu.im * pv[i].im;
+        si += uu.re * pv[i].im - uu.im * pv[i].re;
+      }
+      sr = block_sum(sr, red);
+      si = block_sum(si, red);
+      for (int i = k + 1 + tid; i < n; i += EMI_GEN_THREADS) {
+        const c128 uu = M[(size_t)i * n + k];
+        pv[i].re -= 2.0 * (uu.re * sr - uu.im * si);
+        pv[i].im -= 2.0 * (uu.re * si + uu.im * sr);
+      }
+      __syncthreads();
+    }
+    const c128 vr = pv[ref];
+    for (int i = tid; i < n; i += EMI_GEN_THREADS) {
+      const double pr = pv[i].re * vr.re + pv[i].im * vr.im;
+      const double pim = pv[i].im * vr.re - pv[i].re * vr.im;
+      const double am = sqrt(pr * pr + pim * pim);
+      ph[pt * n + i] = make_float2((float)(pr / am), (float)(pim / am));
+    }
+    __syncthreads();
+  }
+}
+
+static size_t emi_fast_lds(int n, int nm) {
+  const int P = n * (n - 1) / 2;
+  return (((size_t)P * 8 + 15) & ~(size_t)15) + 64 * (8 + 8 + 8) + (size_t)nm * (8 * 3 + 8 + 8 + 8 * 5 + 8 + 4) + 64;
+}
+
+}  // namespace mr
+
+extern "C" {
+
+size_t mr_emi_workspace_bytes(long long n_pts, int n) {
+  if (n <= 64) return 0;
+  const long long nb = n_pts < mr::EMI_GEN_BLOCKS ? n_pts : mr::EMI_GEN_BLOCKS;
+  return (size_t)(nb > 0 ? nb : 1) * mr::emi_gen_slice_bytes(n);
+}
+
+int mr_emi(const float* coh, long long n_pts, int n, int ref, float* ph, float* quality, void* workspace,
+           size_t workspace_bytes, mr_stream_t stream) {
+  MR_CHECK_ARG(n >= 2, "emi: need at least 2 images (n=%d)", n);
+  MR_CHECK_ARG(ref >= 0 && ref < n, "emi: ref=%d outside [0,%d)", ref, n);
+  MR_CHECK_ARG(n_pts >= 0, "emi: negative point count");
+  if (n_pts == 0) return MR_OK;
+  MR_CHECK_ARG(coh && ph && quality, "emi: null pointer");
+  hipStream_t st = mr::as_stream(stream);
+  const float2* c = reinterpret_cast<const float2*>(coh);
+  float2* p = reinterpret_cast<float2*>(ph);
+  if (n <= 64) {
+    const int nm = n <= 32 ? 32 : 64;
+    const size_t lds = mr::emi_fast_lds(n, nm);
+    const int nb = (int)std::min<long long>(n_pts, 1 << 20);
+    if (nm == 32)
+      hipLaunchKernelGGL((mr::emi_fast_kernel<32>), dim3(nb), dim3(mr::EMI_FAST_THREADS), lds, st, c, n_pts, n, ref,
+                         p, quality);
+    else
+      hipLaunchKernelGGL((mr::emi_fast_kernel<64>), dim3(nb), dim3(mr::EMI_FAST_THREADS), lds, st, c, n_pts, n, ref,
+                         p, quality);
+    return mr::launch_status("emi_fast");
+  }
+  const size_t need = mr_emi_workspace_bytes(n_pts, n);
+  MR_CHECK_ARG(workspace != nullptr && workspace_bytes >= need, "emi: workspace %zu < %zu bytes", workspace_bytes,
+               need);
+  const int nb = (int)std::min<long long>(n_pts, mr::EMI_GEN_BLOCKS);
+  hipLaunchKernelGGL(mr::emi_general_kernel, dim3(nb), dim3(mr::EMI_GEN_THREADS), 0, st, c, n_pts, n, ref, p,
+                     quality, reinterpret_cast<unsigned char*>(workspace));
+  return mr::launch_status("emi_general");
+}
+
+}  // extern "C"

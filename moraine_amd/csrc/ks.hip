@@ -1,0 +1,385 @@
+// KS test + fused SHP selection for gfx950.
+//
+// Replaces moraine.ks_test (moraine/shp.py:269-305) and moraine.select_shp
+// (moraine/shp.py:308-323).  Design:
+//   * ks_sort_kernel: one wave per pixel stack; values become order-preserving
+//     unsigned keys (-0 folded onto +0 so key order == IEEE '<=' order for
+//     non-NaN values), bitonic-sorted in registers with cross-lane shuffles,
+//     written once to a workspace with a per-pixel "has NaN" flag (the
+//     reference's "last sorted element is NaN" test).
+//   * ks_pairs_kernel: one 256-thread block per (azimuth line, column tile).
+//     The tile's reference stacks stay in LDS; for each window row offset the
+//     secondary line segment (tile + range halo) is staged in LDS and every
+//     (pixel, range offset) pair runs the merge walk of moraine/shp.py:35-44
+//     on integer keys.  The walk yields the integer step count k = N*D; p,
+//     dist and the SHP bit are table lookups of k, so they are bit-exact with
+//     the reference CPU path (p = dtype(f64 series), dist = dtype(k/N)).
+//     Several walks are interleaved per thread (ILP) to hide LDS latency.
+#include "common.hpp"
+
+namespace mr {
+
+template <typename T>
+struct KsTables;
+template <>
+struct KsTables<float> {
+  float p[MR_KS_MAX_N + 1];
+  float d[MR_KS_MAX_N + 1];
+  uint32_t shp[(MR_KS_MAX_N + 32) / 32];
+};
+constexpr int KS_MAX_N_F64 = 224;  // keeps the f64 table inside the 4 KiB kernel-argument limit
+template <>
+struct KsTables<double> {
+  double p[KS_MAX_N_F64 + 1];
+  double d[KS_MAX_N_F64 + 1];
+  uint32_t shp[(KS_MAX_N_F64 + 32) / 32];
+};
+
+// Series of moraine/shp.py:19-32 evaluated in the same operation order (f64,
+// glibc exp) so that the stored dtype(p) matches the reference bit for bit.
+#pragma clang fp contract(off)
+static double ks_p_series(double x) {
+  double x2 = -2 * x * x;
+  double p = 0.0, p2 = 0.0;
+  int sign = 1;
+  for (int i = 1; i <= 100; i++) {
+    p += sign * 2 * std::exp(x2 * i * i);
+    if (p == p2) return p;
+    sign = -sign;
+    p2 = p;
+  }
+  return p;
+}
+
+template <typename T>
+static void build_tables(int n, double p_max, KsTables<T>& t) {
+  const double en = std::sqrt(n / 2.0);  // moraine/shp.py:76  en = sqrt(n_imag/2)
+  for (int w = 0; w < (int)(sizeof(t.shp) / 4); ++w) t.shp[w] = 0;
+  for (int k = 0; k <= n; ++k) {
+    const double dmax = std::fabs((double)k / (double)n);  // abs((j2-j1)/n_imag)
+    const T pk = (T)ks_p_series((en + 0.12 + 0.11 / en) * dmax);
+    t.p[k] = pk;
+    t.d[k] = (T)dmax;
+    if ((double)pk < p_max) t.shp[k >> 5] |= 1u << (k & 31);  // select_shp: p < p_max
+  }
+}
+#pragma clang fp contract(on)
+
+template <typename T>
+struct KeyOf;
+template <>
+struct KeyOf<float> {
+  using type = uint32_t;
+  __device__ static uint32_t key(float x) {
+    uint32_t b = __float_as_uint(x);
+    if (b == 0x80000000u) b = 0u;  // -0 == +0 under the reference's float compares
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+  }
+};
+template <>
+struct KeyOf<double> {
+  using type = uint64_t;
+  __device__ static uint64_t key(double x) {
+    uint64_t b = (uint64_t)__double_as_longlong(x);
+    if (b == 0x8000000000000000ull) b = 0ull;
+    return (b & 0x8000000000000000ull) ? ~b : (b | 0x8000000000000000ull);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// sort: one wave per pixel, NP = 64*E >= n keys in registers (index e*64+lane)
+// ---------------------------------------------------------------------------
+template <typename T, int E>
+__global__ __launch_bounds__(256) void ks_sort_kernel(const T* __restrict__ rmli, long long npx, int n,
+                                                      typename KeyOf<T>::type* __restrict__ keys,
+                                                      uint8_t* __restrict__ valid) {
+  using Key = typename KeyOf<T>::type;
+  constexpr int NP = 64 * E;
+  const int lane = threadIdx.x & 63;
+  const long long wave0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long long nwaves = ((long long)gridDim.x * blockDim.x) >> 6;
+  for (long long px = wave0; px < npx; px += nwaves) {
+    Key v[E];
+    bool has_nan = false;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int i = e * 64 + lane;
+      if (i < n) {
+        const T x = rmli[px * n + i];
+        has_nan |= (x != x);
+        v[e] = KeyOf<T>::key(x);
+      } else {
+        v[e] = ~(Key)0;
+      }
+    }
+#pragma unroll
+    for (int k = 2; k <= NP; k <<= 1) {
+#pragma unroll
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        Key nv[E];  // all partners are read before any register is overwritten
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const int i = e * 64 + lane;
+          Key pv;
+          if (j >= 64)
+            pv = v[e ^ (j >> 6)];
+          else
+            pv = __shfl_xor(v[e], j, 64);
+          const bool up = (i & k) == 0;
+          const bool lower = (i & j) == 0;
+          const Key lo = v[e] < pv ? v[e] : pv;
+          const Key hi = v[e] < pv ? pv : v[e];
+          nv[e] = (lower == up) ? lo : hi;
+        }
+#pragma unroll
+        for (int e = 0; e < E; ++e) v[e] = nv[e];
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int i = e * 64 + lane;
+      if (i < n) keys[px * n + i] = v[e];
+    }
+    const bool any_nan = __any(has_nan);
+    if (lane == 0) valid[px] = any_nan ? 0 : 1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// pairs
+// ---------------------------------------------------------------------------
+constexpr int KS_THREADS = 256;
+constexpr int KS_ILP = 4;
+
+template <typename Key>
+__device__ __forceinline__ void stage_stacks(Key* __restrict__ dst, const Key* __restrict__ src, int nstk, int n,
+                                             int ns) {
+  const int tot = nstk * n;
+  for (int i = threadIdx.x; i < tot; i += KS_THREADS) {
+    const int s = i / n;
+    const int e = i - s * n;
+    dst[s * ns + e] = src[i];
+  }
+  for (int s = threadIdx.x; s < nstk; s += KS_THREADS) dst[s * ns + n] = ~(Key)0;  // sentinel
+}
+
+template <typename T>
+__global__ __launch_bounds__(KS_THREADS) void ks_pairs_kernel(const typename KeyOf<T>::type* __restrict__ keys,
+                                                              const uint8_t* __restrict__ valid, int nl, int w,
+                                                              int n, int az, int r, int tw, KsTables<T> tab,
+                                                              T* __restrict__ p_out, T* __restrict__ d_out,
+                                                              uint8_t* __restrict__ m_out,
+                                                              int32_t* __restrict__ shp_num) {
+  using Key = typename KeyOf<T>::type;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int ns = n + 1;
+  const int rw = 2 * r + 1, aw = 2 * az + 1;
+  const int a = blockIdx.y;
+  const int c0 = blockIdx.x * tw;
+  const int tcols = min(tw, w - c0);
+  const int scols = tcols + 2 * r;
+  Key* ref = reinterpret_cast<Key*>(smem);
+  Key* sec = ref + tw * ns;
+  int* cnt = reinterpret_cast<int*>(sec + (tw + 2 * r) * ns);
+  uint8_t* refv = reinterpret_cast<uint8_t*>(cnt + tw);
+  uint8_t* secv = refv + tw;
+
+  stage_stacks(ref, keys + ((size_t)a * w + c0) * n, tcols, n, ns);
+  for (int c = threadIdx.x; c < tw; c += KS_THREADS) {
+    refv[c] = c < tcols ? valid[(size_t)a * w + c0 + c] : 0;
+    cnt[c] = 0;
+  }
+  const int npairs = tcols * rw;
+  for (int l = -az; l <= az; ++l) {
+    __syncthreads();
+    const int sa = a + l;
+    const bool row_ok = sa >= 0 && sa < nl;
+    if (row_ok) {
+      const int g0 = max(0, c0 - r);
+      const int g1 = min(w, c0 + tcols + r);
+      const int s0 = g0 - (c0 - r);
+      stage_stacks(sec + s0 * ns, keys + ((size_t)sa * w + g0) * n, g1 - g0, n, ns);
+      for (int s = threadIdx.x; s < scols; s += KS_THREADS) {
+        const int gc = c0 - r + s;
+        secv[s] = (gc >= 0 && gc < w) ? valid[(size_t)sa * w + gc] : 0;
+      }
+    } else {
+      for (int s = threadIdx.x; s < scols; s += KS_THREADS) secv[s] = 0;
+    }
+    __syncthreads();
+    for (int t0 = threadIdx.x; t0 < npairs; t0 += KS_THREADS * KS_ILP) {
+      int ia[KS_ILP], ib[KS_ILP], ra[KS_ILP], rb[KS_ILP], km[KS_ILP], cc[KS_ILP], mm[KS_ILP];
+      bool ok[KS_ILP], in[KS_ILP];
+      Key ka[KS_ILP], kb[KS_ILP];
+#pragma unroll
+      for (int q = 0; q < KS_ILP; ++q) {
+        const int t = t0 + q * KS_THREADS;
+        in[q] = t < npairs;
+        const int c = in[q] ? t / rw : 0;
+        const int m = in[q] ? t - c * rw : 0;
+        cc[q] = c;
+        mm[q] = m;
+        ok[q] = in[q] && refv[c] && secv[c + m];
+        ra[q] = c * ns;
+        rb[q] = ok[q] ? (tw + c + m) * ns : c * ns;  // invalid pairs walk ref vs itself (k=0, discarded)
+        ia[q] = 0;
+        ib[q] = 0;
+        km[q] = 0;
+        ka[q] = ref[ra[q]];
+        kb[q] = ref[rb[q]];
+      }
+      // merge walk of moraine/shp.py:35-44 on keys: '<=' advances ref, '>=' advances sec
+      for (int it = 0; it < 2 * n; it += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+#pragma unroll
+          for (int q = 0; q < KS_ILP; ++q) {
+            const int le = ka[q] <= kb[q];
+            const int ge = ka[q] >= kb[q];
+            ia[q] = min(ia[q] + le, n);
+            ib[q] = min(ib[q] + ge, n);
+            const int d = ib[q] - ia[q];
+            km[q] = max(km[q], max(d, -d));
+            ka[q] = ref[ra[q] + ia[q]];
+            kb[q] = ref[rb[q] + ib[q]];
+          }
+        }
+        bool alive = false;
+#pragma unroll
+        for (int q = 0; q < KS_ILP; ++q) alive |= (ia[q] < n) && (ib[q] < n);
+        if (!__any(alive)) break;
+      }
+#pragma unroll
+      for (int q = 0; q < KS_ILP; ++q) {
+        if (!in[q]) continue;
+        const size_t o = (((size_t)a * w + c0 + cc[q]) * aw + (l + az)) * rw + mm[q];
+        const int k = km[q];
+        if (p_out) p_out[o] = ok[q] ? tab.p[k] : (T)NAN;
+        if (d_out) d_out[o] = ok[q] ? tab.d[k] : (T)NAN;
+        const bool s = ok[q] && ((tab.shp[k >> 5] >> (k & 31)) & 1u);
+        if (m_out) m_out[o] = s ? 1 : 0;
+        if (s) atomicAdd(&cnt[cc[q]], 1);
+      }
+    }
+  }
+  __syncthreads();
+  if (shp_num)
+    for (int c = threadIdx.x; c < tcols; c += KS_THREADS) shp_num[(size_t)a * w + c0 + c] = cnt[c];
+}
+
+// select_shp on an existing p array: one wave per pixel.
+__global__ __launch_bounds__(256) void select_shp_kernel(const float* __restrict__ p, long long npix, int win,
+                                                         double p_max, uint8_t* __restrict__ is_shp,
+                                                         int32_t* __restrict__ shp_num) {
+  const int lane = threadIdx.x & 63;
+  const long long wave0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
+  for (long long px = wave0; px < npix; px += nw) {
+    int c = 0;
+    for (int i = lane; i < win; i += 64) {
+      const float v = p[px * win + i];
+      const bool s = (double)v < p_max;  // NaN -> false
+      if (is_shp) is_shp[px * win + i] = s;
+      c += s;
+    }
+    c = wave_sum(c);
+    if (lane == 0 && shp_num) shp_num[px] = c;
+  }
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+template <typename T>
+static int ks_test_impl(const T* rmli, int nl, int w, int n, int az, int r, double p_max, T* p_out, T* d_out,
+                        uint8_t* m_out, int32_t* shp_num, void* ws, size_t ws_bytes, mr_stream_t stream_) {
+  using Key = typename KeyOf<T>::type;
+  const int maxn = sizeof(T) == 4 ? MR_KS_MAX_N : KS_MAX_N_F64;
+  MR_CHECK_ARG(rmli != nullptr && ws != nullptr, "ks_test: null rmli or workspace");
+  MR_CHECK_ARG(nl > 0 && w > 0, "ks_test: empty raster (%d x %d)", nl, w);
+  if (n < 1 || n > maxn) {
+    set_error("ks_test: n=%d outside 1..%d", n, maxn);
+    return MR_ERR_UNSUPPORTED;
+  }
+  MR_CHECK_ARG(az >= 0 && r >= 0, "ks_test: negative half window");
+  const size_t npx = (size_t)nl * w;
+  const size_t need = align256(npx * n * sizeof(Key)) + align256(npx);
+  MR_CHECK_ARG(ws_bytes >= need, "ks_test: workspace %zu < %zu bytes", ws_bytes, need);
+  hipStream_t st = as_stream(stream_);
+  Key* keys = reinterpret_cast<Key*>(ws);
+  uint8_t* valid = reinterpret_cast<uint8_t*>(ws) + align256(npx * n * sizeof(Key));
+
+  const int nb = (int)std::min<size_t>(div_up(npx, 4), 65536);
+  if (n <= 64)
+    hipLaunchKernelGGL((ks_sort_kernel<T, 1>), dim3(nb), dim3(256), 0, st, rmli, (long long)npx, n, keys, valid);
+  else if (n <= 128)
+    hipLaunchKernelGGL((ks_sort_kernel<T, 2>), dim3(nb), dim3(256), 0, st, rmli, (long long)npx, n, keys, valid);
+  else
+    hipLaunchKernelGGL((ks_sort_kernel<T, 4>), dim3(nb), dim3(256), 0, st, rmli, (long long)npx, n, keys, valid);
+  int rc = launch_status("ks_sort");
+  if (rc) return rc;
+
+  // column tile: as wide as 64 pixels while LDS stays <= 48 KiB (>= 3 blocks/CU)
+  const size_t stk = (size_t)(n + 1) * sizeof(Key);
+  auto lds_for = [&](int tw) { return (2 * (size_t)tw + 2 * r) * stk + (size_t)tw * 4 + 2 * (size_t)tw + 2 * r + 16; };
+  int tw = 64;
+  while (tw > 1 && lds_for(tw) > 48 * 1024) --tw;
+  tw = std::min(tw, w);
+  const size_t lds = lds_for(tw);
+  if (lds > 160 * 1024) {
+    set_error("ks_test: window too large for LDS (n=%d, r_half=%d)", n, r);
+    return MR_ERR_UNSUPPORTED;
+  }
+  KsTables<T> tab;
+  build_tables<T>(n, p_max, tab);
+  dim3 grid(div_up(w, tw), nl);
+  hipLaunchKernelGGL((ks_pairs_kernel<T>), grid, dim3(KS_THREADS), lds, st, keys, valid, nl, w, n, az, r, tw, tab,
+                     p_out, d_out, m_out, shp_num);
+  return launch_status("ks_pairs");
+}
+
+}  // namespace mr
+
+extern "C" {
+
+size_t mr_ks_workspace_bytes(int nl, int w, int n) {
+  const size_t npx = (size_t)nl * w;
+  return mr::align256(npx * n * sizeof(uint64_t)) + mr::align256(npx);
+}
+
+int mr_ks_test_f32(const float* rmli, int nl, int w, int n, int az, int r, double p_max, float* p_out,
+                   float* dist_out, uint8_t* mask_out, int32_t* shp_num, void* workspace, size_t workspace_bytes,
+                   mr_stream_t stream) {
+  return mr::ks_test_impl<float>(rmli, nl, w, n, az, r, p_max, p_out, dist_out, mask_out, shp_num, workspace,
+                                 workspace_bytes, stream);
+}
+
+int mr_ks_test_f64(const double* rmli, int nl, int w, int n, int az, int r, double p_max, double* p_out,
+                   double* dist_out, uint8_t* mask_out, int32_t* shp_num, void* workspace, size_t workspace_bytes,
+                   mr_stream_t stream) {
+  return mr::ks_test_impl<double>(rmli, nl, w, n, az, r, p_max, p_out, dist_out, mask_out, shp_num, workspace,
+                                  workspace_bytes, stream);
+}
+
+int mr_ks_tables_f32(int n, double p_max, float* p, float* dist, uint8_t* shp) {
+  MR_CHECK_ARG(n >= 1 && n <= MR_KS_MAX_N, "ks_tables: n=%d outside 1..%d", n, MR_KS_MAX_N);
+  mr::KsTables<float> t;
+  mr::build_tables<float>(n, p_max, t);
+  for (int k = 0; k <= n; ++k) {
+    if (p) p[k] = t.p[k];
+    if (dist) dist[k] = t.d[k];
+    if (shp) shp[k] = (t.shp[k >> 5] >> (k & 31)) & 1u;
+  }
+  return MR_OK;
+}
+
+int mr_select_shp_f32(const float* p, long long npix, int win, double p_max, uint8_t* is_shp, int32_t* shp_num,
+                      mr_stream_t stream) {
+  MR_CHECK_ARG(p != nullptr && npix >= 0 && win > 0, "select_shp: bad arguments");
+  if (npix == 0) return MR_OK;
+  const int nb = (int)std::min<long long>(mr::div_up(npix, 4), 65536);
+  hipLaunchKernelGGL(mr::select_shp_kernel, dim3(nb), dim3(256), 0, mr::as_stream(stream), p, npix, win, p_max,
+                     is_shp, shp_num);
+  return mr::launch_status("select_shp");
+}
+
+}  // extern "C"

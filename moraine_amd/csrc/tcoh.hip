@@ -1,0 +1,71 @@
+// Temporal coherence for gfx950.
+//
+// mr_ds_temp_coh replaces moraine.ds_temp_coh (moraine/pl.py:126-145; numba
+// :80-100; kernel :103-123):  t = |sum_p coh_p/|coh_p| conj(ph_a) ph_b| / n_pairs.
+// mr_temp_coh replaces moraine.temp_coh (moraine/pqm.py:124-159; numba :46-71;
+// kernel :99-121):  t = |sum_p intf_p conj(r_a) r_b / (|intf_p||r_a||r_b|)| / n_pairs.
+// One wave per point streams the point's pair row (coalesced, the HBM-bound
+// part), gathers the phase terms from an L1-resident N-vector, and reduces with
+// cross-lane shuffles.  Inputs are read only (the numba temp_coh normalises its
+// inputs in place; this one does not).
+#include "common.hpp"
+
+namespace mr {
+
+template <bool DS>
+__global__ __launch_bounds__(256) void tcoh_kernel(const float2* __restrict__ c, const float2* __restrict__ v,
+                                                   long long npts, int n, const int32_t* __restrict__ pairs,
+                                                   int npairs, float* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const long long w0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
+  for (long long pt = w0; pt < npts; pt += nw) {
+    const float2* cp = c + pt * npairs;
+    const float2* vp = v + pt * n;
+    float sr = 0.f, si = 0.f;
+    for (int p = lane; p < npairs; p += 64) {
+      const int a = pairs[2 * p], b = pairs[2 * p + 1];
+      const float2 x = cp[p], va = vp[a], vb = vp[b];
+      // conj(va) * vb
+      float tr = va.x * vb.x + va.y * vb.y;
+      float ti = va.x * vb.y - va.y * vb.x;
+      float sc;
+      if (DS) {
+        sc = 1.f / sqrtf(x.x * x.x + x.y * x.y);
+      } else {
+        sc = 1.f / sqrtf((x.x * x.x + x.y * x.y) * (va.x * va.x + va.y * va.y) * (vb.x * vb.x + vb.y * vb.y));
+      }
+      sr += (x.x * tr - x.y * ti) * sc;
+      si += (x.x * ti + x.y * tr) * sc;
+    }
+    sr = wave_sum(sr);
+    si = wave_sum(si);
+    if (lane == 0) out[pt] = sqrtf(sr * sr + si * si) / (float)npairs;
+  }
+}
+
+template <bool DS>
+static int tcoh_launch(const float* c, const float* v, long long n_pts, int n, const int32_t* pairs, int n_pairs,
+                       float* out, mr_stream_t stream, const char* name) {
+  MR_CHECK_ARG(n_pts >= 0 && n >= 1 && n_pairs >= 0, "%s: bad sizes", name);
+  if (n_pts == 0) return MR_OK;
+  MR_CHECK_ARG(c && v && pairs && out, "%s: null pointer", name);
+  const int nb = (int)std::min<long long>(div_up((unsigned long long)n_pts, 4), 1 << 16);
+  hipLaunchKernelGGL((tcoh_kernel<DS>), dim3(nb), dim3(256), 0, as_stream(stream),
+                     reinterpret_cast<const float2*>(c), reinterpret_cast<const float2*>(v), n_pts, n, pairs,
+                     n_pairs, out);
+  return launch_status(name);
+}
+
+}  // namespace mr
+
+extern "C" {
+int mr_ds_temp_coh(const float* coh, const float* ph, long long n_pts, int n, const int32_t* pairs, int n_pairs,
+                   float* t_coh, mr_stream_t stream) {
+  return mr::tcoh_launch<true>(coh, ph, n_pts, n, pairs, n_pairs, t_coh, stream, "ds_temp_coh");
+}
+int mr_temp_coh(const float* intf, const float* rslc, long long n_pts, int n, const int32_t* pairs, int n_pairs,
+                float* t_coh, mr_stream_t stream) {
+  return mr::tcoh_launch<false>(intf, rslc, n_pts, n, pairs, n_pairs, t_coh, stream, "temp_coh");
+}
+}

@@ -1,0 +1,84 @@
+"""Phase linking on the GPU: EMI, DS temporal coherence and the fused batch driver.
+
+Drop-in for moraine.emi (moraine/pl.py:68-77), moraine.ds_temp_coh
+(moraine/pl.py:126-145) and moraine.emperical_co_emi_temp_coh_pc
+(moraine/pl.py:148-168).  Kernels: csrc/emi.hip, csrc/tcoh.hip.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _dev, _lib
+from .co import _pairs, emperical_co_pc
+from .tnet import nimage_from_npair
+
+
+def _emi_dev(c: torch.Tensor, n: int, ref: int):
+    npts = c.shape[0]
+    ph = torch.empty((npts, n), dtype=torch.complex64, device=c.device)
+    q = torch.empty(npts, dtype=torch.float32, device=c.device)
+    wsb = _lib.query("mr_emi_workspace_bytes", npts, n)
+    ws = _dev.workspace(wsb, c.device)
+    _lib.call("mr_emi", _dev.ptr(c), npts, n, int(ref), _dev.ptr(ph), _dev.ptr(q), _dev.ptr(ws), ws.numel(),
+              _dev.stream())
+    return ph, q
+
+
+def emi(coh, ref: int = 0):
+    """EMI phase history (n_pts, N) complex64 and quality (n_pts,) float32 from
+    compressed coherence (n_pts, N(N-1)/2) in triu order."""
+    as_torch = _dev.is_torch(coh)
+    c = _dev.to_dev(coh, torch.complex64)
+    n = nimage_from_npair(c.shape[-1])
+    lead = tuple(c.shape[:-1])
+    ph, q = _emi_dev(c.reshape(-1, c.shape[-1]), n, ref)
+    return _dev.back(ph.reshape(lead + (n,)), as_torch), _dev.back(q.reshape(lead), as_torch)
+
+
+def _tcoh(kind: str, a, b, image_pairs):
+    as_torch = _dev.is_torch(a)
+    x = _dev.to_dev(a, torch.complex64)
+    y = _dev.to_dev(b, torch.complex64)
+    n = y.shape[-1]
+    lead = tuple(y.shape[:-1])
+    x2 = x.reshape(-1, x.shape[-1])
+    y2 = y.reshape(-1, n)
+    pairs_np = _pairs(image_pairs, n)
+    if x2.shape[-1] != pairs_np.shape[0]:
+        raise ValueError(f"{kind}: last dim {x2.shape[-1]} != number of image pairs {pairs_np.shape[0]}")
+    pairs = _dev.to_dev(pairs_np)
+    out = torch.empty(y2.shape[0], dtype=torch.float32, device=y.device)
+    _lib.call(kind, _dev.ptr(x2), _dev.ptr(y2), y2.shape[0], n, _dev.ptr(pairs), pairs_np.shape[0], _dev.ptr(out),
+              _dev.stream())
+    return _dev.back(out.reshape(lead), as_torch)
+
+
+def ds_temp_coh(coh, ph, image_pairs=None, block_size: int = 128):
+    """|sum_p coh_p/|coh_p| conj(ph_a) ph_b| / n_pairs per point (float32)."""
+    return _tcoh("mr_ds_temp_coh", coh, ph, image_pairs)
+
+
+def emperical_co_emi_temp_coh_pc(rslc, idx, pc_is_shp, batch_size: int = 1000):
+    """Coherence -> EMI -> temporal coherence for a point cloud, in batches.
+
+    Every point is independent and every kernel deterministic, so the result
+    does not depend on the batch size; batches below 65536 points are merged
+    (they only bound memory: 8*N(N-1)/2 bytes of coherence per point).
+    """
+    as_torch = _dev.is_torch(rslc)
+    z = _dev.to_dev(rslc, torch.complex64)
+    ix = _dev.to_dev(idx, torch.int32)
+    m = _dev.to_dev(pc_is_shp, torch.bool)
+    n = z.shape[2]
+    n_pc = ix.shape[0]
+    ph = torch.empty((n_pc, n), dtype=torch.complex64, device=z.device)
+    q = torch.empty(n_pc, dtype=torch.float32, device=z.device)
+    t = torch.empty(n_pc, dtype=torch.float32, device=z.device)
+    bs = max(int(batch_size), 1 << 16)
+    for s in range(0, n_pc, bs):
+        e = min(n_pc, s + bs)
+        c = emperical_co_pc(z, ix[s:e], m[s:e])
+        ph[s:e], q[s:e] = _emi_dev(c, n, 0)
+        t[s:e] = ds_temp_coh(c, ph[s:e])
+    return _dev.back(ph, as_torch), _dev.back(q, as_torch), _dev.back(t, as_torch)

@@ -1,0 +1,124 @@
+"""Seeded synthetic SLC stacks for the DS hot path (workload generator).
+
+Distributed-scatterer model: per pixel z = s * (L w) * exp(j*phi) with
+w ~ CN(0, I_N), L = chol(Gamma), Gamma_ik = 0.6*exp(-|t_i - t_k| / 24 d) + 0.1
+(i != k), t_i = 12 d * i, phi_i = (4*pi/lambda) * v * t_i, v ~ U[0, 10] mm/yr,
+lambda = 5.55 cm (C band).  Amplitudes are therefore Rayleigh distributed with
+scale s.  Brightness regions: background scale 1 plus three rectangles with
+scales 2, 0.5 and 3.  PS-like pixels (optional fraction) have a constant
+amplitude over time and phase noise with coherence 0.95.
+
+``ds_stack`` returns numpy arrays (small, CPU-checkable cases);
+``ds_stack_torch`` generates the same model directly on a torch device (the
+full-size tiles used for throughput measurement).  The two are statistically
+identical, not bitwise identical.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+WAVELENGTH_M = 0.0555
+DT_DAYS = 12.0
+
+
+def coherence_model(n: int, g0: float = 0.6, tau: float = 24.0, floor: float = 0.1, dt: float = DT_DAYS):
+    t = dt * np.arange(n)
+    g = g0 * np.exp(-np.abs(t[:, None] - t[None, :]) / tau) + floor
+    np.fill_diagonal(g, 1.0)
+    return g
+
+
+def region_scale(nl: int, w: int) -> np.ndarray:
+    s = np.ones((nl, w), np.float32)
+    s[: nl // 2, w // 2:] = 2.0
+    s[nl // 2:, : w // 3] = 0.5
+    s[nl // 3: (2 * nl) // 3, w // 3: (2 * w) // 3] = 3.0
+    return s
+
+
+def _phase(v_mm_yr, n):
+    t = DT_DAYS * np.arange(n)
+    v_m_day = v_mm_yr / 1000.0 / 365.25
+    return (4 * np.pi / WAVELENGTH_M) * v_m_day[..., None] * t
+
+
+def ds_stack(nl: int, w: int, n: int, seed: int = 0, ps_frac: float = 0.0, regions: bool = True,
+             g0: float = 0.6, tau: float = 24.0) -> np.ndarray:
+    """(nl, w, n) complex64 stack from the DS/PS model above (numpy)."""
+    rng = np.random.default_rng(seed)
+    L = np.linalg.cholesky(coherence_model(n, g0, tau))
+    wv = (rng.standard_normal((nl, w, n)) + 1j * rng.standard_normal((nl, w, n))) / math.sqrt(2.0)
+    z = wv @ L.T
+    v = rng.uniform(0.0, 10.0, (nl, w))
+    ph = _phase(v, n)
+    s = region_scale(nl, w) if regions else np.ones((nl, w), np.float32)
+    z = z * np.exp(1j * ph) * s[..., None]
+    if ps_frac > 0:
+        is_ps = rng.random((nl, w)) < ps_frac
+        sigma = math.sqrt(-2.0 * math.log(0.95))
+        amp = s * rng.uniform(3.0, 6.0, (nl, w))
+        eps = rng.standard_normal((nl, w, n))
+        zps = amp[..., None] * np.exp(1j * (ph + sigma * eps))
+        z = np.where(is_ps[..., None], zps, z)
+    return z.astype(np.complex64)
+
+
+def ds_stack_torch(nl: int, w: int, n: int, seed: int = 0, ps_frac: float = 0.0, device="cuda",
+                   regions: bool = True, row_block: int = 256):
+    """Same model generated on a torch device in row blocks; returns complex64 (nl, w, n)."""
+    import torch
+
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    L = torch.from_numpy(np.linalg.cholesky(coherence_model(n)).astype(np.float32)).to(device)
+    s_all = torch.from_numpy(region_scale(nl, w) if regions else np.ones((nl, w), np.float32)).to(device)
+    t = torch.arange(n, device=device, dtype=torch.float32) * DT_DAYS
+    out = torch.empty((nl, w, n), dtype=torch.complex64, device=device)
+    sigma = math.sqrt(-2.0 * math.log(0.95))
+    for r0 in range(0, nl, row_block):
+        r1 = min(nl, r0 + row_block)
+        shp = (r1 - r0, w, n)
+        re = torch.randn(shp, generator=g, device=device) @ L.T
+        im = torch.randn(shp, generator=g, device=device) @ L.T
+        v = torch.rand((r1 - r0, w), generator=g, device=device) * 10.0
+        ph = (4 * math.pi / WAVELENGTH_M) * (v / 1000.0 / 365.25)[..., None] * t
+        s = s_all[r0:r1, :, None]
+        amp_re = re / math.sqrt(2.0) * s
+        amp_im = im / math.sqrt(2.0) * s
+        c, sn = torch.cos(ph), torch.sin(ph)
+        zr = amp_re * c - amp_im * sn
+        zi = amp_re * sn + amp_im * c
+        if ps_frac > 0:
+            is_ps = (torch.rand((r1 - r0, w), generator=g, device=device) < ps_frac)[..., None]
+            amp = s * (3.0 + 3.0 * torch.rand((r1 - r0, w, 1), generator=g, device=device))
+            pp = ph + sigma * torch.randn(shp, generator=g, device=device)
+            zr = torch.where(is_ps, amp * torch.cos(pp), zr)
+            zi = torch.where(is_ps, amp * torch.sin(pp), zi)
+        out[r0:r1] = torch.complex(zr, zi)
+    return out
+
+
+def closure_free_coh(n_pts: int, n: int, seed: int = 0, ref: int = 0):
+    """Compressed coherence of closure-free Gamma = |Gamma| o theta theta^H.
+
+    For such inputs EMI has the closed-form answer ph = theta*conj(theta_ref)/|.|
+    with quality 1 (the Schur product |Gamma|^-1 o |Gamma| has smallest
+    eigenvalue 1 with eigenvector 1).  Returns (coh (n_pts, P) complex64,
+    theta_ref (n_pts, n) complex128 expected phase history).
+    """
+    rng = np.random.default_rng(seed)
+    iu = np.triu_indices(n, 1)
+    coh = np.empty((n_pts, iu[0].size), np.complex128)
+    exp = np.empty((n_pts, n), np.complex128)
+    for i in range(n_pts):
+        g0 = rng.uniform(0.3, 0.9)
+        tau = rng.uniform(12.0, 96.0)
+        A = coherence_model(n, g0, tau)
+        th = np.exp(1j * rng.uniform(-np.pi, np.pi, n))
+        G = A * np.outer(th, th.conj())
+        coh[i] = G[iu]
+        e = th * np.conj(th[ref])
+        exp[i] = e / np.abs(e)
+    return coh.astype(np.complex64), exp

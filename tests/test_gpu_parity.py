@@ -1,0 +1,272 @@
+"""GPU parity: HIP kernels (through the C ABI) against the oracle.
+
+Bit-exact for the KS test (p, dist, SHP mask, SHP count); stated tolerances for
+floating-point stages, written next to each assertion.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from moraine_amd import synth
+from oracle import moraine_ref as R
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _case(msg):
+    print(f"[gpu case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def _mr():
+    import moraine_amd as mr
+
+    return mr
+
+
+def _bits_equal(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    assert a.shape == b.shape
+    if a.dtype == np.float32:
+        np.testing.assert_array_equal(a.view(np.uint32), b.view(np.uint32))
+    elif a.dtype == np.float64:
+        np.testing.assert_array_equal(a.view(np.uint64), b.view(np.uint64))
+    else:
+        np.testing.assert_array_equal(a, b)
+
+
+# ---------------------------------------------------------------------------- KS
+def test_ks_config1_bit_exact(port):
+    """200x200x17 DS stack with 4 brightness regions, 11x15 window, p=0.05."""
+    mr = _mr()
+    z = synth.ds_stack(200, 200, 17, seed=1)
+    rmli = (np.abs(z) ** 2).astype(np.float32)
+    _case("ks config1 200x200x17 az=5 r=7")
+    d, p = mr.ks_test(rmli, 5, 7, return_dist=True)
+    d_ref, p_ref = port.ks_test(rmli, 5, 7, return_dist=True)
+    _bits_equal(p, p_ref)
+    _bits_equal(d, d_ref)
+    p2 = mr.ks_test(rmli, 5, 7)
+    _bits_equal(p2, p_ref)
+    pf, m, c = mr.ks_test_select_shp(rmli, 5, 7, 0.05)
+    m_ref, c_ref = R.select_shp(p_ref, 0.05)
+    _bits_equal(pf, p_ref)
+    np.testing.assert_array_equal(m, m_ref)
+    np.testing.assert_array_equal(c, c_ref)
+    m2, c2 = mr.select_shp(p_ref, 0.05)
+    np.testing.assert_array_equal(m2, m_ref)
+    np.testing.assert_array_equal(c2, c_ref)
+
+
+def test_ks_config2_model_subtile_bit_exact(port):
+    """N=60 with 10% PS-like pixels (constant amplitude -> tie-heavy), 15x21 window."""
+    mr = _mr()
+    z = synth.ds_stack(48, 96, 60, seed=2, ps_frac=0.1)
+    rmli = np.abs(z).astype(np.float32)
+    _case("ks N=60 48x96 az=7 r=10")
+    p = mr.ks_test(rmli, 7, 10)
+    p_ref = port.ks_test(rmli, 7, 10)
+    _bits_equal(p, p_ref)
+
+
+@pytest.mark.parametrize("n", [1, 2, 20, 63, 64, 65, 128, 129, 200, 256])
+def test_ks_sizes_and_ties(port, n):
+    mr = _mr()
+    rng = np.random.default_rng(n)
+    rmli = rng.integers(0, 6, (7, 9, n)).astype(np.float32)  # heavy ties
+    rmli[0, 0] = 0.0
+    rmli[0, 1] = -0.0  # signed zero equals zero
+    rmli[2, 3, n // 2] = np.nan
+    rmli[5, 8, -1] = np.inf
+    rmli[4, 4, 0] = -np.inf
+    _case(f"ks ties n={n}")
+    d, p = mr.ks_test(rmli, 2, 3, return_dist=True)
+    d_ref, p_ref = R.ks_test(rmli, 2, 3, return_dist=True)
+    _bits_equal(p, p_ref)
+    _bits_equal(d, d_ref)
+
+
+def test_ks_edge_windows_and_f64(port):
+    mr = _mr()
+    rng = np.random.default_rng(7)
+    rmli = rng.random((3, 4, 20)).astype(np.float32)
+    for az, r in [(0, 0), (0, 3), (2, 0), (5, 9)]:
+        _case(f"ks window az={az} r={r} (window exceeds image)")
+        _bits_equal(mr.ks_test(rmli, az, r), R.ks_test(rmli, az, r))
+    r64 = rng.random((6, 7, 33))
+    _case("ks float64")
+    d64, p64 = mr.ks_test(r64, 2, 2, return_dist=True)
+    d_ref, p_ref = R.ks_test(r64, 2, 2, return_dist=True)
+    assert p64.dtype == np.float64
+    _bits_equal(p64, p_ref)
+    _bits_equal(d64, d_ref)
+
+
+def test_ks_printed_example():
+    mr = _mr()
+    rng = np.random.default_rng(0)
+    while True:
+        s1 = rng.uniform(size=20).astype(np.float32)
+        s2 = rng.standard_normal(20).astype(np.float32)
+        if R.ks_k_walk(np.sort(s1), np.sort(s2)) == 7:
+            break
+    rmli = np.stack((s1, s2)).reshape(1, 2, 20)
+    _case("ks printed 1x2x20 example")
+    d, p = mr.ks_test(rmli, 1, 1, return_dist=True)
+    assert d[0, 0, 1, 2] == np.float32(0.35) and format(p[0, 0, 1, 2], ".8g") == "0.13494715"
+    assert p[0, 0, 1, 1] == 0 and np.isnan(p[0, 0, 0, 0]) and np.isnan(p[0, 1, 1, 2])
+
+
+def test_ks_torch_io():
+    import torch
+
+    mr = _mr()
+    rmli = np.random.default_rng(5).random((10, 12, 30)).astype(np.float32)
+    _case("ks torch in/out")
+    pt = mr.ks_test(torch.from_numpy(rmli).cuda(), 3, 4)
+    assert isinstance(pt, torch.Tensor) and pt.is_cuda
+    _bits_equal(pt.cpu().numpy(), R.ks_test(rmli, 3, 4))
+
+
+# ---------------------------------------------------------------------------- coherence
+def test_co_small_vs_fp64():
+    mr = _mr()
+    g = np.load(os.path.join(GOLD, "co_small.npz"))
+    _case("emperical_co 5x10x17 golden")
+    cov, coh = mr.emperical_co(g["rslc"], g["is_shp"])
+    # tolerance of the reference's own test (nbs/API/co.ipynb:462): 1e-6 absolute
+    np.testing.assert_allclose(coh, g["coh64"], rtol=0, atol=1e-6)
+    np.testing.assert_allclose(cov, g["cov64"], rtol=1e-5, atol=1e-6)
+    c_pc = mr.emperical_co_pc(g["rslc"], g["idx"], g["is_shp"].reshape(-1, 3, 5))
+    iu = np.triu_indices(17, 1)
+    # same accumulation order in both kernels: the upper triangle is identical
+    np.testing.assert_array_equal(coh.reshape(-1, 17, 17)[:, iu[0], iu[1]], c_pc)
+    np.testing.assert_allclose(c_pc, g["coh_pc"], rtol=0, atol=1e-6)
+
+
+@pytest.mark.parametrize("n,az,r", [(60, 7, 10), (30, 5, 7), (120, 4, 6)])
+def test_co_pc_large_k_vs_fp64(n, az, r):
+    mr = _mr()
+    z = synth.ds_stack(20, 30, n, seed=n, ps_frac=0.1)
+    rng = np.random.default_rng(n)
+    shp = rng.random((40, 2 * az + 1, 2 * r + 1)) < 0.6
+    idx = np.stack([rng.integers(0, 20, 40), rng.integers(0, 30, 40)], -1).astype(np.int32)
+    _case(f"emperical_co_pc n={n} window {2*az+1}x{2*r+1}")
+    c = mr.emperical_co_pc(z, idx, shp)
+    c64 = R.emperical_co_pc64(z, idx, shp)
+    # FP32 accumulation over up to 315 SHPs: 1e-5 relative to the largest |coh| per point
+    err = np.max(np.abs(c - c64), axis=1) / np.max(np.abs(c64), axis=1)
+    assert err.max() < 1e-5
+    bw = R.image_pairs_from_bandwidth(n, 3)
+    cb = mr.emperical_co_pc(z, idx, shp, image_pairs=bw)
+    c64b = R.emperical_co_pc64(z, idx, shp, bw)
+    np.testing.assert_allclose(cb, c64b, rtol=0, atol=1e-5)
+
+
+def test_co_zero_shp_is_nan():
+    mr = _mr()
+    z = synth.ds_stack(4, 4, 17, seed=3)
+    shp = np.zeros((2, 3, 3), bool)
+    idx = np.array([[1, 1], [2, 2]], np.int32)
+    _case("emperical_co_pc zero SHPs")
+    c = mr.emperical_co_pc(z, idx, shp)
+    assert np.all(np.isnan(c))
+
+
+def test_uncompress_gpu():
+    mr = _mr()
+    c = (np.random.default_rng(1).random((3, 4, 45)) + 1j).astype(np.complex64)
+    _case("uncompress_coh")
+    np.testing.assert_array_equal(mr.uncompress_coh(c), R.uncompress_coh(c))
+    pairs = R.image_pairs_from_bandwidth(5, 2)
+    c2 = np.random.default_rng(2).random((6, 7)).astype(np.complex64)
+    np.testing.assert_array_equal(mr.uncompress_coh(c2, pairs), R.uncompress_coh(c2, pairs))
+
+
+# ---------------------------------------------------------------------------- EMI
+@pytest.mark.parametrize("n", [2, 3, 17, 30, 32, 33, 60, 64, 65, 90, 120])
+def test_emi_closure_free(n):
+    mr = _mr()
+    coh, exp = synth.closure_free_coh(24, n, seed=n)
+    _case(f"emi closure-free n={n}")
+    ph, q = mr.emi(coh)
+    err = np.abs(np.angle(ph * np.conj(exp)))
+    # closed form: ph = theta conj(theta_ref)/|.|, quality 1 (coherence stored in c64)
+    assert err.max() < 2e-4, err.max()
+    np.testing.assert_allclose(q, 1.0, atol=2e-3)
+    np.testing.assert_allclose(np.abs(ph), 1.0, atol=1e-5)
+
+
+@pytest.mark.parametrize("n", [17, 30, 60, 90])
+def test_emi_ds_vs_fp64(n):
+    mr = _mr()
+    z = synth.ds_stack(6, 8, n, seed=200 + n, ps_frac=0.1)
+    rng = np.random.default_rng(n)
+    win = (5, 7)
+    # mixed SHP counts: rank-deficient (indefinite |Gamma|) and full-rank windows
+    shp = rng.random((48, *win)) < rng.uniform(0.1, 1.0, (48, 1, 1))
+    shp[:, 2, 3] = True
+    idx = np.stack(np.nonzero(np.ones((6, 8), bool)), -1).astype(np.int32)
+    c = R.emperical_co_pc(z, idx, shp)
+    _case(f"emi DS n={n}")
+    ph, q = mr.emi(c)
+    ph64, q64 = R.emi64(c)
+    e = np.angle(ph * np.conj(ph64))
+    rms = np.sqrt(np.mean(e ** 2, axis=1))
+    # linked-phase RMS error vs FP64 <= 1e-3 rad (median far below)
+    assert np.nanmax(rms) < 1e-3, np.nanmax(rms)
+    np.testing.assert_allclose(q, q64, rtol=1e-3, atol=1e-3 * np.abs(q64).max())
+
+
+def test_emi_golden_and_ref():
+    mr = _mr()
+    g = np.load(os.path.join(GOLD, "emi_small.npz"))
+    for n in (17, 30, 60):
+        _case(f"emi golden n={n}")
+        ph, q = mr.emi(g[f"ds_coh_{n}"], ref=0)
+        assert np.abs(np.angle(ph * np.conj(g[f"ds_ph64_{n}"]))).max() < 1e-3
+        ph5, _ = mr.emi(g[f"ds_coh_{n}"], ref=5)
+        np.testing.assert_allclose(ph5[:, 5], 1.0, atol=1e-6)
+        # changing ref only rotates every phase by conj(ph[ref])
+        rot = ph * np.conj(ph[:, 5:6])
+        assert np.abs(np.angle(rot * np.conj(ph5))).max() < 1e-4
+
+
+# ---------------------------------------------------------------------------- temporal coherence
+def test_tcoh_vs_oracle_and_no_mutation():
+    mr = _mr()
+    g = np.load(os.path.join(GOLD, "tcoh_small.npz"))
+    coh, ph = g["coh"].copy(), g["ph"].copy()
+    _case("ds_temp_coh / temp_coh")
+    t = mr.ds_temp_coh(coh, ph)
+    np.testing.assert_allclose(t, g["t_ds"], rtol=1e-5, atol=1e-6)
+    t2 = mr.temp_coh(coh, ph)
+    np.testing.assert_allclose(t2, g["t_pq"], rtol=1e-5, atol=1e-6)
+    np.testing.assert_array_equal(coh, g["coh"])
+    np.testing.assert_array_equal(ph, g["ph"])
+    ras = mr.temp_coh(coh.reshape(6, 6, -1), ph.reshape(6, 6, -1))
+    assert ras.shape == (6, 6)
+    np.testing.assert_allclose(ras.ravel(), t2, rtol=0, atol=0)
+
+
+def test_fused_batch_driver_equals_composition():
+    """nbs/API/pl.ipynb:704-714: the batch driver equals the separate calls exactly."""
+    mr = _mr()
+    z = synth.ds_stack(12, 16, 17, seed=9, ps_frac=0.1)
+    rmli = (np.abs(z) ** 2).astype(np.float32)
+    _case("fused batch driver")
+    _, is_shp, shp_num = mr.ks_test_select_shp(rmli, 2, 3, 0.05, return_p=False)
+    cand = shp_num >= 3
+    idx = np.stack(np.nonzero(cand), -1).astype(np.int32)
+    pc_shp = is_shp[cand]
+    ph, q, t = mr.emperical_co_emi_temp_coh_pc(z, idx, pc_shp, batch_size=50)
+    c = mr.emperical_co_pc(z, idx, pc_shp)
+    ph2, q2 = mr.emi(c)
+    t2 = mr.ds_temp_coh(c, ph2)
+    np.testing.assert_array_equal(ph, ph2)
+    np.testing.assert_array_equal(q, q2)
+    np.testing.assert_array_equal(t, t2)

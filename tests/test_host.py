@@ -1,0 +1,101 @@
+"""CPU-side checks of the product: the C ABI library loads and exports every
+declared symbol, host-side tables and pair logic match the oracle/reference.
+No compute call touches a GPU here."""
+import ctypes
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+import moraine_amd as mr
+from moraine_amd import _lib
+from oracle import moraine_ref as R
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PRINTED = json.load(open(os.path.join(ROOT, "tests", "golden", "printed.json")))
+
+
+def _declared_symbols():
+    src = open(os.path.join(ROOT, "include", "moraine_hip.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(mr_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    syms = _declared_symbols()
+    assert len(syms) >= 14
+    for s in syms:
+        assert hasattr(lib, s), s
+        assert s in _lib.SIGNATURES, f"{s} has no ctypes signature"
+    assert lib.mr_abi_version() == 1
+
+
+def test_ks_tables_bit_exact_with_reference_series():
+    lib = _lib.load()
+    for n in list(range(1, 65)) + [90, 120, 150, 200, 256]:
+        p = np.empty(n + 1, np.float32)
+        d = np.empty(n + 1, np.float32)
+        s = np.empty(n + 1, np.uint8)
+        rc = lib.mr_ks_tables_f32(n, 0.05, p.ctypes.data_as(ctypes.c_void_p), d.ctypes.data_as(ctypes.c_void_p),
+                                  s.ctypes.data_as(ctypes.c_void_p))
+        assert rc == 0
+        pt, dt = R.ks_tables(n)
+        np.testing.assert_array_equal(p.view(np.uint32), pt.view(np.uint32))
+        np.testing.assert_array_equal(d.view(np.uint32), dt.view(np.uint32))
+        np.testing.assert_array_equal(s.astype(bool), pt.astype(np.float64) < 0.05)
+
+
+def test_ks_tables_reject_bad_n():
+    lib = _lib.load()
+    assert lib.mr_ks_tables_f32(0, 0.05, None, None, None) == 1
+    assert lib.mr_ks_tables_f32(257, 0.05, None, None, None) == 1
+    assert b"outside" in lib.mr_last_error_string()
+
+
+def test_argument_errors_raise_without_gpu():
+    # shape errors are detected on the host before any device work
+    with pytest.raises(ValueError):
+        _lib.call("mr_emi", None, 10, 1, 0, None, None, None, 0, None)  # n < 2
+    with pytest.raises(ValueError):
+        _lib.call("mr_emi", None, 10, 17, 17, None, None, None, 0, None)  # ref out of range
+    with pytest.raises(NotImplementedError):
+        _lib.call("mr_ks_test_f32", ctypes.c_void_p(1), 4, 4, 300, 1, 1, 0.05, None, None, None, None,
+                  ctypes.c_void_p(1), 1 << 30, None)
+
+
+def test_tempnet_matches_printed_and_oracle():
+    np.testing.assert_array_equal(mr.TempNet.from_bandwidth(5, 2).image_pairs, PRINTED["tnet_from_bandwidth_5_2"])
+    t = mr.TempNet(np.array(PRINTED["tnet_custom"]))
+    assert t.image_pairs_idx(ref=1, sec=2) == PRINTED["tnet_custom_idx_1_2"]
+    np.testing.assert_array_equal(t.image_pairs_idx(ref=[1, 2], sec=[2, 3]), PRINTED["tnet_custom_idx_12_23"])
+    assert mr.nimage_from_npair(6) == PRINTED["nimage_from_npair_6"]
+    for n in (2, 17, 60, 200):
+        np.testing.assert_array_equal(mr.TempNet.from_bandwidth(n).image_pairs, R.image_pairs_from_bandwidth(n))
+        assert mr.nimage_from_npair(n * (n - 1) // 2) == n
+    with pytest.raises(ValueError):
+        mr.nimage_from_npair(7)
+    with pytest.raises(ValueError):
+        mr.TempNet(np.array([[0, 1], [0, 1], [2, 1]]))
+    with pytest.raises(ValueError):
+        mr.TempNet(np.array([[0, 2], [1, 2]]))
+
+
+def test_api_signatures_match_reference_names():
+    import inspect
+
+    sig = {
+        "ks_test": ["rmli", "az_half_win", "r_half_win", "block_size", "return_dist"],
+        "select_shp": ["p", "p_max"],
+        "emperical_co": ["rslc", "is_shp", "block_size"],
+        "emperical_co_pc": ["rslc", "idx", "pc_is_shp", "block_size", "image_pairs"],
+        "emi": ["coh", "ref"],
+        "ds_temp_coh": ["coh", "ph", "image_pairs", "block_size"],
+        "temp_coh": ["intf", "rslc", "image_pairs", "block_size"],
+        "emperical_co_emi_temp_coh_pc": ["rslc", "idx", "pc_is_shp", "batch_size"],
+        "uncompress_coh": ["coh", "image_pairs"],
+    }
+    for name, params in sig.items():
+        assert list(inspect.signature(getattr(mr, name)).parameters) == params, name
