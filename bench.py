@@ -1,0 +1,262 @@
+#!/usr/bin/env python
+"""DS hot-path throughput on one MI355X: KS + SHP selection + coherence + EMI + temporal coherence.
+
+Workload (default): N=60 complex64 SLCs over a 2500x2500 tile, DS model with 10%
+PS-like pixels, 15x21 search window (az_half=7, r_half=10), p_max = 0.05, every
+pixel taken through the whole path.  One step = one pass over the tile:
+
+    rmli = |rslc|^2
+    p, is_shp, shp_num = ks_test + select_shp              (p materialised, as the drop-in returns it)
+    per batch of pixels: coh = emperical_co_pc -> (ph, quality) = emi -> t = ds_temp_coh
+
+Inputs are generated on the device before timing (synthetic, seeded).  Prints
+one JSON line.  See DESIGN.md for the roofline accounting of each stage.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix); also FP32 vector without packed math
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--size", type=int, default=2500, help="tile is size x size pixels")
+    ap.add_argument("--n", type=int, default=60)
+    ap.add_argument("--az", type=int, default=7)
+    ap.add_argument("--r", type=int, default=10)
+    ap.add_argument("--ps", type=float, default=0.10)
+    ap.add_argument("--batch", type=int, default=1 << 18, help="pixels per coherence/EMI batch")
+    ap.add_argument("--cpu-sample", type=int, default=160, help="CPU baseline: interior sub-tile edge")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--err-sample", type=int, default=2048)
+    return ap.parse_args()
+
+
+def ensure_built():
+    so = os.path.join(ROOT, "moraine_amd", "libmoraine_hip.so")
+    if not os.path.exists(so):
+        from moraine_amd import _build
+
+        _build.build()
+
+
+# ---------------------------------------------------------------------------- CPU baseline
+def cpu_baseline(args):
+    """Reference CPU path restated in C/OpenMP (KS, coherence, temporal coherence) plus
+    LAPACK EMI (float32 inv + complex64 eigh, as moraine/pl.py:45-65) on a bounded
+    interior sub-tile of the same workload model, all host cores."""
+    import concurrent.futures as cf
+
+    import numpy as np
+
+    from moraine_amd import synth
+    from oracle import moraine_ref as R
+    from oracle import port
+
+    port.build()
+    cores = len(os.sched_getaffinity(0))
+    s, az, r, n = args.cpu_sample, args.az, args.r, args.n
+    z = synth.ds_stack(s + 2 * az, s + 2 * r, n, seed=123, ps_frac=args.ps)
+    rmli = (np.abs(z) ** 2).astype(np.float32)
+    t0 = time.perf_counter()
+    p = port.ks_test(rmli, az, r, nthreads=cores)
+    is_shp, _ = R.select_shp(p, 0.05)
+    ii, jj = np.meshgrid(np.arange(az, az + s), np.arange(r, r + s), indexing="ij")
+    idx = np.stack([ii.ravel(), jj.ravel()], -1).astype(np.int32)
+    pc_shp = is_shp[az:az + s, r:r + s].reshape(-1, 2 * az + 1, 2 * r + 1)
+    pairs = R.image_pairs_from_bandwidth(n)
+    coh = port.emperical_co_pc(z, idx, pc_shp, pairs, nthreads=cores)
+    chunks = np.array_split(np.arange(idx.shape[0]), cores * 4)
+    with cf.ThreadPoolExecutor(cores) as ex:
+        res = list(ex.map(lambda c: R.emi_ref32(coh[c]), chunks))
+    ph = np.concatenate([a for a, _ in res])
+    port.ds_temp_coh(coh, ph, pairs, nthreads=cores)
+    dt = time.perf_counter() - t0
+    npx = idx.shape[0]
+    return {
+        "value": npx / dt, "unit": "DS pixels/s", "cores": cores, "kind": "port",
+        "sample": f"{s}x{s} interior pixels (+{az}/{r} halo) of the N={n}, {2*az+1}x{2*r+1}, {int(args.ps*100)}% PS "
+                  f"workload: C/OpenMP KS+select+coherence+tcoh, numpy LAPACK EMI (f32 inv, c64 eigh); {dt:.1f} s",
+    }
+
+
+# ---------------------------------------------------------------------------- GPU
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; use --gpus 1 (multi-GPU sharding is "
+                                   "described in DESIGN.md, not built)"}))
+        sys.exit(2)
+    ensure_built()
+    import numpy as np
+    import torch
+
+    import moraine_amd as mr
+    from moraine_amd import synth
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    S, N, az, r = args.size, args.n, args.az, args.r
+    aw, rw = 2 * az + 1, 2 * r + 1
+    npx = S * S
+    P = N * (N - 1) // 2
+
+    rslc = synth.ds_stack_torch(S, S, N, seed=0, ps_frac=args.ps, device=dev)
+    ii, jj = torch.meshgrid(torch.arange(S, device=dev, dtype=torch.int32),
+                            torch.arange(S, device=dev, dtype=torch.int32), indexing="ij")
+    idx = torch.stack([ii.reshape(-1), jj.reshape(-1)], -1).contiguous()
+    del ii, jj
+    ph = torch.empty((npx, N), dtype=torch.complex64, device=dev)
+    q = torch.empty(npx, dtype=torch.float32, device=dev)
+    tc = torch.empty(npx, dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+
+    batches = [(s, min(npx, s + args.batch)) for s in range(0, npx, args.batch)]
+    ev = {}
+
+    def mark(name):
+        if ev is None:
+            return
+        e = torch.cuda.Event(enable_timing=True)
+        e.record()
+        ev.setdefault(name, []).append(e)
+
+    state = {}
+
+    def step(instrument=False):
+        nonlocal ev
+        if not instrument:
+            ev = None
+        else:
+            ev = {}
+        mark("start")
+        x = torch.view_as_real(rslc)
+        rmli = (x[..., 0] * x[..., 0] + x[..., 1] * x[..., 1]).contiguous()
+        mark("rmli")
+        p, is_shp, shp_num = mr.ks_test_select_shp(rmli, az, r, 0.05, return_p=True)
+        mark("ks")
+        pcs = is_shp.view(npx, aw, rw)
+        for s, e in batches:
+            mark("co0")
+            coh = mr.emperical_co_pc(rslc, idx[s:e], pcs[s:e])
+            mark("co1")
+            ph[s:e], q[s:e] = mr.emi(coh)
+            mark("emi1")
+            tc[s:e] = mr.ds_temp_coh(coh, ph[s:e])
+            mark("tc1")
+        state["p"], state["shp_num"], state["coh_last"] = p, shp_num, coh
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    times = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        step()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - ts)
+    t1 = time.perf_counter()
+    dt = (t1 - t0) / args.steps
+
+    # instrumented step: per-stage device time from HIP events on the launch stream
+    step(instrument=True)
+    torch.cuda.synchronize()
+    e = ev
+
+    def span(a, b):
+        return sum(x.elapsed_time(y) for x, y in zip(e[a], e[b])) / 1e3
+
+    t_rmli = span("start", "rmli")
+    t_ks = span("rmli", "ks")
+    t_co = span("co0", "co1")
+    t_emi = span("co1", "emi1")
+    t_tc = span("emi1", "tc1")
+    nb = len(batches)
+
+    shp_num = state["shp_num"].float()
+    kmean = float(shp_num.mean())
+    # algorithmic units per pixel: bytes for ks/tcoh, flops for coherence/EMI
+    ks_bytes = 4 * N + 4 * aw * rw + aw * rw + 4  # rmli in, p out, mask out, count out
+    co_flops = 4.0 * kmean * N * (N + 1)  # Hermitian rank-K update (unique entries)
+    co_bytes = 8 * P
+    emi_flops = 2.0 * N ** 3 + (16.0 / 3.0) * N ** 3  # FP64 sweep inverse + Hermitian tridiagonalisation
+    tc_bytes = 8 * P + 8 * N + 4
+    stages = {
+        "rmli": {"s": t_rmli},
+        "ks_test+select_shp": {"s": t_ks, "GB/s": ks_bytes * npx / t_ks / 1e9,
+                               "hbm_frac": ks_bytes * npx / t_ks / 1e9 / HBM_PEAK_GBS},
+        "emperical_co_pc": {"s": t_co, "TFLOP/s": co_flops * npx / t_co / 1e12,
+                            "coh_write_GB/s": co_bytes * npx / t_co / 1e9},
+        "emi": {"s": t_emi, "TFLOP/s": emi_flops * npx / t_emi / 1e12},
+        "ds_temp_coh": {"s": t_tc, "GB/s": tc_bytes * npx / t_tc / 1e9,
+                        "hbm_frac": tc_bytes * npx / t_tc / 1e9 / HBM_PEAK_GBS},
+    }
+    dom = max(["ks_test+select_shp", "emperical_co_pc", "emi", "ds_temp_coh"], key=lambda k: stages[k]["s"])
+    if dom == "emi":
+        ach = emi_flops * (npx / nb) / (t_emi / nb) / 1e12
+        roof = {"kernel": "emi_fast_kernel<64>", "bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS, "traffic": None,
+                "note": "VALU-bound (FP64 sweep + FP32 Householder); peak = MI355X FP64 = FP32 non-packed vector rate"}
+    elif dom == "ks_test+select_shp":
+        ach = ks_bytes * npx / t_ks / 1e9
+        roof = {"kernel": "ks_pairs_kernel<float>", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None}
+    elif dom == "emperical_co_pc":
+        ach = co_flops * (npx / nb) / (t_co / nb) / 1e12
+        roof = {"kernel": "co_kernel<false>", "bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS, "traffic": None}
+    else:
+        ach = tc_bytes * npx / t_tc / 1e9
+        roof = {"kernel": "tcoh_kernel<true>", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None}
+
+    # phase error of the last batch vs an FP64 solve (torch/rocSOLVER, complex128) on a sample
+    coh = state["coh_last"]
+    s0 = batches[-1][0]
+    m = min(args.err_sample, coh.shape[0])
+    G = mr.uncompress_coh(coh[:m]).to(torch.complex128)
+    A = G.abs()
+    M = torch.linalg.inv(A).to(torch.complex128) * G
+    lam, V = torch.linalg.eigh(M)
+    v = V[..., :, 0] * V[..., 0:1, 0].conj()
+    ph64 = v / v.abs()
+    err = torch.angle(ph[s0:s0 + m].to(torch.complex128) * ph64.conj())
+    rms = torch.sqrt(torch.nanmean(err ** 2, dim=1))
+    phase = {"rms_median_rad": float(torch.nanmedian(rms)), "rms_p99_rad": float(torch.nanquantile(rms, 0.99)),
+             "rms_max_rad": float(rms[~torch.isnan(rms)].max()), "points": m}
+
+    cpu = None if args.no_cpu_baseline else cpu_baseline(args)
+    value = npx / dt
+    out = {
+        "metric": "DS pixels/s (KS+coh+EMI+tcoh) at N=60; p50/p99 latency; phase err vs FP64",
+        "value": value, "unit": "DS pixels/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (KS: int keys; EMI: f64 inverse + f32 eigen)", "data": "synthetic (seeded DS/PS model)",
+        "config": {"workload": f"plain drop-in pipeline, N={N}, {S}x{S} tile, {aw}x{rw} window, "
+                               f"{int(args.ps*100)}% PS, p_max=0.05, all pixels", "pixels": npx,
+                   "n_images": N, "window": [aw, rw], "batch": args.batch},
+        "latency_ms": {"p50": float(np.percentile(times, 50) * 1e3), "p99": float(np.percentile(times, 99) * 1e3)},
+        "phase_err_vs_fp64": phase,
+        "mean_shp_count": kmean,
+        "stages": stages,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -270,3 +270,20 @@ def test_fused_batch_driver_equals_composition():
     np.testing.assert_array_equal(ph, ph2)
     np.testing.assert_array_equal(q, q2)
     np.testing.assert_array_equal(t, t2)
+
+
+def test_emi_on_ks_selected_shps():
+    """Realistic SHP masks from the KS test (few SHPs -> near-singular |Gamma| at some
+    points).  EMI is compared with FP64 on the same coherence input."""
+    mr = _mr()
+    z = synth.ds_stack(24, 32, 17, seed=4, ps_frac=0.1)
+    rmli = (np.abs(z) ** 2).astype(np.float32)
+    _case("emi on KS-selected SHPs, N=17")
+    _, is_shp, shp_num = mr.ks_test_select_shp(rmli, 3, 4, 0.05, return_p=False)
+    cand = shp_num >= 5
+    idx = np.stack(np.nonzero(cand), -1).astype(np.int32)
+    coh = mr.emperical_co_pc(z, idx, is_shp[cand])
+    ph, q = mr.emi(coh)
+    ph64, q64 = R.emi64(coh)
+    rms = np.sqrt(np.nanmean(np.angle(ph * np.conj(ph64)) ** 2, axis=1))
+    assert np.nanmax(rms) < 1e-3, np.nanmax(rms)
