@@ -64,6 +64,109 @@ __device__ __forceinline__ int wave_sum(int v) {
   return v;
 }
 
+// ---- DPP / readlane cross-lane helpers (no LDS round trip) -----------------
+// DPP controls (gfx9 family): quad_perm [1,0,3,2] and [2,3,0,1] are lane xor 1
+// and xor 2; row_half_mirror / row_mirror pair the halves of 8- and 16-lane
+// groups; wave_shr:1 / wave_shl:1 shift by one lane across the whole wave.
+constexpr int DPP_XOR1 = 0xB1;
+constexpr int DPP_XOR2 = 0x4E;
+constexpr int DPP_HALF_MIRROR = 0x141;
+constexpr int DPP_MIRROR = 0x140;
+constexpr int DPP_WAVE_SHR1 = 0x138;  // lane i <- lane i-1 (lane 0 <- 0)
+constexpr int DPP_WAVE_SHL1 = 0x130;  // lane i <- lane i+1 (lane 63 <- 0)
+
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+  return __int_as_float(dpp_i<CTRL>(__float_as_int(v)));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = dpp_i<CTRL>((int)(b & 0xffffffffll));
+  const int hi = dpp_i<CTRL>((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ int rl_i(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ float rl_f(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
+__device__ __forceinline__ double rl_d(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over all 64 lanes, result uniform.  Fixed association order (deterministic).
+__device__ __forceinline__ float wsum_f(float v) {
+  v += dpp_f<DPP_XOR1>(v);
+  v += dpp_f<DPP_XOR2>(v);
+  v += dpp_f<DPP_HALF_MIRROR>(v);
+  v += dpp_f<DPP_MIRROR>(v);
+  return (rl_f(v, 0) + rl_f(v, 16)) + (rl_f(v, 32) + rl_f(v, 48));
+}
+__device__ __forceinline__ double wsum_d(double v) {
+  v += dpp_d<DPP_XOR1>(v);
+  v += dpp_d<DPP_XOR2>(v);
+  v += dpp_d<DPP_HALF_MIRROR>(v);
+  v += dpp_d<DPP_MIRROR>(v);
+  return (rl_d(v, 0) + rl_d(v, 16)) + (rl_d(v, 32) + rl_d(v, 48));
+}
+__device__ __forceinline__ double wmin_d(double v) {
+  v = fmin(v, dpp_d<DPP_XOR1>(v));
+  v = fmin(v, dpp_d<DPP_XOR2>(v));
+  v = fmin(v, dpp_d<DPP_HALF_MIRROR>(v));
+  v = fmin(v, dpp_d<DPP_MIRROR>(v));
+  return fmin(fmin(rl_d(v, 0), rl_d(v, 16)), fmin(rl_d(v, 32), rl_d(v, 48)));
+}
+__device__ __forceinline__ double wmax_d(double v) {
+  v = fmax(v, dpp_d<DPP_XOR1>(v));
+  v = fmax(v, dpp_d<DPP_XOR2>(v));
+  v = fmax(v, dpp_d<DPP_HALF_MIRROR>(v));
+  v = fmax(v, dpp_d<DPP_MIRROR>(v));
+  return fmax(fmax(rl_d(v, 0), rl_d(v, 16)), fmax(rl_d(v, 32), rl_d(v, 48)));
+}
+// Lane holding the largest key (ties -> smallest lane), uniform result.
+__device__ __forceinline__ int wargmax_d(double key, int lane) {
+  int idx = lane;
+#define MR_AM_STEP(CTRL)                                  \
+  {                                                       \
+    const double ok = dpp_d<CTRL>(key);                   \
+    const int oi = dpp_i<CTRL>(idx);                      \
+    if (ok > key || (ok == key && oi < idx)) {            \
+      key = ok;                                           \
+      idx = oi;                                           \
+    }                                                     \
+  }
+  MR_AM_STEP(DPP_XOR1)
+  MR_AM_STEP(DPP_XOR2)
+  MR_AM_STEP(DPP_HALF_MIRROR)
+  MR_AM_STEP(DPP_MIRROR)
+#undef MR_AM_STEP
+  double bk = rl_d(key, 0);
+  int bi = rl_i(idx, 0);
+#pragma unroll
+  for (int r = 1; r < 4; ++r) {
+    const double k2 = rl_d(key, 16 * r);
+    const int i2 = rl_i(idx, 16 * r);
+    if (k2 > bk || (k2 == bk && i2 < bi)) {
+      bk = k2;
+      bi = i2;
+    }
+  }
+  return __builtin_amdgcn_readfirstlane(bi);
+}
+
+// FP64 reciprocal: hardware approximation refined by two Newton steps (~1 ulp).
+__device__ __forceinline__ double rcp_d(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return r;
+}
+
 // ---- complex helpers (float2 / double2 as complex) ------------------------
 struct c64 {
   float re, im;

@@ -178,14 +178,6 @@ __global__ __launch_bounds__(EMI_FAST_THREADS) void emi_fast_kernel(const float2
   double* colb = reinterpret_cast<double*>(smem + ((P * 8 + 15) & ~15));  // 64 doubles (sweep broadcast)
   float2* ub = reinterpret_cast<float2*>(colb + 64);                      // 64 (reflector broadcast)
   float2* qb = ub + 64;                                                   // 64
-  double* td = reinterpret_cast<double*>(qb + 64);                        // NM  T diagonal
-  double* tae = td + NM;                                                  // NM  |sub-diagonal|
-  double* te2 = tae + NM;                                                 // NM  |sub-diagonal|^2
-  float2* dph = reinterpret_cast<float2*>(te2 + NM);                      // NM  phase scaling D
-  double* ty = reinterpret_cast<double*>(dph + NM);                       // NM  eigenvector of T
-  double* wk = ty + NM;                                                   // 5*NM work (LU)
-  float2* hal = reinterpret_cast<float2*>(wk + 5 * NM);                   // NM  Householder alpha_k
-  float* hinv = reinterpret_cast<float*>(hal + NM);                       // NM  Householder 1/den_k
 
   for (long long pt = blockIdx.x; pt < npts; pt += gridDim.x) {
     // Opaque per-point copy of the lane id: keeps the compiler from hoisting
@@ -217,7 +209,7 @@ __global__ __launch_bounds__(EMI_FAST_THREADS) void emi_fast_kernel(const float2
     double dg = lane < n ? 1.0 : 0.0;  // own diagonal entry, tracked through the sweep
     bool swept = lane >= n;
     for (int s = 0; s < n; ++s) {
-      const int k = wave_argmax(swept ? -1.0 : fabs(dg), lane);
+      const int k = wargmax_d(swept ? -1.0 : fabs(dg), lane);
       const double c = reg_get<double, NM>(a, k);
       // Broadcasting d-1 in place of the pivot d lets the rank-1 update itself
       // write column k:  c - (c/d)(d-1) = c/d.  The pivot lane's own diagonal
@@ -225,13 +217,27 @@ __global__ __launch_bounds__(EMI_FAST_THREADS) void emi_fast_kernel(const float2
       // by the sweep, so the +2 offset is removed once, when M is formed.
       colb[lane] = (lane == k) ? c - 1.0 : c;
       __builtin_amdgcn_wave_barrier();
-      const double dk = __shfl(c, k, 64);
-      const double rd = 1.0 / dk;
+      const double dk = rl_d(c, k);
+      const double rd = rcp_d(dk);
       const double f = (lane == k) ? (1.0 - rd) : c * rd;
+      {
+        // software-pipelined broadcast of colb: group g+1 is loaded before the
+        // FMAs of group g, so LDS latency overlaps the FP64 FMAs
+        double cur[8], nxt[8];
 #pragma unroll
-      for (int j = 0; j < NM; ++j) {
-        a[j] = fma(-f, colb[j], a[j]);
-        if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);  // bound live broadcast registers
+        for (int t = 0; t < 8; ++t) cur[t] = colb[t];
+#pragma unroll
+        for (int g = 0; g < NM / 8; ++g) {
+          if (g + 1 < NM / 8) {
+#pragma unroll
+            for (int t = 0; t < 8; ++t) nxt[t] = colb[(g + 1) * 8 + t];
+          }
+#pragma unroll
+          for (int t = 0; t < 8; ++t) a[g * 8 + t] = fma(-f, cur[t], a[g * 8 + t]);
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int t = 0; t < 8; ++t) cur[t] = nxt[t];
+        }
       }
       dg = fma(-f, c, dg);
       swept = swept || (lane == k);
@@ -264,78 +270,127 @@ __global__ __launch_bounds__(EMI_FAST_THREADS) void emi_fast_kernel(const float2
     }
 
     // ---- step 3: Householder tridiagonalisation -----------------------------
-    float2 dcur = make_float2(1.f, 0.f);  // running phase product D_k
-    if (lane == 0) dph[0] = dcur;
+    // T is kept distributed: lane i holds d_i = T[i][i], e_i = |T[i+1][i]|, the
+    // unitary scaling D_i, and (alpha_i, 1/den_i) of reflector i.
+    double my_d = 0.0, my_e = 0.0;
+    float my_ar = 0.f, my_ai = 0.f, my_hv = 0.f;
+    float my_dr = 1.f, my_di = 0.f;
+    float dcr = 1.f, dci = 0.f;  // running phase product D_k (uniform)
     for (int k = 0; k + 2 < n; ++k) {
       const float xr_raw = reg_get<float, NM>(mr_, k);
       const float xi_raw = reg_get<float, NM>(mi_, k);
       const bool below = lane > k && lane < n;
       const float xr = below ? xr_raw : 0.f, xi = below ? xi_raw : 0.f;
-      const float nrm2 = wave_sum(xr * xr + xi * xi);
-      const float x0r = __shfl(xr, k + 1, 64), x0i = __shfl(xi, k + 1, 64);
-      const float dkk = __shfl(xr_raw, k, 64);
-      const float nx = sqrtf(nrm2);
-      const float ax0 = sqrtf(x0r * x0r + x0i * x0i);
-      const float phr = ax0 > 0.f ? x0r / ax0 : 1.f, phi = ax0 > 0.f ? x0i / ax0 : 0.f;
+      const float nrm2 = wsum_f(xr * xr + xi * xi);
+      const float x0r = rl_f(xr, k + 1), x0i = rl_f(xi, k + 1);
+      const float dkk = rl_f(xr_raw, k);
+      const float nx = __builtin_amdgcn_sqrtf(nrm2);
+      const float ax0 = __builtin_amdgcn_sqrtf(x0r * x0r + x0i * x0i);
+      const float rax0 = ax0 > 0.f ? __builtin_amdgcn_rcpf(ax0) : 0.f;
+      const float phr = ax0 > 0.f ? x0r * rax0 : 1.f, phi = ax0 > 0.f ? x0i * rax0 : 0.f;
       const float alr = -phr * nx, ali = -phi * nx;  // alpha = -phase * ||x||
-      const float den = sqrtf(2.f * nx * (nx + ax0));
-      const float inv = den > 0.f ? 1.f / den : 0.f;
+      const float den2 = 2.f * nx * (nx + ax0);
+      const float inv = den2 > 0.f ? __builtin_amdgcn_rsqf(den2) : 0.f;
       float ur = (lane == k + 1) ? (x0r - alr) : xr;
       float ui = (lane == k + 1) ? (x0i - ali) : xi;
       ur *= inv;
       ui *= inv;
       // column k keeps x (rows > k are never updated again); the reflector is
       // rebuilt from x, alpha and 1/den during the back-transformation
-      if (lane == 0) {
-        hal[k] = make_float2(alr, ali);
-        hinv[k] = inv;
-        td[k] = dkk;
-        tae[k] = nx;
-        te2[k] = (double)nx * nx;
-        // D_{k+1} = D_k * alpha/|alpha|
-        const float sr = nx > 0.f ? alr / nx : 1.f, si = nx > 0.f ? ali / nx : 0.f;
-        dcur = make_float2(dcur.x * sr - dcur.y * si, dcur.x * si + dcur.y * sr);
-        dph[k + 1] = dcur;
+      if (lane == k) {
+        my_d = dkk;
+        my_e = nx;
+        my_ar = alr;
+        my_ai = ali;
+        my_hv = inv;
+      }
+      {
+        const float sr = nx > 0.f ? -phr : 1.f, si = nx > 0.f ? -phi : 0.f;  // alpha/|alpha|
+        const float r = dcr * sr - dci * si, i = dcr * si + dci * sr;
+        dcr = r;
+        dci = i;
+      }
+      if (lane == k + 1) {
+        my_dr = dcr;
+        my_di = dci;
       }
       ub[lane] = make_float2(ur, ui);
       __builtin_amdgcn_wave_barrier();
-      // p = M22 u
+      // p = M22 u  (4-column groups left of the active block or beyond n are skipped)
       float pr = 0.f, pi = 0.f;
+      {
+        float2 cu[4], nu[4];
 #pragma unroll
-      for (int j = 0; j < NM; ++j) {
-        if ((j | 7) > k) {  // skip 8-column groups left of the active block (uniform)
-          const float2 uj = ub[j];
-          pr = fmaf(mr_[j], uj.x, pr);
-          pr = fmaf(-mi_[j], uj.y, pr);
-          pi = fmaf(mr_[j], uj.y, pi);
-          pi = fmaf(mi_[j], uj.x, pi);
+        for (int t = 0; t < 4; ++t) cu[t] = ub[t];
+#pragma unroll
+        for (int g = 0; g < NM / 4; ++g) {
+          const bool act = (g * 4 + 3 > k) && (g * 4 < n);
+          if (g + 1 < NM / 4) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) nu[t] = ub[(g + 1) * 4 + t];
+          }
+          if (act) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+              const int j = g * 4 + t;
+              pr = fmaf(mr_[j], cu[t].x, pr);
+              pr = fmaf(-mi_[j], cu[t].y, pr);
+              pi = fmaf(mr_[j], cu[t].y, pi);
+              pi = fmaf(mi_[j], cu[t].x, pi);
+            }
+          }
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int t = 0; t < 4; ++t) cu[t] = nu[t];
         }
-        if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
       }
-      const float cval = wave_sum(ur * pr + ui * pi);  // real(u^H p)
+      const float cval = wsum_f(ur * pr + ui * pi);  // real(u^H p)
       const float qr = below ? 2.f * (pr - cval * ur) : 0.f;
       const float qi = below ? 2.f * (pi - cval * ui) : 0.f;
       qb[lane] = make_float2(qr, qi);
       __builtin_amdgcn_wave_barrier();
       // M22 -= u q^H + q u^H
+      {
+        float2 cu[4], cq[4], nu[4], nq[4];
 #pragma unroll
-      for (int j = 0; j < NM; ++j) {
-        if ((j | 7) > k) {
-          const float2 uj = ub[j];
-          const float2 qj = qb[j];
-          float r = mr_[j], i = mi_[j];
-          r = fmaf(-ur, qj.x, r);
-          r = fmaf(-ui, qj.y, r);
-          i = fmaf(ur, qj.y, i);
-          i = fmaf(-ui, qj.x, i);
-          r = fmaf(-qr, uj.x, r);
-          r = fmaf(-qi, uj.y, r);
-          i = fmaf(qr, uj.y, i);
-          i = fmaf(-qi, uj.x, i);
-          mr_[j] = r;
-          mi_[j] = i;
+        for (int t = 0; t < 4; ++t) {
+          cu[t] = ub[t];
+          cq[t] = qb[t];
         }
-        if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int g = 0; g < NM / 4; ++g) {
+          const bool act = (g * 4 + 3 > k) && (g * 4 < n);
+          if (g + 1 < NM / 4) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+              nu[t] = ub[(g + 1) * 4 + t];
+              nq[t] = qb[(g + 1) * 4 + t];
+            }
+          }
+          if (act) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+              const int j = g * 4 + t;
+              float r = mr_[j], i = mi_[j];
+              r = fmaf(-ur, cq[t].x, r);
+              r = fmaf(-ui, cq[t].y, r);
+              i = fmaf(ur, cq[t].y, i);
+              i = fmaf(-ui, cq[t].x, i);
+              r = fmaf(-qr, cu[t].x, r);
+              r = fmaf(-qi, cu[t].y, r);
+              i = fmaf(qr, cu[t].y, i);
+              i = fmaf(-qi, cu[t].x, i);
+              mr_[j] = r;
+              mi_[j] = i;
+            }
+          }
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            cu[t] = nu[t];
+            cq[t] = nq[t];
+          }
+        }
       }
       __builtin_amdgcn_wave_barrier();
     }
@@ -344,103 +399,134 @@ __global__ __launch_bounds__(EMI_FAST_THREADS) void emi_fast_kernel(const float2
       const float r0 = reg_get<float, NM>(mr_, n - 2);
       const float i0 = reg_get<float, NM>(mi_, n - 2);
       const float r1 = reg_get<float, NM>(mr_, n - 1);
-      const float dm2 = __shfl(r0, n - 2, 64);
-      const float dm1 = __shfl(r1, n - 1, 64);
-      const float er = __shfl(r0, n - 1, 64), ei = __shfl(i0, n - 1, 64);
-      if (lane == 0) {
-        const float ae = sqrtf(er * er + ei * ei);
-        td[n - 2] = dm2;
-        td[n - 1] = dm1;
-        tae[n - 2] = ae;
-        te2[n - 2] = (double)ae * ae;
+      const float dm2 = rl_f(r0, n - 2);
+      const float dm1 = rl_f(r1, n - 1);
+      const float er = rl_f(r0, n - 1), ei = rl_f(i0, n - 1);
+      const float ae = sqrtf(er * er + ei * ei);
+      if (lane == n - 2) {
+        my_d = dm2;
+        my_e = ae;
+      }
+      if (lane == n - 1) {
+        my_d = dm1;
+        my_e = 0.0;
         const float sr = ae > 0.f ? er / ae : 1.f, si = ae > 0.f ? ei / ae : 0.f;
-        dph[n - 1] = make_float2(dcur.x * sr - dcur.y * si, dcur.x * si + dcur.y * sr);
-        tae[n - 1] = 0.0;
-        te2[n - 1] = 0.0;
+        my_dr = dcr * sr - dci * si;
+        my_di = dcr * si + dci * sr;
       }
     }
-    __builtin_amdgcn_wave_barrier();
+    const double my_e2 = my_e * my_e;
 
     // ---- step 4a: smallest eigenvalue of T by multisection ------------------
+    const double e_prev = dpp_d<DPP_WAVE_SHR1>(my_e);  // e_{i-1} (0 for lane 0)
     double lo, hi, scale;
     {
-      double gl = 0.0, gh = 0.0, sc = 0.0;
-      if (lane < n) {
-        const double off = (lane > 0 ? tae[lane - 1] : 0.0) + tae[lane];
-        gl = td[lane] - off;
-        gh = td[lane] + off;
-        sc = fabs(td[lane]) + off;
-      } else {
-        gl = 1e308;
-        gh = -1e308;
-      }
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) {
-        gl = fmin(gl, __shfl_xor(gl, o, 64));
-        gh = fmax(gh, __shfl_xor(gh, o, 64));
-        sc = fmax(sc, __shfl_xor(sc, o, 64));
-      }
+      const double off = e_prev + my_e;
+      lo = wmin_d(lane < n ? my_d - off : 1e308);
+      hi = wmax_d(lane < n ? my_d + off : -1e308);
+      const double sc = wmax_d(lane < n ? fabs(my_d) + off : 0.0);
       scale = sc > 0.0 ? sc : 1.0;
-      lo = gl - 1e-14 * scale;
-      hi = gh + 1e-14 * scale;
+      lo -= 1e-14 * scale;
+      hi += 1e-14 * scale;
     }
     const double pivmin = 1e-300 + scale * scale * 1e-290;
-    for (int round = 0; round < 8; ++round) {
+    for (int round = 0; round < 6; ++round) {
       if (hi - lo <= 2e-15 * scale) break;
-      const double x = lo + (hi - lo) * (double)(lane + 1) / 65.0;
-      const int cnt = sturm_count(td, te2, n, x, pivmin);
+      const double x = lo + (hi - lo) * (double)(lane + 1) * (1.0 / 65.0);
+      // Sturm count of eigenvalues below x; d_i, e_{i-1}^2 broadcast by readlane
+      double q = rl_d(my_d, 0) - x;
+      int cnt = q < 0.0;
+      for (int i = 1; i < n; ++i) {
+        if (fabs(q) < pivmin) q = -pivmin;
+        double r = __builtin_amdgcn_rcp(q);
+        r = fma(r, fma(-q, r, 1.0), r);  // one Newton step on the reciprocal
+        q = (rl_d(my_d, i) - x) - rl_d(my_e2, i - 1) * r;
+        cnt += q < 0.0;
+      }
       const unsigned long long m = __ballot(cnt >= 1);
       if (m == 0ull) {
-        lo = __shfl(x, 63, 64);
+        lo = rl_d(x, 63);
       } else {
         const int L = __ffsll((long long)m) - 1;
-        const double xl = __shfl(x, L, 64);
-        const double xp = __shfl(x, L > 0 ? L - 1 : 0, 64);
+        const double xl = rl_d(x, L);
+        const double xp = rl_d(x, L > 0 ? L - 1 : 0);
         hi = xl;
         if (L > 0) lo = xp;
       }
     }
     const double lam = 0.5 * (lo + hi);
 
-    // ---- step 4b: eigenvector of T, back-transformation ----------------------
-    if (lane == 0) tridiag_min_eigpair(td, tae, te2, ty, wk, n, lam, scale);
-    __builtin_amdgcn_wave_barrier();
-    // Rayleigh quotient of T for the reported eigenvalue
-    double rq;
+    // ---- step 4b: eigenvector of T by a twisted factorisation ---------------
+    // D+_i = (d_i - lam) - e_{i-1}^2 / D+_{i-1},  D-_i = (d_i - lam) - e_i^2 / D-_{i+1},
+    // gamma_i = D+_i + D-_i - (d_i - lam); twist r = argmin |gamma|;
+    // z_r = 1, z_i = -(e_i / D+_i) z_{i+1} (i < r), z_{i+1} = -(e_i / D-_{i+1}) z_i (i >= r).
+    const double tiny = 2.2e-16 * scale + 1e-300;
+    double my_dp = 0.0, my_dm = 0.0, my_rhi = 0.0;
     {
-      double t = 0.0;
-      if (lane < n) {
-        const double yi = ty[lane];
-        double ty_ = td[lane] * yi;
-        if (lane > 0) ty_ += tae[lane - 1] * ty[lane - 1];
-        if (lane + 1 < n) ty_ += tae[lane] * ty[lane + 1];
-        t = yi * ty_;
+      double dp = rl_d(my_d, 0) - lam;
+      if (fabs(dp) < tiny) dp = tiny;
+      if (lane == 0) my_dp = dp;
+      for (int i = 1; i < n; ++i) {
+        dp = (rl_d(my_d, i) - lam) - rl_d(my_e2, i - 1) / dp;
+        if (fabs(dp) < tiny) dp = tiny;
+        if (lane == i) my_dp = dp;
       }
-      rq = wave_sum(t);
+      double dm = rl_d(my_d, n - 1) - lam;
+      if (fabs(dm) < tiny) dm = tiny;
+      if (lane == n - 1) my_dm = dm;
+      for (int i = n - 2; i >= 0; --i) {
+        const double ei = rl_d(my_e, i);
+        if (lane == i) my_rhi = -ei / dm;  // dm = D-_{i+1}
+        dm = (rl_d(my_d, i) - lam) - ei * ei / dm;
+        if (fabs(dm) < tiny) dm = tiny;
+        if (lane == i) my_dm = dm;
+      }
     }
+    const double gam = lane < n ? fabs(my_dp + my_dm - (my_d - lam)) : 1e308;
+    const int rtw = wargmax_d(-gam, lane);
+    const double my_rlo = -my_e / my_dp;
+    double my_z = (lane == rtw) ? 1.0 : 0.0;
+    {
+      double z = 1.0;
+      for (int i = rtw - 1; i >= 0; --i) {
+        z *= rl_d(my_rlo, i);
+        if (lane == i) my_z = z;
+      }
+      z = 1.0;
+      for (int i = rtw + 1; i < n; ++i) {
+        z *= rl_d(my_rhi, i - 1);
+        if (lane == i) my_z = z;
+      }
+    }
+    my_z *= 1.0 / sqrt(wsum_d(my_z * my_z));
+    // Rayleigh quotient of T for the reported eigenvalue
+    const double z_prev = dpp_d<DPP_WAVE_SHR1>(my_z);
+    const double z_next = dpp_d<DPP_WAVE_SHL1>(my_z);
+    const double tz = lane < n ? my_d * my_z + e_prev * z_prev + my_e * z_next : 0.0;
+    const double rq = wsum_d(my_z * tz);
+
+    // ---- back-transformation v = H_0 ... H_{n-3} D z ------------------------
     float vr = 0.f, vi = 0.f;
     if (lane < n) {
-      const float2 dd = dph[lane];
-      const float yi = (float)ty[lane];
-      vr = dd.x * yi;
-      vi = dd.y * yi;
+      const float zi = (float)my_z;
+      vr = my_dr * zi;
+      vi = my_di * zi;
     }
     for (int k = n - 3; k >= 0; --k) {
       const float xr = reg_get<float, NM>(mr_, k);
       const float xi = reg_get<float, NM>(mi_, k);
       const bool below = lane > k && lane < n;
-      const float2 al = hal[k];
-      const float hv = hinv[k];
-      const float ar = below ? ((lane == k + 1) ? xr - al.x : xr) * hv : 0.f;
-      const float ai = below ? ((lane == k + 1) ? xi - al.y : xi) * hv : 0.f;
+      const float alr = rl_f(my_ar, k), ali = rl_f(my_ai, k), hv = rl_f(my_hv, k);
+      const float ar = below ? ((lane == k + 1) ? xr - alr : xr) * hv : 0.f;
+      const float ai = below ? ((lane == k + 1) ? xi - ali : xi) * hv : 0.f;
       // s = u^H v
-      const float sr = wave_sum(ar * vr + ai * vi);
-      const float si = wave_sum(ar * vi - ai * vr);
+      const float sr = wsum_f(ar * vr + ai * vi);
+      const float si = wsum_f(ar * vi - ai * vr);
       vr -= 2.f * (ar * sr - ai * si);
       vi -= 2.f * (ar * si + ai * sr);
     }
     // ---- output: ph = v conj(v_ref) / |.|, quality --------------------------
-    const float rr = __shfl(vr, ref, 64), ri = __shfl(vi, ref, 64);
+    const float rr = rl_f(vr, ref), ri = rl_f(vi, ref);
     if (lane < n) {
       const float pr = vr * rr + vi * ri;
       const float pim = vi * rr - vr * ri;
@@ -719,7 +805,7 @@ __global__ __launch_bounds__(EMI_GEN_THREADS) void emi_general_kernel(const floa
 
 static size_t emi_fast_lds(int n, int nm) {
   const int P = n * (n - 1) / 2;
-  return (((size_t)P * 8 + 15) & ~(size_t)15) + 64 * (8 + 8 + 8) + (size_t)nm * (8 * 3 + 8 + 8 + 8 * 5 + 8 + 4) + 64;
+  return (((size_t)P * 8 + 15) & ~(size_t)15) + 64 * (8 + 8 + 8) + 64;
 }
 
 }  // namespace mr
