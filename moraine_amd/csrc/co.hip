@@ -11,6 +11,11 @@
 
 namespace mr {
 
+int co_mfma_launch(bool raster, const float2* rslc, int nl, int w, int n, const int32_t* idx, const uint8_t* shp,
+                   long long npts, int aw, int rw, long long pix0, const int32_t* pairs, int npairs, float2* cov,
+                   float2* coh, hipStream_t st);
+constexpr int CO_MFMA_MAX_N = 64;
+
 constexpr int CO_THREADS = 256;
 constexpr int CO_QB = 8;             // pairs per thread per batch
 constexpr int CO_MAX_WIN = 4096;     // window entries
@@ -207,6 +212,10 @@ int mr_emperical_co(const float* rslc, const uint8_t* is_shp, int nl, int w, int
   if (int rc = mr::co_check(n, aw, rw)) return rc;
   const long long npts = (long long)nrows * w;
   if (npts == 0) return MR_OK;
+  if (n <= mr::CO_MFMA_MAX_N)
+    return mr::co_mfma_launch(true, reinterpret_cast<const float2*>(rslc), nl, w, n, nullptr, is_shp, npts, aw, rw,
+                              (long long)row0 * w, nullptr, 0, reinterpret_cast<float2*>(cov),
+                              reinterpret_cast<float2*>(coh), mr::as_stream(stream));
   const int nb = (int)std::min<long long>(npts, 1 << 16);
   const size_t lds = mr::CO_STAGE_BYTES + (size_t)n * 4;
   hipLaunchKernelGGL((mr::co_kernel<true>), dim3(nb), dim3(mr::CO_THREADS), lds, mr::as_stream(stream),
@@ -221,6 +230,9 @@ int mr_emperical_co_pc(const float* rslc, int nl, int w, int n, const int32_t* i
   if (n_pc == 0 || n_pairs == 0) return MR_OK;
   MR_CHECK_ARG(rslc && idx && pc_is_shp && pairs && coh, "emperical_co_pc: null pointer");
   if (int rc = mr::co_check(n, aw, rw)) return rc;
+  if (n <= mr::CO_MFMA_MAX_N)
+    return mr::co_mfma_launch(false, reinterpret_cast<const float2*>(rslc), nl, w, n, idx, pc_is_shp, n_pc, aw, rw, 0,
+                              pairs, n_pairs, nullptr, reinterpret_cast<float2*>(coh), mr::as_stream(stream));
   const int nb = std::min(n_pc, 1 << 16);
   const size_t lds = mr::CO_STAGE_BYTES + (size_t)n * 4;
   hipLaunchKernelGGL((mr::co_kernel<false>), dim3(nb), dim3(mr::CO_THREADS), lds, mr::as_stream(stream),

@@ -1,0 +1,265 @@
+// Coherence estimation on MFMA (gfx950), n <= 64.
+//
+// Per point the SHP stacks z_k (k = 1..K, in the reference's window order,
+// moraine/co.py:83-111) form the Hermitian rank-K update S = sum_k z_k z_k^H.
+// With the real 2x2 embedding
+//     Re S = X X^T,  Im S = Y X^T,   X = [Re z | Im z],  Y = [Im z | -Re z]
+// both halves are real GEMMs over 2K "k-slots" (slot 2k = real part of SHP k,
+// slot 2k+1 = imaginary part).  They run on v_mfma_f32_32x32x16_f16 with every
+// operand split into FP16 hi + lo (x = hi + lo, |x - hi - lo| <= 2^-22 |x|); the
+// three products hi*hi + hi*lo + lo*hi accumulate in FP32, which gives FP32-level
+// products.  Before the split every image row is scaled by a power of two so its
+// largest component lies in [32, 64): products are exact in FP32, sums stay far
+// from overflow, and coherence is invariant under the per-image scaling
+// (cov is unscaled exactly, the factors being powers of two).
+// Output tiles: (0,0) for n <= 32; (0,0), (0,1), (1,1) for 32 < n <= 64 (the
+// (1,0) tile is the conjugate transpose of (0,1)).
+#include "common.hpp"
+
+namespace mr {
+
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+typedef float f16x_t __attribute__((ext_vector_type(16)));
+
+constexpr int COM_WAVES = 2;  // points per block (one per wave); 2 x ~25 KiB LDS per block
+
+__host__ __device__ inline int com_tile_floats(bool two) { return two ? 3 * 32 * 32 * 2 : 32 * 32 * 2; }
+__host__ __device__ inline size_t com_wave_bytes(int win, bool two) {
+  const size_t list = ((size_t)win * 4 + 15) & ~(size_t)15;
+  const size_t tile = (size_t)com_tile_floats(two) * 4;
+  return (list > tile ? list : tile) + 64 * 4 /* scales */;
+}
+
+// split 8 floats into hi/lo halves (hi truncated, lo = exact remainder rounded)
+__device__ __forceinline__ void split8(const float (&v)[8], half8_t& hi, half8_t& lo) {
+#pragma unroll
+  for (int i = 0; i < 8; i += 2) {
+    const half2_t h = __builtin_bit_cast(half2_t, __builtin_amdgcn_cvt_pkrtz(v[i], v[i + 1]));
+    const float r0 = v[i] - (float)h[0];
+    const float r1 = v[i + 1] - (float)h[1];
+    const half2_t l = __builtin_bit_cast(half2_t, __builtin_amdgcn_cvt_pkrtz(r0, r1));
+    hi[i] = h[0];
+    hi[i + 1] = h[1];
+    lo[i] = l[0];
+    lo[i + 1] = l[1];
+  }
+}
+
+__device__ __forceinline__ f16x_t mfma3(const half8_t& ah, const half8_t& al, const half8_t& bh, const half8_t& bl,
+                                        f16x_t c) {
+  c = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, c, 0, 0, 0);
+  return c;
+}
+
+template <bool RASTER, bool TWO>
+__global__ __launch_bounds__(64 * COM_WAVES) void co_mfma_kernel(
+    const float2* __restrict__ rslc, int nl, int w, int n, const int32_t* __restrict__ idx,
+    const uint8_t* __restrict__ shp, long long npts, int aw, int rw, long long pix0,
+    const int32_t* __restrict__ pairs, int npairs, float2* __restrict__ cov, float2* __restrict__ coh) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int win = aw * rw;
+  const int az = (aw - 1) / 2, r = (rw - 1) / 2;
+  unsigned char* base = smem + (size_t)wv * com_wave_bytes(win, TWO);
+  int* list = reinterpret_cast<int*>(base);
+  float2* tiles = reinterpret_cast<float2*>(base);  // reused after the main loop
+  float* sc = reinterpret_cast<float*>(base + (com_wave_bytes(win, TWO) - 64 * 4));
+  const int hf = lane >> 5;       // which 8-slot half of a k-chunk this lane feeds
+  const int r0 = lane & 31;       // row of block 0 (= column of block 0 as B operand)
+  const int r1 = 32 + (lane & 31);
+
+  for (long long pt = (long long)blockIdx.x * COM_WAVES + wv; pt < npts; pt += (long long)gridDim.x * COM_WAVES) {
+    int ia, ir;
+    const uint8_t* ms;
+    if (RASTER) {
+      const long long px = pix0 + pt;
+      ia = (int)(px / w);
+      ir = (int)(px - (long long)ia * w);
+      ms = shp + px * win;
+    } else {
+      ia = idx[2 * pt];
+      ir = idx[2 * pt + 1];
+      ms = shp + pt * win;
+    }
+    // ---- in-bounds SHPs, compacted in window order ---------------------------
+    int K = 0;
+    for (int t0 = 0; t0 < win; t0 += 64) {
+      const int t = t0 + lane;
+      bool v = false;
+      int lin = 0;
+      if (t < win) {
+        const int k = t / rw, l = t - (t / rw) * rw;
+        const int sa = ia - az + k, sr = ir - r + l;
+        v = ms[t] && sa >= 0 && sa < nl && sr >= 0 && sr < w;
+        lin = sa * w + sr;
+      }
+      const unsigned long long b = __ballot(v);
+      if (v) list[K + __popcll(b & ((1ull << lane) - 1ull))] = lin;
+      K += __popcll(b);
+    }
+    __builtin_amdgcn_wave_barrier();
+    // ---- per-image power-of-two scale (lane = image) --------------------------
+    float mx0 = 0.f, mx1 = 0.f;
+    if (lane < n) {
+      int k = 0;
+      for (; k + 1 < K; k += 2) {
+        const float2 a = rslc[(size_t)list[k] * n + lane];
+        const float2 c = rslc[(size_t)list[k + 1] * n + lane];
+        mx0 = fmaxf(mx0, fmaxf(fabsf(a.x), fabsf(a.y)));
+        mx1 = fmaxf(mx1, fmaxf(fabsf(c.x), fabsf(c.y)));
+      }
+      if (k < K) {
+        const float2 a = rslc[(size_t)list[k] * n + lane];
+        mx0 = fmaxf(mx0, fmaxf(fabsf(a.x), fabsf(a.y)));
+      }
+    }
+    const float mx = fmaxf(mx0, mx1);
+    float s = 1.f;
+    if (mx > 0.f && mx < INFINITY) {
+      int e;
+      frexpf(mx, &e);
+      s = ldexpf(1.f, 6 - e);  // scaled maximum in [32, 64)
+    }
+    const float s0 = __shfl(s, r0, 64);
+    const float s1 = TWO ? __shfl(s, r1 < 64 ? r1 : 0, 64) : 1.f;
+    // ---- main loop: 8 SHPs (16 k-slots) per chunk -----------------------------
+    f16x_t cr00 = {}, ci00 = {}, cr01 = {}, ci01 = {}, cr11 = {}, ci11 = {};
+    const int nch = (K + 7) >> 3;
+    float2 za[4], zb[4];  // this lane's 4 SHPs at rows r0 / r1 (prefetched one chunk ahead)
+    auto load_chunk = [&](int c, float2 (&xa)[4], float2 (&xb)[4]) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int k = c * 8 + hf * 4 + q;
+        const bool okk = k < K;
+        const size_t off = okk ? (size_t)list[k] * n : 0;
+        xa[q] = (okk && r0 < n) ? rslc[off + r0] : make_float2(0.f, 0.f);
+        if (TWO) xb[q] = (okk && r1 < n) ? rslc[off + r1] : make_float2(0.f, 0.f);
+      }
+    };
+    if (nch > 0) load_chunk(0, za, zb);
+    for (int c = 0; c < nch; ++c) {
+      float2 na[4], nb[4];
+      if (c + 1 < nch) load_chunk(c + 1, na, nb);
+      float xv[8], yv[8];
+      half8_t x0h, x0l, y0h, y0l;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        xv[2 * q] = za[q].x * s0;
+        xv[2 * q + 1] = za[q].y * s0;
+        yv[2 * q] = za[q].y * s0;
+        yv[2 * q + 1] = -za[q].x * s0;
+      }
+      split8(xv, x0h, x0l);
+      split8(yv, y0h, y0l);
+      cr00 = mfma3(x0h, x0l, x0h, x0l, cr00);
+      ci00 = mfma3(y0h, y0l, x0h, x0l, ci00);
+      if (TWO) {
+        half8_t x1h, x1l;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          xv[2 * q] = zb[q].x * s1;
+          xv[2 * q + 1] = zb[q].y * s1;
+        }
+        split8(xv, x1h, x1l);
+        cr01 = mfma3(x0h, x0l, x1h, x1l, cr01);
+        ci01 = mfma3(y0h, y0l, x1h, x1l, ci01);
+        half8_t y1h, y1l;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          yv[2 * q] = zb[q].y * s1;
+          yv[2 * q + 1] = -zb[q].x * s1;
+        }
+        split8(yv, y1h, y1l);
+        cr11 = mfma3(x1h, x1l, x1h, x1l, cr11);
+        ci11 = mfma3(y1h, y1l, x1h, x1l, ci11);
+      }
+      if (c + 1 < nch) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          za[q] = na[q];
+          zb[q] = nb[q];
+        }
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    // ---- tiles -> LDS (row-major 32x32 complex per tile) -----------------------
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int row = 8 * (q >> 2) + 4 * hf + (q & 3);
+      const int col = lane & 31;
+      tiles[row * 32 + col] = make_float2(cr00[q], ci00[q]);
+      if (TWO) {
+        tiles[1024 + row * 32 + col] = make_float2(cr01[q], ci01[q]);
+        tiles[2048 + row * 32 + col] = make_float2(cr11[q], ci11[q]);
+      }
+    }
+    sc[lane] = s;
+    __builtin_amdgcn_wave_barrier();
+    // S'(a,b) = scaled sum; element (a,b) with a <= b
+    auto tget = [&](int a, int b) -> float2 {
+      const bool sw = a > b;
+      const int lo = sw ? b : a, hi = sw ? a : b;
+      int t = 0;
+      if (TWO) t = (lo >= 32) ? 2 : (hi >= 32 ? 1 : 0);
+      const float2 v = tiles[t * 1024 + (lo & 31) * 32 + (hi & 31)];
+      return sw ? make_float2(v.x, -v.y) : v;
+    };
+    auto diag = [&](int m) -> float {
+      const int t = TWO && m >= 32 ? 2 : 0;
+      return tiles[t * 1024 + (m & 31) * 33].x;
+    };
+    if (RASTER) {
+      const float fk = (float)K;
+      for (int e = lane; e < n * n; e += 64) {
+        const int m = e / n, j = e - (e / n) * n;
+        const float2 v = tget(m, j);
+        const float den = sqrtf(diag(m) * diag(j));
+        const size_t o = (size_t)pt * n * n + e;
+        coh[o] = make_float2(v.x / den, v.y / den);
+        if (cov) {
+          const float us = 1.f / (sc[m] * sc[j]);  // exact: powers of two
+          cov[o] = make_float2(v.x * us / fk, v.y * us / fk);
+        }
+      }
+    } else {
+      for (int p = lane; p < npairs; p += 64) {
+        const int a = pairs[2 * p], b = pairs[2 * p + 1];
+        const float2 v = tget(a, b);
+        const float den = sqrtf(diag(a) * diag(b));
+        coh[(size_t)pt * npairs + p] = make_float2(v.x / den, v.y / den);
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+int co_mfma_launch(bool raster, const float2* rslc, int nl, int w, int n, const int32_t* idx, const uint8_t* shp,
+                   long long npts, int aw, int rw, long long pix0, const int32_t* pairs, int npairs, float2* cov,
+                   float2* coh, hipStream_t st) {
+  const bool two = n > 32;
+  const size_t lds = com_wave_bytes(aw * rw, two) * COM_WAVES;
+  const long long nb = std::min<long long>((npts + COM_WAVES - 1) / COM_WAVES, 1 << 20);
+  dim3 g((unsigned)nb), b(64 * COM_WAVES);
+#define MR_COM(R, T)                                                                                              \
+  hipLaunchKernelGGL((co_mfma_kernel<R, T>), g, b, lds, st, rslc, nl, w, n, idx, shp, npts, aw, rw, pix0, pairs, \
+                     npairs, cov, coh)
+  if (raster) {
+    if (two)
+      MR_COM(true, true);
+    else
+      MR_COM(true, false);
+  } else {
+    if (two)
+      MR_COM(false, true);
+    else
+      MR_COM(false, false);
+  }
+#undef MR_COM
+  return launch_status("co_mfma");
+}
+
+}  // namespace mr
