@@ -28,22 +28,35 @@ __host__ __device__ inline int com_tile_floats(bool two) { return two ? 3 * 32 *
 __host__ __device__ inline size_t com_wave_bytes(int win, bool two) {
   const size_t list = ((size_t)win * 4 + 15) & ~(size_t)15;
   const size_t tile = (size_t)com_tile_floats(two) * 4;
-  return (list > tile ? list : tile) + 64 * 4 /* scales */;
+  return (list > tile ? list : tile) + 2 * 64 * 4 /* scales, 1/sqrt(diag) */;
 }
 
-// split 8 floats into hi/lo halves (hi truncated, lo = exact remainder rounded)
-__device__ __forceinline__ void split8(const float (&v)[8], half8_t& hi, half8_t& lo) {
+typedef unsigned int u4_t __attribute__((ext_vector_type(4)));
+
+// Split 4 complex values (8 floats: re0, im0, re1, im1, ...) into FP16 hi/lo
+// halves packed as the MFMA operand: hi = RTZ(x), lo = RTZ(x - hi) (exact
+// remainder, then rounded).  Also returns Y = (im, -re) pairs derived from the
+// same halves (a swap of the two halves of each word and a sign flip), which is
+// the operand of Im S = Y X^T.
+__device__ __forceinline__ void split4c(const float (&v)[8], half8_t& xh, half8_t& xl, half8_t& yh, half8_t& yl) {
+  u4_t h, l, hy, ly;
 #pragma unroll
-  for (int i = 0; i < 8; i += 2) {
-    const half2_t h = __builtin_bit_cast(half2_t, __builtin_amdgcn_cvt_pkrtz(v[i], v[i + 1]));
-    const float r0 = v[i] - (float)h[0];
-    const float r1 = v[i + 1] - (float)h[1];
-    const half2_t l = __builtin_bit_cast(half2_t, __builtin_amdgcn_cvt_pkrtz(r0, r1));
-    hi[i] = h[0];
-    hi[i + 1] = h[1];
-    lo[i] = l[0];
-    lo[i + 1] = l[1];
+  for (int i = 0; i < 4; ++i) {
+    const half2_t hh = __builtin_bit_cast(half2_t, __builtin_amdgcn_cvt_pkrtz(v[2 * i], v[2 * i + 1]));
+    const float r0 = v[2 * i] - (float)hh[0];
+    const float r1 = v[2 * i + 1] - (float)hh[1];
+    const unsigned hw = __builtin_bit_cast(unsigned, hh);
+    const unsigned lw = __builtin_bit_cast(unsigned, __builtin_bit_cast(half2_t, __builtin_amdgcn_cvt_pkrtz(r0, r1)));
+    h[i] = hw;
+    l[i] = lw;
+    // (re, im) -> (im, -re): rotate the 16-bit halves, flip the sign of the new high half
+    hy[i] = ((hw >> 16) | (hw << 16)) ^ 0x80000000u;
+    ly[i] = ((lw >> 16) | (lw << 16)) ^ 0x80000000u;
   }
+  xh = __builtin_bit_cast(half8_t, h);
+  xl = __builtin_bit_cast(half8_t, l);
+  yh = __builtin_bit_cast(half8_t, hy);
+  yl = __builtin_bit_cast(half8_t, ly);
 }
 
 __device__ __forceinline__ f16x_t mfma3(const half8_t& ah, const half8_t& al, const half8_t& bh, const half8_t& bl,
@@ -67,12 +80,18 @@ __global__ __launch_bounds__(64 * COM_WAVES) void co_mfma_kernel(
   unsigned char* base = smem + (size_t)wv * com_wave_bytes(win, TWO);
   int* list = reinterpret_cast<int*>(base);
   float2* tiles = reinterpret_cast<float2*>(base);  // reused after the main loop
-  float* sc = reinterpret_cast<float*>(base + (com_wave_bytes(win, TWO) - 64 * 4));
+  float* sc = reinterpret_cast<float*>(base + (com_wave_bytes(win, TWO) - 2 * 64 * 4));
   const int hf = lane >> 5;       // which 8-slot half of a k-chunk this lane feeds
   const int r0 = lane & 31;       // row of block 0 (= column of block 0 as B operand)
   const int r1 = 32 + (lane & 31);
 
-  for (long long pt = (long long)blockIdx.x * COM_WAVES + wv; pt < npts; pt += (long long)gridDim.x * COM_WAVES) {
+  // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs, so
+  // block b runs on XCD b % 8.  Renumbering gives every XCD a contiguous range
+  // of points, whose overlapping SHP windows then share that XCD's L2.
+  const unsigned nbx = (gridDim.x + 7) / 8;
+  const unsigned bx = (blockIdx.x % 8) * nbx + blockIdx.x / 8;
+  if (bx >= gridDim.x) return;  // only when gridDim.x is not a multiple of 8
+  for (long long pt = (long long)bx * COM_WAVES + wv; pt < npts; pt += (long long)gridDim.x * COM_WAVES) {
     int ia, ir;
     const uint8_t* ms;
     if (RASTER) {
@@ -144,36 +163,26 @@ __global__ __launch_bounds__(64 * COM_WAVES) void co_mfma_kernel(
     for (int c = 0; c < nch; ++c) {
       float2 na[4], nb[4];
       if (c + 1 < nch) load_chunk(c + 1, na, nb);
-      float xv[8], yv[8];
+      float xv[8];
       half8_t x0h, x0l, y0h, y0l;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         xv[2 * q] = za[q].x * s0;
         xv[2 * q + 1] = za[q].y * s0;
-        yv[2 * q] = za[q].y * s0;
-        yv[2 * q + 1] = -za[q].x * s0;
       }
-      split8(xv, x0h, x0l);
-      split8(yv, y0h, y0l);
+      split4c(xv, x0h, x0l, y0h, y0l);
       cr00 = mfma3(x0h, x0l, x0h, x0l, cr00);
       ci00 = mfma3(y0h, y0l, x0h, x0l, ci00);
       if (TWO) {
-        half8_t x1h, x1l;
+        half8_t x1h, x1l, y1h, y1l;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           xv[2 * q] = zb[q].x * s1;
           xv[2 * q + 1] = zb[q].y * s1;
         }
-        split8(xv, x1h, x1l);
+        split4c(xv, x1h, x1l, y1h, y1l);
         cr01 = mfma3(x0h, x0l, x1h, x1l, cr01);
         ci01 = mfma3(y0h, y0l, x1h, x1l, ci01);
-        half8_t y1h, y1l;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          yv[2 * q] = zb[q].y * s1;
-          yv[2 * q + 1] = -zb[q].x * s1;
-        }
-        split8(yv, y1h, y1l);
         cr11 = mfma3(x1h, x1l, x1h, x1l, cr11);
         ci11 = mfma3(y1h, y1l, x1h, x1l, ci11);
       }
@@ -199,6 +208,15 @@ __global__ __launch_bounds__(64 * COM_WAVES) void co_mfma_kernel(
     }
     sc[lane] = s;
     __builtin_amdgcn_wave_barrier();
+    // 1/sqrt(S'_mm) per image, so that coh_mj = S'_mj * rs_m * rs_j
+    float* rs = sc + 64;
+    {
+      const int m = lane;
+      float dm = 0.f;
+      if (m < n) dm = tiles[(TWO && m >= 32 ? 2048 : 0) + (m & 31) * 33].x;
+      rs[lane] = 1.f / sqrtf(dm);
+    }
+    __builtin_amdgcn_wave_barrier();
     // S'(a,b) = scaled sum; element (a,b) with a <= b
     auto tget = [&](int a, int b) -> float2 {
       const bool sw = a > b;
@@ -208,18 +226,14 @@ __global__ __launch_bounds__(64 * COM_WAVES) void co_mfma_kernel(
       const float2 v = tiles[t * 1024 + (lo & 31) * 32 + (hi & 31)];
       return sw ? make_float2(v.x, -v.y) : v;
     };
-    auto diag = [&](int m) -> float {
-      const int t = TWO && m >= 32 ? 2 : 0;
-      return tiles[t * 1024 + (m & 31) * 33].x;
-    };
     if (RASTER) {
       const float fk = (float)K;
       for (int e = lane; e < n * n; e += 64) {
         const int m = e / n, j = e - (e / n) * n;
         const float2 v = tget(m, j);
-        const float den = sqrtf(diag(m) * diag(j));
+        const float f = rs[m] * rs[j];
         const size_t o = (size_t)pt * n * n + e;
-        coh[o] = make_float2(v.x / den, v.y / den);
+        coh[o] = make_float2(v.x * f, v.y * f);
         if (cov) {
           const float us = 1.f / (sc[m] * sc[j]);  // exact: powers of two
           cov[o] = make_float2(v.x * us / fk, v.y * us / fk);
@@ -229,8 +243,8 @@ __global__ __launch_bounds__(64 * COM_WAVES) void co_mfma_kernel(
       for (int p = lane; p < npairs; p += 64) {
         const int a = pairs[2 * p], b = pairs[2 * p + 1];
         const float2 v = tget(a, b);
-        const float den = sqrtf(diag(a) * diag(b));
-        coh[(size_t)pt * npairs + p] = make_float2(v.x / den, v.y / den);
+        const float f = rs[a] * rs[b];
+        coh[(size_t)pt * npairs + p] = make_float2(v.x * f, v.y * f);
       }
     }
     __builtin_amdgcn_wave_barrier();
@@ -242,7 +256,8 @@ int co_mfma_launch(bool raster, const float2* rslc, int nl, int w, int n, const 
                    float2* coh, hipStream_t st) {
   const bool two = n > 32;
   const size_t lds = com_wave_bytes(aw * rw, two) * COM_WAVES;
-  const long long nb = std::min<long long>((npts + COM_WAVES - 1) / COM_WAVES, 1 << 20);
+  // a multiple of 8 blocks keeps the XCD renumbering a bijection
+  const long long nb = (std::min<long long>((npts + COM_WAVES - 1) / COM_WAVES, 1 << 20) + 7) / 8 * 8;
   dim3 g((unsigned)nb), b(64 * COM_WAVES);
 #define MR_COM(R, T)                                                                                              \
   hipLaunchKernelGGL((co_mfma_kernel<R, T>), g, b, lds, st, rslc, nl, w, n, idx, shp, npts, aw, rw, pix0, pairs, \
