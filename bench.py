@@ -10,7 +10,9 @@ pixel taken through the whole path.  One step = one pass over the tile:
     per batch of pixels: coh = emperical_co_pc -> (ph, quality) = emi -> t = ds_temp_coh
 
 Inputs are generated on the device before timing (synthetic, seeded).  Prints
-one JSON line.  See DESIGN.md for the roofline accounting of each stage.
+one JSON line.  Algorithmic units per pixel: KS 4N + 5W + 4 bytes, coherence 4K N(N+1)
+flops, EMI (2 + 16/3) N^3 flops, temporal coherence 8P + 8N + 4 bytes (W window entries,
+K SHPs, P = N(N-1)/2 pairs).
 """
 from __future__ import annotations
 
@@ -42,6 +44,19 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--err-sample", type=int, default=2048)
     return ap.parse_args()
+
+
+def measured_traffic(kernel, args):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE/WRITE_SIZE counter
+    summary (copy-kernel calibrated), valid only for the workload it was measured on."""
+    if (args.size, args.n, args.az, args.r, args.batch) != (2500, 60, 7, 10, 1 << 18):
+        return None
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            t = json.load(f)["per_dispatch"][kernel]
+    except (OSError, KeyError, ValueError):
+        return None
+    return t["read_bytes"] + t["write_bytes"]
 
 
 def ensure_built():
@@ -96,8 +111,8 @@ def cpu_baseline(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; use --gpus 1 (multi-GPU sharding is "
-                                   "described in DESIGN.md, not built)"}))
+        print(json.dumps({"error": "single-GPU build: only --gpus 1 is supported (tile sharding "
+                                   "across GPUs is not built)"}))
         sys.exit(2)
     ensure_built()
     import numpy as np
@@ -188,7 +203,7 @@ def main():
 
     shp_num = state["shp_num"].float()
     kmean = float(shp_num.mean())
-    # algorithmic units per pixel (DESIGN.md): bytes for ks/tcoh, flops for coherence/EMI
+    # algorithmic units per pixel: bytes for ks/tcoh, flops for coherence/EMI
     ks_bytes = 4 * N + 4 * aw * rw + aw * rw + 4  # rmli in, p out, mask out, count out
     co_flops = 4.0 * kmean * N * (N + 1)  # Hermitian rank-K update (unique entries)
     co_bytes = 8 * P
@@ -208,20 +223,22 @@ def main():
     if dom == "emi":
         ach = emi_flops * (npx / nb) / (t_emi / nb) / 1e12
         roof = {"kernel": "emi_fast_kernel<64>", "bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS, "traffic": None,
+                "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
+                "traffic": measured_traffic("emi_fast_kernel<64>", args),
                 "note": "VALU-bound (FP64 sweep + FP32 Householder); peak = MI355X FP64 = FP32 non-packed vector rate"}
     elif dom == "ks_test+select_shp":
         ach = ks_bytes * npx / t_ks / 1e9
         roof = {"kernel": "ks_pairs_kernel<float>", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None}
+                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic("ks_pairs_kernel<float>", args)}
     elif dom == "emperical_co_pc":
         ach = co_flops * (npx / nb) / (t_co / nb) / 1e12
-        roof = {"kernel": "co_kernel<false>", "bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS, "traffic": None}
+        roof = {"kernel": "co_mfma_kernel<false, true>", "bound": "mfma", "achieved": ach, "peak": 2500.0,
+                "unit": "TFLOP/s", "frac": ach / 2500.0,
+                "traffic": measured_traffic("co_mfma_kernel<pc,n<=64>", args)}
     else:
         ach = tc_bytes * npx / t_tc / 1e9
         roof = {"kernel": "tcoh_kernel<true>", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None}
+                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic("tcoh_kernel<ds>", args)}
 
     # phase error of the last batch vs an FP64 solve (torch/rocSOLVER, complex128) on a sample
     coh = state["coh_last"]
