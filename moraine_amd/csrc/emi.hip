@@ -21,61 +21,12 @@
 // Fast path (n <= 64): one wave per point, lane i owns row i of the matrix in
 // VGPRs (FP64 for step 1, FP32 complex for steps 2-4); a uniform runtime
 // column index is served by a switch (scalar branch tree + v_mov), never by
-// scratch memory.  General path (n > 64): one 256-thread block per point,
-// everything FP64 in a per-block global workspace.
-#include "common.hpp"
+// scratch memory.  64 < n <= 160: the same algorithm with two lanes per row in
+// one workgroup (emi_wide.hip).  General path (n > 160): one 256-thread block
+// per point, everything FP64 in a per-block global workspace.
+#include "emi_common.hpp"
 
 namespace mr {
-
-#define MR_C8(B) C(B + 0) C(B + 1) C(B + 2) C(B + 3) C(B + 4) C(B + 5) C(B + 6) C(B + 7)
-#define MR_C64 MR_C8(0) MR_C8(8) MR_C8(16) MR_C8(24) MR_C8(32) MR_C8(40) MR_C8(48) MR_C8(56)
-
-template <typename V, int NM>
-__device__ __forceinline__ V reg_get(const V (&a)[NM], int k) {
-  V v = V(0);
-  switch (k) {
-#define C(i) \
-  case i:    \
-    if constexpr ((i) < NM) v = a[i]; \
-    break;
-    MR_C64
-#undef C
-    default:
-      break;
-  }
-  return v;
-}
-
-template <typename V, int NM>
-__device__ __forceinline__ void reg_set(V (&a)[NM], int k, V v) {
-  switch (k) {
-#define C(i) \
-  case i:    \
-    if constexpr ((i) < NM) a[i] = v; \
-    break;
-    MR_C64
-#undef C
-    default:
-      break;
-  }
-}
-
-__device__ __forceinline__ int wave_argmax(double key, int lane) {
-  int idx = lane;
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    const double ok = __shfl_xor(key, o, 64);
-    const int oi = __shfl_xor(idx, o, 64);
-    if (ok > key || (ok == key && oi < idx)) {
-      key = ok;
-      idx = oi;
-    }
-  }
-  return __builtin_amdgcn_readfirstlane(idx);
-}
-
-// index of pair (a, b), a < b, in triu order (moraine/tnet.py:15-20)
-__device__ __forceinline__ int pair_index(int a, int b, int n) { return a * (2 * n - a - 1) / 2 + (b - a - 1); }
 
 // Sturm count: number of eigenvalues of the real symmetric tridiagonal (d, e2=|e|^2) below x.
 __device__ __forceinline__ int sturm_count(const double* d, const double* e2, int n, double x, double pivmin) {
@@ -803,6 +754,8 @@ __global__ __launch_bounds__(EMI_GEN_THREADS) void emi_general_kernel(const floa
   }
 }
 
+int emi_wide_launch(const float2* coh, long long npts, int n, int ref, float2* ph, float* quality, hipStream_t st);
+
 static size_t emi_fast_lds(int n, int nm) {
   const int P = n * (n - 1) / 2;
   return (((size_t)P * 8 + 15) & ~(size_t)15) + 64 * (8 + 8 + 8) + 64;
@@ -813,7 +766,7 @@ static size_t emi_fast_lds(int n, int nm) {
 extern "C" {
 
 size_t mr_emi_workspace_bytes(long long n_pts, int n) {
-  if (n <= 64) return 0;
+  if (n <= mr::EMI_WIDE_MAX_N) return 0;
   const long long nb = n_pts < mr::EMI_GEN_BLOCKS ? n_pts : mr::EMI_GEN_BLOCKS;
   return (size_t)(nb > 0 ? nb : 1) * mr::emi_gen_slice_bytes(n);
 }
@@ -840,6 +793,7 @@ int mr_emi(const float* coh, long long n_pts, int n, int ref, float* ph, float* 
                          p, quality);
     return mr::launch_status("emi_fast");
   }
+  if (n <= mr::EMI_WIDE_MAX_N) return mr::emi_wide_launch(c, n_pts, n, ref, p, quality, st);
   const size_t need = mr_emi_workspace_bytes(n_pts, n);
   MR_CHECK_ARG(workspace != nullptr && workspace_bytes >= need, "emi: workspace %zu < %zu bytes", workspace_bytes,
                need);

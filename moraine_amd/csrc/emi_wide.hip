@@ -1,0 +1,566 @@
+// EMI phase linking for 64 < n <= 160 on gfx950: one workgroup per point, two
+// lanes per matrix row.  Same algorithm and precision split as the one-wave
+// kernel (moraine/pl.py:45-65 restated: inv(|Gamma|) by the FP64 symmetric
+// sweep with largest-|diagonal| pivoting, M = inv(|Gamma|) o Gamma, FP32
+// Householder tridiagonalisation, FP64 multisection + twisted factorisation
+// for the smallest eigenpair, back-transformation, ph = v conj(v_ref)/|.|).
+//
+// Layout: thread t holds row i = t / 2 and the interleaved columns
+// j = 2 s + (t % 2), s = 0..E-1, in VGPRs (FP64 during the sweep, FP32 complex
+// afterwards), so a workgroup of 2n lanes (3-5 waves) keeps the whole matrix in
+// registers with the per-lane footprint of the one-wave kernel at n <= 64.
+// The two halves of a row are adjacent lanes: row partial sums combine with one
+// DPP exchange; column vectors are broadcast through LDS; reductions over rows
+// are wave DPP sums plus one LDS slot per wave.  Two workgroup barriers per
+// sweep step and per Householder step, one per back-transformation step.
+#include "emi_common.hpp"
+
+namespace mr {
+
+constexpr int EW_MAX_WAVES = 8;
+
+// LDS layout for NM = 2E columns (bytes):
+//   doubles: colb[NM] td[NM] te[NM] dpl[NM] dmi[NM] zz[NM] red[2*16] misc[8]
+//   float2 : xb[NM] ub[NM] pb[NM] dph[NM] hal[NM]
+//   float  : hinv[NM] redf[2*16]
+//   int    : redi[2*16]
+__host__ __device__ inline size_t ew_lds_bytes(int nm) {
+  return (size_t)8 * (6 * nm + 32 + 8) + (size_t)8 * 5 * nm + (size_t)4 * (nm + 32) + 4 * 32;
+}
+
+template <int E>
+__global__ __launch_bounds__(64 * EW_MAX_WAVES) void emi_wide_kernel(const float2* __restrict__ coh, long long npts,
+                                                                     int n, int ref, float2* __restrict__ ph,
+                                                                     float* __restrict__ quality) {
+  constexpr int NM = 2 * E;
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* colb = reinterpret_cast<double*>(smem);
+  double* td = colb + NM;
+  double* te = td + NM;
+  double* dpl = te + NM;
+  double* dmi = dpl + NM;
+  double* zz = dmi + NM;
+  double* red = zz + NM;   // [2][16]
+  double* misc = red + 32;  // [8]
+  float2* xb = reinterpret_cast<float2*>(misc + 8);
+  float2* ub = xb + NM;
+  float2* pb = ub + NM;
+  float2* dph = pb + NM;
+  float2* hal = dph + NM;
+  float* hinv = reinterpret_cast<float*>(hal + NM);
+  float* redf = hinv + NM;  // [2][16]
+  int* redi = reinterpret_cast<int*>(redf + 32);  // [2][16]
+
+  const int W = blockDim.x >> 6;
+  const int P = n * (n - 1) / 2;
+  const int wv = threadIdx.x >> 6;
+  // column broadcast buffers read zero beyond n (rows >= n never write them)
+  for (int t = threadIdx.x; t < NM; t += blockDim.x) {
+    colb[t] = 0.0;
+    xb[t] = make_float2(0.f, 0.f);
+    ub[t] = make_float2(0.f, 0.f);
+    pb[t] = make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+
+  for (long long pt = blockIdx.x; pt < npts; pt += gridDim.x) {
+    int tid = threadIdx.x;
+    __asm__ volatile("" : "+v"(tid));  // keeps per-lane addresses from being hoisted out of the point loop
+    const int lane = tid & 63;
+    const int row = tid >> 1, c = tid & 1;
+    const bool live = row < n;
+    const float2* src = coh + pt * (long long)P;
+
+    // ---- step 1: A = |Gamma| (FP64), sweep inverse --------------------------
+    double a[E];
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      const int j = 2 * s + c;
+      double v = 0.0;
+      if (live && j < n) {
+        if (j == row) {
+          v = 1.0;
+        } else {
+          const float2 g = src[j > row ? pair_index(row, j, n) : pair_index(j, row, n)];
+          v = sqrt((double)g.x * g.x + (double)g.y * g.y);
+        }
+      }
+      a[s] = v;
+      if ((s & 7) == 7) __builtin_amdgcn_sched_barrier(0);  // bounds the loads in flight
+    }
+    const bool dlane = live && (c == (row & 1));  // this lane holds the diagonal entry of its row
+    double dg = dlane ? 1.0 : 0.0;
+    bool swept = !dlane;
+    for (int st = 0; st < n; ++st) {
+      // pivot: unswept row with the largest |diagonal| (ties -> smallest row)
+      {
+        const double key = swept ? -1.0 : fabs(dg);
+        const int bl = wargmax_d(key, lane);
+        const double bk = rl_d(key, bl);
+        if (lane == 0) {
+          red[wv] = bk;
+          redi[wv] = (wv * 64 + bl) >> 1;
+        }
+      }
+      __syncthreads();
+      double kk = red[0];
+      int k = redi[0];
+      for (int w2 = 1; w2 < W; ++w2) {
+        const double k2 = red[w2];
+        if (k2 > kk) {
+          kk = k2;
+          k = redi[w2];
+        }
+      }
+      k = min(__builtin_amdgcn_readfirstlane(k), n - 1);
+      const bool holdk = c == (k & 1);
+      const double own = reg_get<double, E>(a, k >> 1);
+      // the exchange runs with every lane active (a DPP read of a lane that is
+      // masked off returns 0), then each lane selects
+      const double oth = dpp_d<DPP_XOR1>(own);
+      const double aik = holdk ? own : oth;  // A[row][k] on both lanes of the row
+      // d-1 in place of the pivot: the rank-1 update then writes column k itself
+      // (c - (c/d)(d-1) = c/d); the pivot's own diagonal becomes 2 - 1/d, which
+      // the sweep never reads again and which is corrected when M is formed.
+      if (live && holdk) colb[row] = (row == k) ? aik - 1.0 : aik;
+      if (live && holdk && row == k) misc[0] = aik;
+      __syncthreads();
+      const double rd = rcp_d(misc[0]);
+      const double f = !live ? 0.0 : (row == k) ? (1.0 - rd) : aik * rd;
+      {
+        double cur[8], nxt[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) cur[t] = colb[2 * t + c];
+#pragma unroll
+        for (int g = 0; g < E / 8; ++g) {
+          if (g + 1 < E / 8) {
+#pragma unroll
+            for (int t = 0; t < 8; ++t) nxt[t] = colb[2 * ((g + 1) * 8 + t) + c];
+          }
+#pragma unroll
+          for (int t = 0; t < 8; ++t) a[g * 8 + t] = fma(-f, cur[t], a[g * 8 + t]);
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int t = 0; t < 8; ++t) cur[t] = nxt[t];
+        }
+      }
+      dg = fma(-f, aik, dg);
+      swept = swept || (row == k);
+    }
+    // a holds -inv(A) (diagonal entries offset by +2)
+
+    // ---- step 2: M = inv(A) o Gamma (FP32 complex) ---------------------------
+    float mr_[E], mi_[E];
+    {
+    // fresh opaque copies of row and column parity: the E column indices and
+    // pair offsets of step 1 are recomputed here instead of being kept live in
+    // VGPRs through the sweep
+    int row2 = row, c2 = c;
+    __asm__ volatile("" : "+v"(row2), "+v"(c2));
+#pragma unroll
+    for (int s = 0; s < E; ++s) {
+      const int j = 2 * s + c2;
+      float re = 0.f, im = 0.f;
+      if (live && j < n) {
+        const double ai = (j == row2) ? 2.0 - a[s] : -a[s];
+        if (j == row2) {
+          re = (float)ai;
+        } else if (j > row2) {
+          const float2 g = src[pair_index(row2, j, n)];
+          re = (float)(ai * g.x);
+          im = (float)(ai * g.y);
+        } else {
+          const float2 g = src[pair_index(j, row2, n)];
+          re = (float)(ai * g.x);
+          im = (float)(-ai * g.y);
+        }
+      }
+      mr_[s] = re;
+      mi_[s] = im;
+      if ((s & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+    }
+    }
+
+    // ---- step 3: Householder tridiagonalisation -----------------------------
+    // p = M u is formed from x before ||x|| is known:
+    //   u = inv (x - alpha e_{k+1})  =>  M u = inv (M x - alpha M[:, k+1]),
+    // so one barrier publishes x and the norm partials, the next publishes
+    // u, 2p and the u^H p partials; q = 2p - 2 (u^H p) u is formed per column.
+    float dcr = 1.f, dci = 0.f;  // running phase product D_k (uniform)
+    if (threadIdx.x == 0) dph[0] = make_float2(1.f, 0.f);
+    for (int k = 0; k + 2 < n; ++k) {
+      const bool holdk = c == (k & 1);
+      const float ownr = reg_get<float, E>(mr_, k >> 1), owni = reg_get<float, E>(mi_, k >> 1);
+      const float pxr = dpp_f<DPP_XOR1>(ownr), pxi = dpp_f<DPP_XOR1>(owni);
+      const float mrk = holdk ? ownr : pxr, mik = holdk ? owni : pxi;  // M[row][k]
+      const bool below = live && row > k;
+      const float xr = below ? mrk : 0.f, xi = below ? mik : 0.f;
+      {
+        const float ws = wsum_f(c == 0 ? xr * xr + xi * xi : 0.f);
+        if (lane == 0) redf[wv] = ws;
+      }
+      if (live && c == 0) {
+        xb[row] = make_float2(xr, xi);
+        if (row == k) misc[1] = (double)mrk;
+        if (row == k + 1) {
+          misc[2] = (double)xr;
+          misc[3] = (double)xi;
+        }
+      }
+      __syncthreads();
+      float nrm2 = redf[0];
+      for (int w2 = 1; w2 < W; ++w2) nrm2 += redf[w2];
+      const float dkk = (float)misc[1], x0r = (float)misc[2], x0i = (float)misc[3];
+      const float nx = __builtin_amdgcn_sqrtf(nrm2);
+      const float ax0 = __builtin_amdgcn_sqrtf(x0r * x0r + x0i * x0i);
+      const float rax0 = ax0 > 0.f ? __builtin_amdgcn_rcpf(ax0) : 0.f;
+      const float phr = ax0 > 0.f ? x0r * rax0 : 1.f, phi = ax0 > 0.f ? x0i * rax0 : 0.f;
+      const float alr = -phr * nx, ali = -phi * nx;  // alpha = -phase(x0) ||x||
+      const float den2 = 2.f * nx * (nx + ax0);
+      const float inv = den2 > 0.f ? __builtin_amdgcn_rsqf(den2) : 0.f;
+      {
+        const float sr = nx > 0.f ? -phr : 1.f, si = nx > 0.f ? -phi : 0.f;  // alpha/|alpha|
+        const float r = dcr * sr - dci * si, i = dcr * si + dci * sr;
+        dcr = r;
+        dci = i;
+      }
+      if (threadIdx.x == 0) {
+        td[k] = dkk;
+        te[k] = nx;
+        hal[k] = make_float2(alr, ali);
+        hinv[k] = inv;
+        dph[k + 1] = make_float2(dcr, dci);
+      }
+      // s = partial of M x over this lane's columns; m1 = M[row][k+1] (from its holder)
+      float sr = 0.f, si = 0.f;
+      {
+        float2 cu[4], nu[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) cu[t] = xb[2 * t + c];
+#pragma unroll
+        for (int g = 0; g < E / 4; ++g) {
+          const bool act = (8 * g + 7 > k) && (8 * g < n);
+          if (g + 1 < E / 4) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) nu[t] = xb[2 * ((g + 1) * 4 + t) + c];
+          }
+          if (act) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+              const int s = g * 4 + t;
+              sr = fmaf(mr_[s], cu[t].x, sr);
+              sr = fmaf(-mi_[s], cu[t].y, sr);
+              si = fmaf(mr_[s], cu[t].y, si);
+              si = fmaf(mi_[s], cu[t].x, si);
+            }
+          }
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int t = 0; t < 4; ++t) cu[t] = nu[t];
+        }
+      }
+      const bool hold1 = c == ((k + 1) & 1);
+      const float m1o_r = reg_get<float, E>(mr_, (k + 1) >> 1), m1o_i = reg_get<float, E>(mi_, (k + 1) >> 1);
+      float m1r = hold1 ? m1o_r : 0.f, m1i = hold1 ? m1o_i : 0.f;
+      sr += dpp_f<DPP_XOR1>(sr);
+      si += dpp_f<DPP_XOR1>(si);
+      m1r += dpp_f<DPP_XOR1>(m1r);
+      m1i += dpp_f<DPP_XOR1>(m1i);
+      float pr = inv * (sr - (alr * m1r - ali * m1i));
+      float pi = inv * (si - (alr * m1i + ali * m1r));
+      float ur = (row == k + 1) ? (xr - alr) : xr;
+      float ui = (row == k + 1) ? (xi - ali) : xi;
+      ur = below ? ur * inv : 0.f;
+      ui = below ? ui * inv : 0.f;
+      if (!below) {
+        pr = 0.f;
+        pi = 0.f;
+      }
+      {
+        const float ws = wsum_f(c == 0 ? ur * pr + ui * pi : 0.f);  // real(u^H p)
+        if (lane == 0) redf[16 + wv] = ws;
+      }
+      if (live && c == 0) {
+        ub[row] = make_float2(ur, ui);
+        pb[row] = make_float2(2.f * pr, 2.f * pi);
+      }
+      __syncthreads();
+      float cval = redf[16];
+      for (int w2 = 1; w2 < W; ++w2) cval += redf[16 + w2];
+      const float c2 = 2.f * cval;
+      const float qr = 2.f * pr - c2 * ur, qi = 2.f * pi - c2 * ui;
+      // M22 -= u q^H + q u^H  over this lane's active columns
+      {
+        float2 cu[4], cp[4], nu[4], np[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          cu[t] = ub[2 * t + c];
+          cp[t] = pb[2 * t + c];
+        }
+#pragma unroll
+        for (int g = 0; g < E / 4; ++g) {
+          const bool act = (8 * g + 7 > k) && (8 * g < n);
+          if (g + 1 < E / 4) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+              nu[t] = ub[2 * ((g + 1) * 4 + t) + c];
+              np[t] = pb[2 * ((g + 1) * 4 + t) + c];
+            }
+          }
+          if (act) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+              const int s = g * 4 + t;
+              const float cqr = fmaf(-c2, cu[t].x, cp[t].x);
+              const float cqi = fmaf(-c2, cu[t].y, cp[t].y);
+              float r = mr_[s], i = mi_[s];
+              r = fmaf(-ur, cqr, r);
+              r = fmaf(-ui, cqi, r);
+              i = fmaf(ur, cqi, i);
+              i = fmaf(-ui, cqr, i);
+              r = fmaf(-qr, cu[t].x, r);
+              r = fmaf(-qi, cu[t].y, r);
+              i = fmaf(qr, cu[t].y, i);
+              i = fmaf(-qi, cu[t].x, i);
+              mr_[s] = r;
+              mi_[s] = i;
+            }
+          }
+          __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            cu[t] = nu[t];
+            cp[t] = np[t];
+          }
+        }
+      }
+    }
+    // trailing 2x2 block: rows n-2, n-1
+    {
+      const int ka = n - 2, kb = n - 1;
+      const float va = reg_get<float, E>(mr_, ka >> 1);  // M[row][n-2] (real part) on its holder
+      const float vai = reg_get<float, E>(mi_, ka >> 1);
+      const float vb = reg_get<float, E>(mr_, kb >> 1);  // M[row][n-1] on its holder
+      if (row == ka && c == (ka & 1)) td[ka] = va;
+      if (row == kb && c == (kb & 1)) td[kb] = vb;
+      if (row == kb && c == (ka & 1)) {
+        const float ae = sqrtf(va * va + vai * vai);
+        const float sr = ae > 0.f ? va / ae : 1.f, si = ae > 0.f ? vai / ae : 0.f;
+        te[ka] = ae;
+        te[kb] = 0.0;
+        dph[kb] = make_float2(dcr * sr - dci * si, dcr * si + dci * sr);
+      }
+    }
+    __syncthreads();
+
+    // ---- step 4a: smallest eigenvalue of T by multisection (64 W shifts) ---
+    double lo, hi, scale;
+    {
+      double plo = 1e308, phi_ = -1e308, psc = 0.0;
+      for (int i = lane; i < n; i += 64) {
+        const double off = (i > 0 ? te[i - 1] : 0.0) + te[i];
+        plo = fmin(plo, td[i] - off);
+        phi_ = fmax(phi_, td[i] + off);
+        psc = fmax(psc, fabs(td[i]) + off);
+      }
+      lo = wmin_d(plo);
+      hi = wmax_d(phi_);
+      const double sc = wmax_d(psc);
+      scale = sc > 0.0 ? sc : 1.0;
+      lo -= 1e-14 * scale;
+      hi += 1e-14 * scale;
+    }
+    const double pivmin = 1e-300 + scale * scale * 1e-290;
+    {
+      const int G = 64 * W;
+      const double rg = 1.0 / (double)(G + 1);
+      for (int round = 0; round < 6; ++round) {
+        if (hi - lo <= 2e-15 * scale) break;
+        const int g = wv * 64 + lane;
+        const double x = lo + (hi - lo) * (double)(g + 1) * rg;
+        double q = td[0] - x;
+        int cnt = q < 0.0;
+        for (int i = 1; i < n; ++i) {
+          if (fabs(q) < pivmin) q = -pivmin;
+          double r = __builtin_amdgcn_rcp(q);
+          r = fma(r, fma(-q, r, 1.0), r);
+          q = (td[i] - x) - (te[i - 1] * te[i - 1]) * r;
+          cnt += q < 0.0;
+        }
+        const unsigned long long m = __ballot(cnt >= 1);
+        if (lane == 0) redi[(round & 1) * 16 + wv] = m ? wv * 64 + __ffsll((long long)m) - 1 : G;
+        __syncthreads();
+        int L = G;
+        for (int w2 = 0; w2 < W; ++w2) L = min(L, redi[(round & 1) * 16 + w2]);
+        const double xl = lo + (hi - lo) * (double)(L + 1) * rg;
+        const double xp = lo + (hi - lo) * (double)L * rg;  // shift L-1 (or lo itself for L = 0)
+        if (L == G) {
+          lo = lo + (hi - lo) * (double)G * rg;
+        } else {
+          if (L > 0) lo = xp;
+          hi = xl;
+        }
+      }
+    }
+    const double lam = 0.5 * (lo + hi);
+
+    // ---- step 4b: eigenvector of T by a twisted factorisation ---------------
+    const double tiny = 2.2e-16 * scale + 1e-300;
+    if (threadIdx.x == 0) {
+      double dp = td[0] - lam;
+      if (fabs(dp) < tiny) dp = tiny;
+      dpl[0] = dp;
+      for (int i = 1; i < n; ++i) {
+        dp = (td[i] - lam) - te[i - 1] * te[i - 1] / dp;
+        if (fabs(dp) < tiny) dp = tiny;
+        dpl[i] = dp;
+      }
+    } else if (threadIdx.x == 64) {
+      double dm = td[n - 1] - lam;
+      if (fabs(dm) < tiny) dm = tiny;
+      dmi[n - 1] = dm;
+      for (int i = n - 2; i >= 0; --i) {
+        dm = (td[i] - lam) - te[i] * te[i] / dm;
+        if (fabs(dm) < tiny) dm = tiny;
+        dmi[i] = dm;
+      }
+    }
+    __syncthreads();
+    int rtw;
+    {
+      const int i = tid;  // one row per thread for the twist search
+      const double gam = i < n ? fabs(dpl[i] + dmi[i] - (td[i] - lam)) : 1e308;
+      const int bl = wargmax_d(-gam, lane);
+      if (lane == 0) {
+        red[16 + wv] = -rl_d(gam, bl);
+        redi[wv] = wv * 64 + bl;
+      }
+      __syncthreads();
+      double best = red[16];
+      rtw = redi[0];
+      for (int w2 = 1; w2 < W; ++w2)
+        if (red[16 + w2] > best) {
+          best = red[16 + w2];
+          rtw = redi[w2];
+        }
+      rtw = min(__builtin_amdgcn_readfirstlane(rtw), n - 1);
+    }
+    // z_r = 1; z_i = -(e_i / D+_i) z_{i+1} below r; z_{i+1} = -(e_i / D-_{i+1}) z_i above r
+    if (threadIdx.x == 0) {
+      double z = 1.0;
+      zz[rtw] = 1.0;
+      for (int i = rtw - 1; i >= 0; --i) {
+        z *= -te[i] / dpl[i];
+        zz[i] = z;
+      }
+    } else if (threadIdx.x == 64) {
+      double z = 1.0;
+      for (int i = rtw + 1; i < n; ++i) {
+        z *= -te[i - 1] / dmi[i];
+        zz[i] = z;
+      }
+    }
+    __syncthreads();
+    double rq, znorm;
+    {
+      const int i = tid;
+      double zs = 0.0, ztz = 0.0;
+      if (i < n) {
+        const double zi = zz[i];
+        double tz = td[i] * zi;
+        if (i > 0) tz += te[i - 1] * zz[i - 1];
+        if (i + 1 < n) tz += te[i] * zz[i + 1];
+        zs = zi * zi;
+        ztz = zi * tz;
+      }
+      zs = wsum_d(zs);
+      ztz = wsum_d(ztz);
+      if (lane == 0) {
+        red[wv] = zs;
+        red[16 + wv] = ztz;
+      }
+      __syncthreads();
+      double s1 = 0.0, s2 = 0.0;
+      for (int w2 = 0; w2 < W; ++w2) {
+        s1 += red[w2];
+        s2 += red[16 + w2];
+      }
+      rq = s2 / s1;
+      znorm = 1.0 / sqrt(s1);
+    }
+
+    // ---- back-transformation v = H_0 ... H_{n-3} D z ------------------------
+    float vr = 0.f, vi = 0.f;
+    if (live) {
+      const float zi = (float)(zz[row] * znorm);
+      const float2 d = dph[row];
+      vr = d.x * zi;
+      vi = d.y * zi;
+    }
+    for (int k = n - 3; k >= 0; --k) {
+      const bool holdk = c == (k & 1);
+      const float ownr = reg_get<float, E>(mr_, k >> 1), owni = reg_get<float, E>(mi_, k >> 1);
+      const float pxr = dpp_f<DPP_XOR1>(ownr), pxi = dpp_f<DPP_XOR1>(owni);
+      const bool below = live && row > k;
+      const float2 al = hal[k];
+      const float hv = hinv[k];
+      float ar = holdk ? ownr : pxr, ai = holdk ? owni : pxi;
+      if (row == k + 1) {
+        ar -= al.x;
+        ai -= al.y;
+      }
+      ar = below ? ar * hv : 0.f;
+      ai = below ? ai * hv : 0.f;
+      const int b = k & 1;
+      {
+        const float s1 = wsum_f(c == 0 ? ar * vr + ai * vi : 0.f);
+        const float s2 = wsum_f(c == 0 ? ar * vi - ai * vr : 0.f);
+        if (lane == 0) {
+          redf[b * 16 + wv] = s1;
+          redf[b * 16 + 8 + wv] = s2;
+        }
+      }
+      __syncthreads();
+      float sr = redf[b * 16], si = redf[b * 16 + 8];
+      for (int w2 = 1; w2 < W; ++w2) {
+        sr += redf[b * 16 + w2];
+        si += redf[b * 16 + 8 + w2];
+      }
+      vr -= 2.f * (ar * sr - ai * si);
+      vi -= 2.f * (ar * si + ai * sr);
+    }
+    // ---- output ---------------------------------------------------------------
+    if (row == ref && c == 0) {
+      misc[4] = vr;
+      misc[5] = vi;
+    }
+    __syncthreads();
+    const float rr = (float)misc[4], ri = (float)misc[5];
+    if (live && c == 0) {
+      const float pr = vr * rr + vi * ri;
+      const float pim = vi * rr - vr * ri;
+      const float am = sqrtf(pr * pr + pim * pim);
+      ph[pt * n + row] = make_float2(pr / am, pim / am);
+    }
+    if (threadIdx.x == 0) quality[pt] = (float)rq;
+    __syncthreads();
+  }
+}
+
+int emi_wide_launch(const float2* coh, long long npts, int n, int ref, float2* ph, float* quality, hipStream_t st) {
+  const int threads = (2 * n + 63) / 64 * 64;
+  const int nb = (int)std::min<long long>(npts, 1 << 20);
+#define MR_EW(E)                                                                                              \
+  hipLaunchKernelGGL((emi_wide_kernel<E>), dim3(nb), dim3(threads), ew_lds_bytes(2 * (E)), st, coh, npts, n, \
+                     ref, ph, quality)
+  if (n <= 96)
+    MR_EW(48);
+  else if (n <= 128)
+    MR_EW(64);
+  else
+    MR_EW(80);
+#undef MR_EW
+  return launch_status("emi_wide");
+}
+
+}  // namespace mr

@@ -188,7 +188,7 @@ def test_uncompress_gpu():
 
 
 # ---------------------------------------------------------------------------- EMI
-@pytest.mark.parametrize("n", [2, 3, 17, 30, 32, 33, 60, 64, 65, 90, 120])
+@pytest.mark.parametrize("n", [2, 3, 17, 30, 32, 33, 60, 64, 65, 90, 96, 97, 120, 128, 129, 150, 160, 161])
 def test_emi_closure_free(n):
     mr = _mr()
     coh, exp = synth.closure_free_coh(24, n, seed=n)
@@ -201,18 +201,21 @@ def test_emi_closure_free(n):
     np.testing.assert_allclose(np.abs(ph), 1.0, atol=1e-5)
 
 
-@pytest.mark.parametrize("n", [17, 30, 60, 90])
-def test_emi_ds_vs_fp64(n):
+@pytest.mark.parametrize("n,win", [(17, (5, 7)), (30, (5, 7)), (60, (5, 7)), (90, (5, 7)), (90, (11, 15)),
+                                   (120, (11, 15)), (150, (11, 15)), (160, (11, 15))])
+def test_emi_ds_vs_fp64(n, win):
     mr = _mr()
-    z = synth.ds_stack(6, 8, n, seed=200 + n, ps_frac=0.1)
+    h, w = (6, 8) if win == (5, 7) else (12, 16)
+    z = synth.ds_stack(h, w, n, seed=200 + n, ps_frac=0.1)
     rng = np.random.default_rng(n)
-    win = (5, 7)
     # mixed SHP counts: rank-deficient (indefinite |Gamma|) and full-rank windows
     shp = rng.random((48, *win)) < rng.uniform(0.1, 1.0, (48, 1, 1))
-    shp[:, 2, 3] = True
-    idx = np.stack(np.nonzero(np.ones((6, 8), bool)), -1).astype(np.int32)
+    shp[:, win[0] // 2, win[1] // 2] = True
+    ii, jj = np.nonzero(np.ones((h, w), bool))
+    sel = np.linspace(0, ii.size - 1, 48).astype(int)
+    idx = np.stack([ii[sel], jj[sel]], -1).astype(np.int32)
     c = R.emperical_co_pc(z, idx, shp)
-    _case(f"emi DS n={n}")
+    _case(f"emi DS n={n} window={win}")
     ph, q = mr.emi(c)
     ph64, q64 = R.emi64(c)
     e = np.angle(ph * np.conj(ph64))
