@@ -21,8 +21,8 @@
 // Fast path (n <= 64): one wave per point, lane i owns row i of the matrix in
 // VGPRs (FP64 for step 1, FP32 complex for steps 2-4); a uniform runtime
 // column index is served by a switch (scalar branch tree + v_mov), never by
-// scratch memory.  64 < n <= 160: the same algorithm with two lanes per row in
-// one workgroup (emi_wide.hip).  General path (n > 160): one 256-thread block
+// scratch memory.  64 < n <= 200: the same algorithm with two lanes per row in
+// one workgroup (emi_wide.hip).  General path (n > 200): one 256-thread block
 // per point, everything FP64 in a per-block global workspace.
 #include "emi_common.hpp"
 

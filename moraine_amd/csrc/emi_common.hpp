@@ -5,7 +5,7 @@
 
 namespace mr {
 
-constexpr int EMI_WIDE_MAX_N = 160;  // largest n served by the register-resident kernels
+constexpr int EMI_WIDE_MAX_N = 200;  // largest n served by the register-resident kernels
 
 #define MR_C8(B) C(B + 0) C(B + 1) C(B + 2) C(B + 3) C(B + 4) C(B + 5) C(B + 6) C(B + 7)
 #define MR_C64 MR_C8(0) MR_C8(8) MR_C8(16) MR_C8(24) MR_C8(32) MR_C8(40) MR_C8(48) MR_C8(56)

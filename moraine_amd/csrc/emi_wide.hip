@@ -1,4 +1,4 @@
-// EMI phase linking for 64 < n <= 160 on gfx950: one workgroup per point, two
+// EMI phase linking for 64 < n <= 200 on gfx950: one workgroup per point, two
 // lanes per matrix row.  Same algorithm and precision split as the one-wave
 // kernel (moraine/pl.py:45-65 restated: inv(|Gamma|) by the FP64 symmetric
 // sweep with largest-|diagonal| pivoting, M = inv(|Gamma|) o Gamma, FP32
@@ -7,8 +7,8 @@
 //
 // Layout: thread t holds row i = t / 2 and the interleaved columns
 // j = 2 s + (t % 2), s = 0..E-1, in VGPRs (FP64 during the sweep, FP32 complex
-// afterwards), so a workgroup of 2n lanes (3-5 waves) keeps the whole matrix in
-// registers with the per-lane footprint of the one-wave kernel at n <= 64.
+// afterwards), so a workgroup of 2n lanes (3-7 waves) keeps the whole matrix in
+// registers (E = 48, 64, 80, 100 entries per lane for n <= 96, 128, 160, 200).
 // The two halves of a row are adjacent lanes: row partial sums combine with one
 // DPP exchange; column vectors are broadcast through LDS; reductions over rows
 // are wave DPP sums plus one LDS slot per wave.  Two workgroup barriers per
@@ -28,11 +28,14 @@ __host__ __device__ inline size_t ew_lds_bytes(int nm) {
   return (size_t)8 * (6 * nm + 32 + 8) + (size_t)8 * 5 * nm + (size_t)4 * (nm + 32) + 4 * 32;
 }
 
-template <int E>
+// GS / GH: columns per software-pipelined LDS group in the sweep / Householder
+// updates (smaller groups trade latency hiding for registers at large E).
+template <int E, int GS, int GH>
 __global__ __launch_bounds__(64 * EW_MAX_WAVES) void emi_wide_kernel(const float2* __restrict__ coh, long long npts,
                                                                      int n, int ref, float2* __restrict__ ph,
                                                                      float* __restrict__ quality) {
   constexpr int NM = 2 * E;
+  static_assert(E % GS == 0 && E % GH == 0, "groups must tile the row");
   extern __shared__ __align__(16) unsigned char smem[];
   double* colb = reinterpret_cast<double*>(smem);
   double* td = colb + NM;
@@ -128,20 +131,20 @@ __global__ __launch_bounds__(64 * EW_MAX_WAVES) void emi_wide_kernel(const float
       const double rd = rcp_d(misc[0]);
       const double f = !live ? 0.0 : (row == k) ? (1.0 - rd) : aik * rd;
       {
-        double cur[8], nxt[8];
+        double cur[GS], nxt[GS];
 #pragma unroll
-        for (int t = 0; t < 8; ++t) cur[t] = colb[2 * t + c];
+        for (int t = 0; t < GS; ++t) cur[t] = colb[2 * t + c];
 #pragma unroll
-        for (int g = 0; g < E / 8; ++g) {
-          if (g + 1 < E / 8) {
+        for (int g = 0; g < E / GS; ++g) {
+          if (g + 1 < E / GS) {
 #pragma unroll
-            for (int t = 0; t < 8; ++t) nxt[t] = colb[2 * ((g + 1) * 8 + t) + c];
+            for (int t = 0; t < GS; ++t) nxt[t] = colb[2 * ((g + 1) * GS + t) + c];
           }
 #pragma unroll
-          for (int t = 0; t < 8; ++t) a[g * 8 + t] = fma(-f, cur[t], a[g * 8 + t]);
+          for (int t = 0; t < GS; ++t) a[g * GS + t] = fma(-f, cur[t], a[g * GS + t]);
           __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-          for (int t = 0; t < 8; ++t) cur[t] = nxt[t];
+          for (int t = 0; t < GS; ++t) cur[t] = nxt[t];
         }
       }
       dg = fma(-f, aik, dg);
@@ -189,6 +192,11 @@ __global__ __launch_bounds__(64 * EW_MAX_WAVES) void emi_wide_kernel(const float
     float dcr = 1.f, dci = 0.f;  // running phase product D_k (uniform)
     if (threadIdx.x == 0) dph[0] = make_float2(1.f, 0.f);
     for (int k = 0; k + 2 < n; ++k) {
+      // per-step opaque copy of n: the column-group activity tests below are
+      // recomputed each step (scalar compares) instead of being hoisted out of
+      // the loop as E/GH live masks
+      int ns = n;
+      __asm__ volatile("" : "+s"(ns));
       const bool holdk = c == (k & 1);
       const float ownr = reg_get<float, E>(mr_, k >> 1), owni = reg_get<float, E>(mi_, k >> 1);
       const float pxr = dpp_f<DPP_XOR1>(ownr), pxi = dpp_f<DPP_XOR1>(owni);
@@ -234,20 +242,20 @@ __global__ __launch_bounds__(64 * EW_MAX_WAVES) void emi_wide_kernel(const float
       // s = partial of M x over this lane's columns; m1 = M[row][k+1] (from its holder)
       float sr = 0.f, si = 0.f;
       {
-        float2 cu[4], nu[4];
+        float2 cu[GH], nu[GH];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) cu[t] = xb[2 * t + c];
+        for (int t = 0; t < GH; ++t) cu[t] = xb[2 * t + c];
 #pragma unroll
-        for (int g = 0; g < E / 4; ++g) {
-          const bool act = (8 * g + 7 > k) && (8 * g < n);
-          if (g + 1 < E / 4) {
+        for (int g = 0; g < E / GH; ++g) {
+          const bool act = (2 * GH * g + 2 * GH - 1 > k) && (2 * GH * g < ns);
+          if (g + 1 < E / GH) {
 #pragma unroll
-            for (int t = 0; t < 4; ++t) nu[t] = xb[2 * ((g + 1) * 4 + t) + c];
+            for (int t = 0; t < GH; ++t) nu[t] = xb[2 * ((g + 1) * GH + t) + c];
           }
           if (act) {
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-              const int s = g * 4 + t;
+            for (int t = 0; t < GH; ++t) {
+              const int s = g * GH + t;
               sr = fmaf(mr_[s], cu[t].x, sr);
               sr = fmaf(-mi_[s], cu[t].y, sr);
               si = fmaf(mr_[s], cu[t].y, si);
@@ -256,7 +264,7 @@ __global__ __launch_bounds__(64 * EW_MAX_WAVES) void emi_wide_kernel(const float
           }
           __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-          for (int t = 0; t < 4; ++t) cu[t] = nu[t];
+          for (int t = 0; t < GH; ++t) cu[t] = nu[t];
         }
       }
       const bool hold1 = c == ((k + 1) & 1);
@@ -291,26 +299,26 @@ __global__ __launch_bounds__(64 * EW_MAX_WAVES) void emi_wide_kernel(const float
       const float qr = 2.f * pr - c2 * ur, qi = 2.f * pi - c2 * ui;
       // M22 -= u q^H + q u^H  over this lane's active columns
       {
-        float2 cu[4], cp[4], nu[4], np[4];
+        float2 cu[GH], cp[GH], nu[GH], np[GH];
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
+        for (int t = 0; t < GH; ++t) {
           cu[t] = ub[2 * t + c];
           cp[t] = pb[2 * t + c];
         }
 #pragma unroll
-        for (int g = 0; g < E / 4; ++g) {
-          const bool act = (8 * g + 7 > k) && (8 * g < n);
-          if (g + 1 < E / 4) {
+        for (int g = 0; g < E / GH; ++g) {
+          const bool act = (2 * GH * g + 2 * GH - 1 > k) && (2 * GH * g < ns);
+          if (g + 1 < E / GH) {
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-              nu[t] = ub[2 * ((g + 1) * 4 + t) + c];
-              np[t] = pb[2 * ((g + 1) * 4 + t) + c];
+            for (int t = 0; t < GH; ++t) {
+              nu[t] = ub[2 * ((g + 1) * GH + t) + c];
+              np[t] = pb[2 * ((g + 1) * GH + t) + c];
             }
           }
           if (act) {
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-              const int s = g * 4 + t;
+            for (int t = 0; t < GH; ++t) {
+              const int s = g * GH + t;
               const float cqr = fmaf(-c2, cu[t].x, cp[t].x);
               const float cqi = fmaf(-c2, cu[t].y, cp[t].y);
               float r = mr_[s], i = mi_[s];
@@ -328,7 +336,7 @@ __global__ __launch_bounds__(64 * EW_MAX_WAVES) void emi_wide_kernel(const float
           }
           __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-          for (int t = 0; t < 4; ++t) {
+          for (int t = 0; t < GH; ++t) {
             cu[t] = nu[t];
             cp[t] = np[t];
           }
@@ -550,15 +558,17 @@ __global__ __launch_bounds__(64 * EW_MAX_WAVES) void emi_wide_kernel(const float
 int emi_wide_launch(const float2* coh, long long npts, int n, int ref, float2* ph, float* quality, hipStream_t st) {
   const int threads = (2 * n + 63) / 64 * 64;
   const int nb = (int)std::min<long long>(npts, 1 << 20);
-#define MR_EW(E)                                                                                              \
-  hipLaunchKernelGGL((emi_wide_kernel<E>), dim3(nb), dim3(threads), ew_lds_bytes(2 * (E)), st, coh, npts, n, \
-                     ref, ph, quality)
+#define MR_EW(E, GS, GH)                                                                                   \
+  hipLaunchKernelGGL((emi_wide_kernel<E, GS, GH>), dim3(nb), dim3(threads), ew_lds_bytes(2 * (E)), st, coh, npts, \
+                     n, ref, ph, quality)
   if (n <= 96)
-    MR_EW(48);
+    MR_EW(48, 8, 4);
   else if (n <= 128)
-    MR_EW(64);
+    MR_EW(64, 8, 4);
+  else if (n <= 160)
+    MR_EW(80, 8, 4);
   else
-    MR_EW(80);
+    MR_EW(100, 4, 2);
 #undef MR_EW
   return launch_status("emi_wide");
 }

@@ -188,7 +188,7 @@ def test_uncompress_gpu():
 
 
 # ---------------------------------------------------------------------------- EMI
-@pytest.mark.parametrize("n", [2, 3, 17, 30, 32, 33, 60, 64, 65, 90, 96, 97, 120, 128, 129, 150, 160, 161])
+@pytest.mark.parametrize("n", [2, 3, 17, 30, 32, 33, 60, 64, 65, 90, 96, 97, 120, 128, 129, 150, 160, 161, 200, 201])
 def test_emi_closure_free(n):
     mr = _mr()
     coh, exp = synth.closure_free_coh(24, n, seed=n)
@@ -202,7 +202,8 @@ def test_emi_closure_free(n):
 
 
 @pytest.mark.parametrize("n,win", [(17, (5, 7)), (30, (5, 7)), (60, (5, 7)), (90, (5, 7)), (90, (11, 15)),
-                                   (120, (11, 15)), (150, (11, 15)), (160, (11, 15))])
+                                   (120, (11, 15)), (150, (11, 15)), (160, (11, 15)),
+                                   (200, (11, 15))])
 def test_emi_ds_vs_fp64(n, win):
     mr = _mr()
     h, w = (6, 8) if win == (5, 7) else (12, 16)
