@@ -124,6 +124,29 @@ int mr_ds_temp_coh(const float* coh, const float* ph, long long n_pts, int n, co
 int mr_temp_coh(const float* intf, const float* rslc, long long n_pts, int n, const int32_t* pairs,
                 int n_pairs, float* t_coh, mr_stream_t stream);
 
+/* ---------------------------------------------------------------------------
+ * Batched DS form.  No single reference function: it is the batch driver
+ * moraine.emperical_co_emi_temp_coh_pc (moraine/pl.py:148-168, i.e.
+ * emperical_co_pc -> emi -> ds_temp_coh) applied to independent pixels that
+ * each carry their own sample matrix instead of a window of a raster.
+ *   samples    complex64 rows; pixel id's K = n_samples[id] rows of n images
+ *              start at complex element row_off[id] (row-major K x n);
+ *   pix        (n_pix,) pixel ids of this call, all with the same n;
+ *   pairs      the full triu network (n_pairs = n(n-1)/2);
+ *   ph         complex64, pixel id's n phases at complex element ph_off[id];
+ *   quality, t_coh  float32 indexed by pixel id.
+ * n <= 64 runs one fused launch (coherence stays on chip; `blocks` sets its
+ * grid, 0 = default); n > 64 runs per chunk of `chunk` pixels with
+ * workspace >= mr_ds_batch_workspace_bytes(chunk, n).  For a sample matrix
+ * made of a pixel's SHP stacks in window order the outputs equal the drop-in
+ * composition exactly.
+ * ------------------------------------------------------------------------- */
+size_t mr_ds_batch_workspace_bytes(long long chunk, int n);
+int mr_ds_batch(const float* samples, const long long* row_off, const int32_t* n_samples,
+                const long long* ph_off, const int32_t* pix, long long n_pix, int n, int ref,
+                const int32_t* pairs, int n_pairs, float* ph, float* quality, float* t_coh, int blocks,
+                long long chunk, void* workspace, size_t workspace_bytes, mr_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

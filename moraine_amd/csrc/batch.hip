@@ -1,0 +1,157 @@
+// Batched DS form for gfx950: many independent DS pixels, each with its own
+// complex64 sample matrix (K_i rows of N images; the SHP stacks of the pixel),
+// taken through coherence -> EMI -> temporal coherence.  This is the batch
+// driver of moraine (moraine/pl.py:148-168: emperical_co_pc -> emi ->
+// ds_temp_coh per batch) over ragged per-pixel inputs, grouped by N by the
+// caller (one call per image count).
+//
+// n <= 64: one fused launch.  One wave per pixel accumulates the coherence on
+// MFMA into LDS tiles (co_mfma.hpp), runs EMI reading the tiles (emi_fast.hpp),
+// and forms the temporal coherence from the same tiles (tcoh.hpp); the
+// coherence matrix never reaches HBM.  Every stage shares its code with the
+// drop-in kernels, so for a sample matrix made of a pixel's SHP stacks in
+// window order the outputs equal emperical_co_pc -> emi -> ds_temp_coh exactly.
+// n > 64: per chunk of pixels, the VALU coherence kernel in sample-matrix mode,
+// the register-resident EMI and the temporal-coherence kernel, then a scatter
+// into the ragged outputs.
+#include "co_mfma.hpp"
+#include "emi_fast.hpp"
+#include "tcoh.hpp"
+
+namespace mr {
+
+int co_samples_launch(const float2* samples, const long long* row_off, const int32_t* n_samp, const int32_t* pix,
+                      long long pix0, long long cn, int n, const int32_t* pairs, int npairs, float2* coh,
+                      hipStream_t st);
+constexpr int BATCH_DEFAULT_BLOCKS = 1 << 15;
+
+__host__ __device__ inline size_t batch_fused_lds(bool two) {
+  return (size_t)com_tile_floats(two) * 4 + 128 * 4 /* scales */ + 64 * 8 /* colb */ + 2 * 64 * 8 /* ub, qb */;
+}
+
+template <int NM, bool TWO>
+__global__ __launch_bounds__(64) void ds_batch_fused_kernel(const float2* __restrict__ samples,
+                                                            const long long* __restrict__ row_off,
+                                                            const int32_t* __restrict__ n_samp,
+                                                            const long long* __restrict__ ph_off,
+                                                            const int32_t* __restrict__ pix, long long npix, int n,
+                                                            int ref, const int32_t* __restrict__ pairs, int npairs,
+                                                            float2* __restrict__ ph, float* __restrict__ quality,
+                                                            float* __restrict__ tcoh) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  float2* tiles = reinterpret_cast<float2*>(smem);
+  float* sc = reinterpret_cast<float*>(smem + (size_t)com_tile_floats(TWO) * 4);  // scales, 1/sqrt(diag)
+  double* colb = reinterpret_cast<double*>(sc + 128);
+  float2* ub = reinterpret_cast<float2*>(colb + 64);
+  float2* qb = ub + 64;
+  for (long long t = blockIdx.x; t < npix; t += gridDim.x) {
+    int lane = threadIdx.x;
+    __asm__ volatile("" : "+v"(lane));  // per-pixel opaque lane (see emi.hip)
+    const int id = pix[t];
+    co_mfma_point<TWO>(samples + row_off[id], RowIdentity{}, n_samp[id], n, lane, tiles, sc);
+    const CohTiles<TWO> ct{tiles, sc + 64};
+    float vr, vi;
+    double rq;
+    emi_fast_point<NM>(ct, n, lane, colb, ub, qb, vr, vi, rq);
+    const float2 pv = emi_link_phase(vr, vi, ref);
+    __builtin_amdgcn_wave_barrier();
+    ub[lane] = pv;  // linked phases for the temporal coherence
+    if (lane < n) ph[ph_off[id] + lane] = pv;
+    if (lane == 0) quality[id] = (float)rq;
+    __builtin_amdgcn_wave_barrier();
+    const float tc = tcoh_point<true>([&](int, int a, int b) { return ct(a, b); }, [&](int i) { return ub[i]; },
+                                      pairs, npairs, lane);
+    if (lane == 0) tcoh[id] = tc;
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+__global__ void ds_batch_scatter_kernel(const float2* __restrict__ ph_t, const float* __restrict__ q_t,
+                                        const float* __restrict__ t_t, const int32_t* __restrict__ pix, long long c0,
+                                        long long cn, int n, const long long* __restrict__ ph_off,
+                                        float2* __restrict__ ph, float* __restrict__ quality,
+                                        float* __restrict__ tcoh) {
+  const long long tot = cn * n;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (long long)gridDim.x * blockDim.x) {
+    const long long pt = t / n;
+    const int j = (int)(t - pt * n);
+    const int id = pix[c0 + pt];
+    ph[ph_off[id] + j] = ph_t[t];
+    if (j == 0) {
+      quality[id] = q_t[pt];
+      tcoh[id] = t_t[pt];
+    }
+  }
+}
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+}  // namespace mr
+
+extern "C" {
+
+size_t mr_ds_batch_workspace_bytes(long long chunk, int n) {
+  if (n <= 64 || chunk <= 0) return 0;
+  const long long P = (long long)n * (n - 1) / 2;
+  return mr::align256((size_t)chunk * P * 8) + mr::align256((size_t)chunk * n * 8) +
+         2 * mr::align256((size_t)chunk * 4) + mr::align256(mr_emi_workspace_bytes(chunk, n));
+}
+
+int mr_ds_batch(const float* samples, const long long* row_off, const int32_t* n_samples, const long long* ph_off,
+                const int32_t* pix, long long n_pix, int n, int ref, const int32_t* pairs, int n_pairs, float* ph,
+                float* quality, float* t_coh, int blocks, long long chunk, void* workspace, size_t workspace_bytes,
+                mr_stream_t stream) {
+  MR_CHECK_ARG(n >= 2 && n <= 1024, "ds_batch: n=%d outside [2, 1024]", n);
+  MR_CHECK_ARG(ref >= 0 && ref < n, "ds_batch: ref=%d outside [0,%d)", ref, n);
+  MR_CHECK_ARG(n_pix >= 0, "ds_batch: negative pixel count");
+  MR_CHECK_ARG(n_pairs == n * (n - 1) / 2, "ds_batch: needs the full pair network (%d pairs for n=%d, got %d)",
+               n * (n - 1) / 2, n, n_pairs);
+  if (n_pix == 0) return MR_OK;
+  MR_CHECK_ARG(samples && row_off && n_samples && ph_off && pix && pairs && ph && quality && t_coh,
+               "ds_batch: null pointer");
+  hipStream_t st = mr::as_stream(stream);
+  const float2* smp = reinterpret_cast<const float2*>(samples);
+  float2* phc = reinterpret_cast<float2*>(ph);
+  if (n <= 64) {
+    const int nb = (int)std::min<long long>(n_pix, blocks > 0 ? blocks : mr::BATCH_DEFAULT_BLOCKS);
+    if (n <= 32)
+      hipLaunchKernelGGL((mr::ds_batch_fused_kernel<32, false>), dim3(nb), dim3(64), mr::batch_fused_lds(false), st,
+                         smp, row_off, n_samples, ph_off, pix, n_pix, n, ref, pairs, n_pairs, phc, quality, t_coh);
+    else
+      hipLaunchKernelGGL((mr::ds_batch_fused_kernel<64, true>), dim3(nb), dim3(64), mr::batch_fused_lds(true), st,
+                         smp, row_off, n_samples, ph_off, pix, n_pix, n, ref, pairs, n_pairs, phc, quality, t_coh);
+    return mr::launch_status("ds_batch");
+  }
+  MR_CHECK_ARG(chunk > 0, "ds_batch: chunk must be positive for n > 64");
+  const size_t need = mr_ds_batch_workspace_bytes(chunk, n);
+  MR_CHECK_ARG(workspace && workspace_bytes >= need, "ds_batch: workspace %zu < %zu bytes", workspace_bytes, need);
+  const long long P = (long long)n * (n - 1) / 2;
+  unsigned char* w = reinterpret_cast<unsigned char*>(workspace);
+  float2* coh = reinterpret_cast<float2*>(w);
+  w += mr::align256((size_t)chunk * P * 8);
+  float2* ph_t = reinterpret_cast<float2*>(w);
+  w += mr::align256((size_t)chunk * n * 8);
+  float* q_t = reinterpret_cast<float*>(w);
+  w += mr::align256((size_t)chunk * 4);
+  float* t_t = reinterpret_cast<float*>(w);
+  w += mr::align256((size_t)chunk * 4);
+  void* emi_ws = w;
+  const size_t emi_ws_bytes = mr::align256(mr_emi_workspace_bytes(chunk, n));
+  for (long long c0 = 0; c0 < n_pix; c0 += chunk) {
+    const long long cn = std::min(chunk, n_pix - c0);
+    if (int rc = mr::co_samples_launch(smp, row_off, n_samples, pix, c0, cn, n, pairs, n_pairs, coh, st)) return rc;
+    if (int rc = mr_emi(reinterpret_cast<const float*>(coh), cn, n, ref, reinterpret_cast<float*>(ph_t), q_t, emi_ws,
+                        emi_ws_bytes, stream))
+      return rc;
+    if (int rc = mr_ds_temp_coh(reinterpret_cast<const float*>(coh), reinterpret_cast<const float*>(ph_t), cn, n,
+                                pairs, n_pairs, t_t, stream))
+      return rc;
+    const int nb = (int)std::min<long long>(mr::div_up((unsigned long long)cn * n, 256), 1 << 16);
+    hipLaunchKernelGGL(mr::ds_batch_scatter_kernel, dim3(nb), dim3(256), 0, st, ph_t, q_t, t_t, pix, c0, cn, n,
+                       ph_off, phc, quality, t_coh);
+    if (int rc = mr::launch_status("ds_batch scatter")) return rc;
+  }
+  return MR_OK;
+}
+
+}  // extern "C"

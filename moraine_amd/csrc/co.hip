@@ -21,13 +21,18 @@ constexpr int CO_QB = 8;             // pairs per thread per batch
 constexpr int CO_MAX_WIN = 4096;     // window entries
 constexpr int CO_STAGE_BYTES = 32768;
 
-template <bool RASTER>
+// SAMP: batched form, point pt is pixel pix[pix0 + pt] whose K = n_samp[id]
+// stacks are the rows of its sample matrix at rslc + row_off[id] (no window).
+template <bool RASTER, bool SAMP = false>
 __global__ __launch_bounds__(CO_THREADS) void co_kernel(const float2* __restrict__ rslc, int nl, int w, int n,
                                                         const int32_t* __restrict__ idx,
                                                         const uint8_t* __restrict__ shp, long long npts, int aw,
                                                         int rw, long long pix0, const int32_t* __restrict__ pairs,
                                                         int npairs, float2* __restrict__ cov,
-                                                        float2* __restrict__ coh) {
+                                                        float2* __restrict__ coh,
+                                                        const long long* __restrict__ row_off = nullptr,
+                                                        const int32_t* __restrict__ n_samp = nullptr,
+                                                        const int32_t* __restrict__ pix = nullptr) {
   __shared__ int list[CO_MAX_WIN];
   __shared__ int wtot[CO_THREADS / 64];
   __shared__ int s_k;
@@ -41,9 +46,14 @@ __global__ __launch_bounds__(CO_THREADS) void co_kernel(const float2* __restrict
   const int np = RASTER ? n * n : npairs;
 
   for (long long pt = blockIdx.x; pt < npts; pt += gridDim.x) {
-    int ia, ir;
-    const uint8_t* ms;
-    if (RASTER) {
+    int ia = 0, ir = 0;
+    const uint8_t* ms = nullptr;
+    const float2* base = rslc;
+    if (SAMP) {
+      const int id = pix[pix0 + pt];
+      base = rslc + row_off[id];
+      if (tid == 0) s_k = min((int)n_samp[id], CO_MAX_WIN);
+    } else if (RASTER) {
       const long long px = pix0 + pt;
       ia = (int)(px / w);
       ir = (int)(px - (long long)ia * w);
@@ -54,9 +64,9 @@ __global__ __launch_bounds__(CO_THREADS) void co_kernel(const float2* __restrict
       ms = shp + pt * win;
     }
     // ---- compact in-bounds SHPs in window order ----
-    if (tid == 0) s_k = 0;
+    if (!SAMP && tid == 0) s_k = 0;
     __syncthreads();
-    for (int t0 = 0; t0 < win; t0 += CO_THREADS) {
+    for (int t0 = 0; t0 < (SAMP ? 0 : win); t0 += CO_THREADS) {
       const int t = t0 + tid;
       bool v = false;
       int lin = 0;
@@ -111,7 +121,7 @@ __global__ __launch_bounds__(CO_THREADS) void co_kernel(const float2* __restrict
           __syncthreads();
           for (int t = tid; t < kn * n; t += CO_THREADS) {
             const int s = t / n, e = t - (t / n) * n;
-            stage[t] = rslc[(size_t)list[k0 + s] * n + e];
+            stage[t] = base[(size_t)(SAMP ? k0 + s : list[k0 + s]) * n + e];
           }
           __syncthreads();
           if (pb == 0) {
@@ -187,6 +197,16 @@ __global__ void uncompress_scatter_kernel(const float2* __restrict__ coh, long l
     out[(pt * n + a) * n + b] = c;
     out[(pt * n + b) * n + a] = make_float2(c.x, -c.y);
   }
+}
+
+int co_samples_launch(const float2* samples, const long long* row_off, const int32_t* n_samp,
+                      const int32_t* pix, long long pix0, long long cn, int n, const int32_t* pairs, int npairs,
+                      float2* coh, hipStream_t st) {
+  const int nb = (int)std::min<long long>(cn, 1 << 16);
+  const size_t lds = CO_STAGE_BYTES + (size_t)n * 4;
+  hipLaunchKernelGGL((co_kernel<false, true>), dim3(nb), dim3(CO_THREADS), lds, st, samples, 0, 0, n, nullptr,
+                     nullptr, cn, 1, 1, pix0, pairs, npairs, nullptr, coh, row_off, n_samp, pix);
+  return launch_status("ds_batch coherence");
 }
 
 static int co_check(int n, int aw, int rw) {

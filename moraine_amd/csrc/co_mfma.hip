@@ -1,70 +1,17 @@
-// Coherence estimation on MFMA (gfx950), n <= 64.
-//
-// Per point the SHP stacks z_k (k = 1..K, in the reference's window order,
-// moraine/co.py:83-111) form the Hermitian rank-K update S = sum_k z_k z_k^H.
-// With the real 2x2 embedding
-//     Re S = X X^T,  Im S = Y X^T,   X = [Re z | Im z],  Y = [Im z | -Re z]
-// both halves are real GEMMs over 2K "k-slots" (slot 2k = real part of SHP k,
-// slot 2k+1 = imaginary part).  They run on v_mfma_f32_32x32x16_f16 with every
-// operand split into FP16 hi + lo (x = hi + lo, |x - hi - lo| <= 2^-22 |x|); the
-// three products hi*hi + hi*lo + lo*hi accumulate in FP32, which gives FP32-level
-// products.  Before the split every image row is scaled by a power of two so its
-// largest component lies in [32, 64): products are exact in FP32, sums stay far
-// from overflow, and coherence is invariant under the per-image scaling
-// (cov is unscaled exactly, the factors being powers of two).
-// Output tiles: (0,0) for n <= 32; (0,0), (0,1), (1,1) for 32 < n <= 64 (the
-// (1,0) tile is the conjugate transpose of (0,1)).
-#include "common.hpp"
+// Coherence estimation on MFMA (gfx950), n <= 64: raster (moraine/co.py:117-137)
+// and point-cloud (moraine/co.py:230-266) forms.  One wave per point compacts
+// the in-bounds SHPs of its window, then runs the shared per-point MFMA
+// accumulation (co_mfma.hpp) and writes coh (and cov) from the tiles.
+#include "co_mfma.hpp"
 
 namespace mr {
 
-typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
-typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
-typedef float f16x_t __attribute__((ext_vector_type(16)));
-
 constexpr int COM_WAVES = 2;  // points per block (one per wave); 2 x ~25 KiB LDS per block
 
-__host__ __device__ inline int com_tile_floats(bool two) { return two ? 3 * 32 * 32 * 2 : 32 * 32 * 2; }
 __host__ __device__ inline size_t com_wave_bytes(int win, bool two) {
   const size_t list = ((size_t)win * 4 + 15) & ~(size_t)15;
   const size_t tile = (size_t)com_tile_floats(two) * 4;
   return (list > tile ? list : tile) + 2 * 64 * 4 /* scales, 1/sqrt(diag) */;
-}
-
-typedef unsigned int u4_t __attribute__((ext_vector_type(4)));
-
-// Split 4 complex values (8 floats: re0, im0, re1, im1, ...) into FP16 hi/lo
-// halves packed as the MFMA operand: hi = RTZ(x), lo = RTZ(x - hi) (exact
-// remainder, then rounded).  Also returns Y = (im, -re) pairs derived from the
-// same halves (a swap of the two halves of each word and a sign flip), which is
-// the operand of Im S = Y X^T.
-__device__ __forceinline__ void split4c(const float (&v)[8], half8_t& xh, half8_t& xl, half8_t& yh, half8_t& yl) {
-  u4_t h, l, hy, ly;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const half2_t hh = __builtin_bit_cast(half2_t, __builtin_amdgcn_cvt_pkrtz(v[2 * i], v[2 * i + 1]));
-    const float r0 = v[2 * i] - (float)hh[0];
-    const float r1 = v[2 * i + 1] - (float)hh[1];
-    const unsigned hw = __builtin_bit_cast(unsigned, hh);
-    const unsigned lw = __builtin_bit_cast(unsigned, __builtin_bit_cast(half2_t, __builtin_amdgcn_cvt_pkrtz(r0, r1)));
-    h[i] = hw;
-    l[i] = lw;
-    // (re, im) -> (im, -re): rotate the 16-bit halves, flip the sign of the new high half
-    hy[i] = ((hw >> 16) | (hw << 16)) ^ 0x80000000u;
-    ly[i] = ((lw >> 16) | (lw << 16)) ^ 0x80000000u;
-  }
-  xh = __builtin_bit_cast(half8_t, h);
-  xl = __builtin_bit_cast(half8_t, l);
-  yh = __builtin_bit_cast(half8_t, hy);
-  yl = __builtin_bit_cast(half8_t, ly);
-}
-
-__device__ __forceinline__ f16x_t mfma3(const half8_t& ah, const half8_t& al, const half8_t& bh, const half8_t& bl,
-                                        f16x_t c) {
-  c = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, c, 0, 0, 0);
-  c = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, c, 0, 0, 0);
-  c = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, c, 0, 0, 0);
-  return c;
 }
 
 template <bool RASTER, bool TWO>
@@ -81,9 +28,7 @@ __global__ __launch_bounds__(64 * COM_WAVES) void co_mfma_kernel(
   int* list = reinterpret_cast<int*>(base);
   float2* tiles = reinterpret_cast<float2*>(base);  // reused after the main loop
   float* sc = reinterpret_cast<float*>(base + (com_wave_bytes(win, TWO) - 2 * 64 * 4));
-  const int hf = lane >> 5;       // which 8-slot half of a k-chunk this lane feeds
-  const int r0 = lane & 31;       // row of block 0 (= column of block 0 as B operand)
-  const int r1 = 32 + (lane & 31);
+  const float* rs = sc + 64;
 
   // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs, so
   // block b runs on XCD b % 8.  Renumbering gives every XCD a contiguous range
@@ -121,116 +66,12 @@ __global__ __launch_bounds__(64 * COM_WAVES) void co_mfma_kernel(
       K += __popcll(b);
     }
     __builtin_amdgcn_wave_barrier();
-    // ---- per-image power-of-two scale (lane = image) --------------------------
-    float mx0 = 0.f, mx1 = 0.f;
-    if (lane < n) {
-      int k = 0;
-      for (; k + 1 < K; k += 2) {
-        const float2 a = rslc[(size_t)list[k] * n + lane];
-        const float2 c = rslc[(size_t)list[k + 1] * n + lane];
-        mx0 = fmaxf(mx0, fmaxf(fabsf(a.x), fabsf(a.y)));
-        mx1 = fmaxf(mx1, fmaxf(fabsf(c.x), fabsf(c.y)));
-      }
-      if (k < K) {
-        const float2 a = rslc[(size_t)list[k] * n + lane];
-        mx0 = fmaxf(mx0, fmaxf(fabsf(a.x), fabsf(a.y)));
-      }
-    }
-    const float mx = fmaxf(mx0, mx1);
-    float s = 1.f;
-    if (mx > 0.f && mx < INFINITY) {
-      int e;
-      frexpf(mx, &e);
-      s = ldexpf(1.f, 6 - e);  // scaled maximum in [32, 64)
-    }
-    const float s0 = __shfl(s, r0, 64);
-    const float s1 = TWO ? __shfl(s, r1 < 64 ? r1 : 0, 64) : 1.f;
-    // ---- main loop: 8 SHPs (16 k-slots) per chunk -----------------------------
-    f16x_t cr00 = {}, ci00 = {}, cr01 = {}, ci01 = {}, cr11 = {}, ci11 = {};
-    const int nch = (K + 7) >> 3;
-    float2 za[4], zb[4];  // this lane's 4 SHPs at rows r0 / r1 (prefetched one chunk ahead)
-    auto load_chunk = [&](int c, float2 (&xa)[4], float2 (&xb)[4]) {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int k = c * 8 + hf * 4 + q;
-        const bool okk = k < K;
-        const size_t off = okk ? (size_t)list[k] * n : 0;
-        xa[q] = (okk && r0 < n) ? rslc[off + r0] : make_float2(0.f, 0.f);
-        if (TWO) xb[q] = (okk && r1 < n) ? rslc[off + r1] : make_float2(0.f, 0.f);
-      }
-    };
-    if (nch > 0) load_chunk(0, za, zb);
-    for (int c = 0; c < nch; ++c) {
-      float2 na[4], nb[4];
-      if (c + 1 < nch) load_chunk(c + 1, na, nb);
-      float xv[8];
-      half8_t x0h, x0l, y0h, y0l;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        xv[2 * q] = za[q].x * s0;
-        xv[2 * q + 1] = za[q].y * s0;
-      }
-      split4c(xv, x0h, x0l, y0h, y0l);
-      cr00 = mfma3(x0h, x0l, x0h, x0l, cr00);
-      ci00 = mfma3(y0h, y0l, x0h, x0l, ci00);
-      if (TWO) {
-        half8_t x1h, x1l, y1h, y1l;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          xv[2 * q] = zb[q].x * s1;
-          xv[2 * q + 1] = zb[q].y * s1;
-        }
-        split4c(xv, x1h, x1l, y1h, y1l);
-        cr01 = mfma3(x0h, x0l, x1h, x1l, cr01);
-        ci01 = mfma3(y0h, y0l, x1h, x1l, ci01);
-        cr11 = mfma3(x1h, x1l, x1h, x1l, cr11);
-        ci11 = mfma3(y1h, y1l, x1h, x1l, ci11);
-      }
-      if (c + 1 < nch) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          za[q] = na[q];
-          zb[q] = nb[q];
-        }
-      }
-    }
-    __builtin_amdgcn_wave_barrier();
-    // ---- tiles -> LDS (row-major 32x32 complex per tile) -----------------------
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int row = 8 * (q >> 2) + 4 * hf + (q & 3);
-      const int col = lane & 31;
-      tiles[row * 32 + col] = make_float2(cr00[q], ci00[q]);
-      if (TWO) {
-        tiles[1024 + row * 32 + col] = make_float2(cr01[q], ci01[q]);
-        tiles[2048 + row * 32 + col] = make_float2(cr11[q], ci11[q]);
-      }
-    }
-    sc[lane] = s;
-    __builtin_amdgcn_wave_barrier();
-    // 1/sqrt(S'_mm) per image, so that coh_mj = S'_mj * rs_m * rs_j
-    float* rs = sc + 64;
-    {
-      const int m = lane;
-      float dm = 0.f;
-      if (m < n) dm = tiles[(TWO && m >= 32 ? 2048 : 0) + (m & 31) * 33].x;
-      rs[lane] = 1.f / sqrtf(dm);
-    }
-    __builtin_amdgcn_wave_barrier();
-    // S'(a,b) = scaled sum; element (a,b) with a <= b
-    auto tget = [&](int a, int b) -> float2 {
-      const bool sw = a > b;
-      const int lo = sw ? b : a, hi = sw ? a : b;
-      int t = 0;
-      if (TWO) t = (lo >= 32) ? 2 : (hi >= 32 ? 1 : 0);
-      const float2 v = tiles[t * 1024 + (lo & 31) * 32 + (hi & 31)];
-      return sw ? make_float2(v.x, -v.y) : v;
-    };
+    co_mfma_point<TWO>(rslc, RowList{list}, K, n, lane, tiles, sc);
     if (RASTER) {
       const float fk = (float)K;
       for (int e = lane; e < n * n; e += 64) {
         const int m = e / n, j = e - (e / n) * n;
-        const float2 v = tget(m, j);
+        const float2 v = co_tile_get<TWO>(tiles, m, j);
         const float f = rs[m] * rs[j];
         const size_t o = (size_t)pt * n * n + e;
         coh[o] = make_float2(v.x * f, v.y * f);
@@ -240,12 +81,8 @@ __global__ __launch_bounds__(64 * COM_WAVES) void co_mfma_kernel(
         }
       }
     } else {
-      for (int p = lane; p < npairs; p += 64) {
-        const int a = pairs[2 * p], b = pairs[2 * p + 1];
-        const float2 v = tget(a, b);
-        const float f = rs[a] * rs[b];
-        coh[(size_t)pt * npairs + p] = make_float2(v.x * f, v.y * f);
-      }
+      const CohTiles<TWO> ct{tiles, rs};
+      for (int p = lane; p < npairs; p += 64) coh[(size_t)pt * npairs + p] = ct(pairs[2 * p], pairs[2 * p + 1]);
     }
     __builtin_amdgcn_wave_barrier();
   }

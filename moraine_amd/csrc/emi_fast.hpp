@@ -1,0 +1,369 @@
+// One-wave EMI for n <= 64 (moraine/pl.py:45-65 restated; algorithm notes at
+// the top of emi.hip): shared by the EMI kernel and the fused batched kernel.
+#pragma once
+#include "emi_common.hpp"
+
+namespace mr {
+
+// One wave, n <= NM <= 64: the smallest eigenpair of M = inv(|Gamma|) o Gamma
+// for one point.  coh_at(a, b), a < b, returns the coherence of pair (a, b).
+// On return lane i holds (vr, vi) = v_i, the eigenvector before the reference
+// rotation, and rq the Rayleigh quotient (smallest eigenvalue).  colb/ub/qb:
+// 64-entry LDS buffers.
+template <int NM, class Src>
+__device__ __forceinline__ void emi_fast_point(const Src& coh_at, int n, int lane, double* colb, float2* ub,
+                                               float2* qb, float& vr, float& vi, double& rq) {
+  // ---- step 1: A = |Gamma| (row per lane, FP64), sweep inverse ----------
+  double a[NM];
+#pragma unroll
+  for (int j = 0; j < NM; ++j) {
+    double v = 0.0;
+    if (lane < n && j < n) {
+      if (j == lane) {
+        v = 1.0;
+      } else {
+        const float2 g = coh_at(j > lane ? lane : j, j > lane ? j : lane);
+        v = sqrt((double)g.x * g.x + (double)g.y * g.y);
+      }
+    }
+    a[j] = v;
+    if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+  }
+  double dg = lane < n ? 1.0 : 0.0;  // own diagonal entry, tracked through the sweep
+  bool swept = lane >= n;
+  for (int s = 0; s < n; ++s) {
+    const int k = wargmax_d(swept ? -1.0 : fabs(dg), lane);
+    const double c = reg_get<double, NM>(a, k);
+    // Broadcasting d-1 in place of the pivot d lets the rank-1 update itself
+    // write column k:  c - (c/d)(d-1) = c/d.  The pivot lane's own diagonal
+    // comes out as 2 - 1/d instead of -1/d; that entry is never read again
+    // by the sweep, so the +2 offset is removed once, when M is formed.
+    colb[lane] = (lane == k) ? c - 1.0 : c;
+    __builtin_amdgcn_wave_barrier();
+    const double dk = rl_d(c, k);
+    const double rd = rcp_d(dk);
+    const double f = (lane == k) ? (1.0 - rd) : c * rd;
+    {
+      // software-pipelined broadcast of colb: group g+1 is loaded before the
+      // FMAs of group g, so LDS latency overlaps the FP64 FMAs
+      double cur[8], nxt[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) cur[t] = colb[t];
+#pragma unroll
+      for (int g = 0; g < NM / 8; ++g) {
+        if (g + 1 < NM / 8) {
+#pragma unroll
+          for (int t = 0; t < 8; ++t) nxt[t] = colb[(g + 1) * 8 + t];
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t) a[g * 8 + t] = fma(-f, cur[t], a[g * 8 + t]);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int t = 0; t < 8; ++t) cur[t] = nxt[t];
+      }
+    }
+    dg = fma(-f, c, dg);
+    swept = swept || (lane == k);
+    __builtin_amdgcn_wave_barrier();
+  }
+  // a now holds -inv(A) row `lane` (diagonal entry offset by +2)
+
+  // ---- step 2: M = inv(A) o Gamma (FP32 complex row per lane) ------------
+  float mr_[NM], mi_[NM];
+#pragma unroll
+  for (int j = 0; j < NM; ++j) {
+    float re = 0.f, im = 0.f;
+    if (lane < n && j < n) {
+      const double ai = (j == lane) ? 2.0 - a[j] : -a[j];  // undo the sweep's +2 on the diagonal
+      if (j == lane) {
+        re = (float)ai;
+      } else if (j > lane) {
+        const float2 g = coh_at(lane, j);
+        re = (float)(ai * g.x);
+        im = (float)(ai * g.y);
+      } else {
+        const float2 g = coh_at(j, lane);
+        re = (float)(ai * g.x);
+        im = (float)(-ai * g.y);
+      }
+    }
+    mr_[j] = re;
+    mi_[j] = im;
+    if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+  }
+
+  // ---- step 3: Householder tridiagonalisation -----------------------------
+  // T is kept distributed: lane i holds d_i = T[i][i], e_i = |T[i+1][i]|, the
+  // unitary scaling D_i, and (alpha_i, 1/den_i) of reflector i.
+  double my_d = 0.0, my_e = 0.0;
+  float my_ar = 0.f, my_ai = 0.f, my_hv = 0.f;
+  float my_dr = 1.f, my_di = 0.f;
+  float dcr = 1.f, dci = 0.f;  // running phase product D_k (uniform)
+  for (int k = 0; k + 2 < n; ++k) {
+    const float xr_raw = reg_get<float, NM>(mr_, k);
+    const float xi_raw = reg_get<float, NM>(mi_, k);
+    const bool below = lane > k && lane < n;
+    const float xr = below ? xr_raw : 0.f, xi = below ? xi_raw : 0.f;
+    const float nrm2 = wsum_f(xr * xr + xi * xi);
+    const float x0r = rl_f(xr, k + 1), x0i = rl_f(xi, k + 1);
+    const float dkk = rl_f(xr_raw, k);
+    const float nx = __builtin_amdgcn_sqrtf(nrm2);
+    const float ax0 = __builtin_amdgcn_sqrtf(x0r * x0r + x0i * x0i);
+    const float rax0 = ax0 > 0.f ? __builtin_amdgcn_rcpf(ax0) : 0.f;
+    const float phr = ax0 > 0.f ? x0r * rax0 : 1.f, phi = ax0 > 0.f ? x0i * rax0 : 0.f;
+    const float alr = -phr * nx, ali = -phi * nx;  // alpha = -phase * ||x||
+    const float den2 = 2.f * nx * (nx + ax0);
+    const float inv = den2 > 0.f ? __builtin_amdgcn_rsqf(den2) : 0.f;
+    float ur = (lane == k + 1) ? (x0r - alr) : xr;
+    float ui = (lane == k + 1) ? (x0i - ali) : xi;
+    ur *= inv;
+    ui *= inv;
+    // column k keeps x (rows > k are never updated again); the reflector is
+    // rebuilt from x, alpha and 1/den during the back-transformation
+    if (lane == k) {
+      my_d = dkk;
+      my_e = nx;
+      my_ar = alr;
+      my_ai = ali;
+      my_hv = inv;
+    }
+    {
+      const float sr = nx > 0.f ? -phr : 1.f, si = nx > 0.f ? -phi : 0.f;  // alpha/|alpha|
+      const float r = dcr * sr - dci * si, i = dcr * si + dci * sr;
+      dcr = r;
+      dci = i;
+    }
+    if (lane == k + 1) {
+      my_dr = dcr;
+      my_di = dci;
+    }
+    ub[lane] = make_float2(ur, ui);
+    __builtin_amdgcn_wave_barrier();
+    // p = M22 u  (4-column groups left of the active block or beyond n are skipped)
+    float pr = 0.f, pi = 0.f;
+    {
+      float2 cu[4], nu[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) cu[t] = ub[t];
+#pragma unroll
+      for (int g = 0; g < NM / 4; ++g) {
+        const bool act = (g * 4 + 3 > k) && (g * 4 < n);
+        if (g + 1 < NM / 4) {
+#pragma unroll
+          for (int t = 0; t < 4; ++t) nu[t] = ub[(g + 1) * 4 + t];
+        }
+        if (act) {
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            const int j = g * 4 + t;
+            pr = fmaf(mr_[j], cu[t].x, pr);
+            pr = fmaf(-mi_[j], cu[t].y, pr);
+            pi = fmaf(mr_[j], cu[t].y, pi);
+            pi = fmaf(mi_[j], cu[t].x, pi);
+          }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) cu[t] = nu[t];
+      }
+    }
+    const float cval = wsum_f(ur * pr + ui * pi);  // real(u^H p)
+    const float qr = below ? 2.f * (pr - cval * ur) : 0.f;
+    const float qi = below ? 2.f * (pi - cval * ui) : 0.f;
+    qb[lane] = make_float2(qr, qi);
+    __builtin_amdgcn_wave_barrier();
+    // M22 -= u q^H + q u^H
+    {
+      float2 cu[4], cq[4], nu[4], nq[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        cu[t] = ub[t];
+        cq[t] = qb[t];
+      }
+#pragma unroll
+      for (int g = 0; g < NM / 4; ++g) {
+        const bool act = (g * 4 + 3 > k) && (g * 4 < n);
+        if (g + 1 < NM / 4) {
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            nu[t] = ub[(g + 1) * 4 + t];
+            nq[t] = qb[(g + 1) * 4 + t];
+          }
+        }
+        if (act) {
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            const int j = g * 4 + t;
+            float r = mr_[j], i = mi_[j];
+            r = fmaf(-ur, cq[t].x, r);
+            r = fmaf(-ui, cq[t].y, r);
+            i = fmaf(ur, cq[t].y, i);
+            i = fmaf(-ui, cq[t].x, i);
+            r = fmaf(-qr, cu[t].x, r);
+            r = fmaf(-qi, cu[t].y, r);
+            i = fmaf(qr, cu[t].y, i);
+            i = fmaf(-qi, cu[t].x, i);
+            mr_[j] = r;
+            mi_[j] = i;
+          }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          cu[t] = nu[t];
+          cq[t] = nq[t];
+        }
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  {
+    // trailing 2x2 block
+    const float r0 = reg_get<float, NM>(mr_, n - 2);
+    const float i0 = reg_get<float, NM>(mi_, n - 2);
+    const float r1 = reg_get<float, NM>(mr_, n - 1);
+    const float dm2 = rl_f(r0, n - 2);
+    const float dm1 = rl_f(r1, n - 1);
+    const float er = rl_f(r0, n - 1), ei = rl_f(i0, n - 1);
+    const float ae = sqrtf(er * er + ei * ei);
+    if (lane == n - 2) {
+      my_d = dm2;
+      my_e = ae;
+    }
+    if (lane == n - 1) {
+      my_d = dm1;
+      my_e = 0.0;
+      const float sr = ae > 0.f ? er / ae : 1.f, si = ae > 0.f ? ei / ae : 0.f;
+      my_dr = dcr * sr - dci * si;
+      my_di = dcr * si + dci * sr;
+    }
+  }
+  const double my_e2 = my_e * my_e;
+
+  // ---- step 4a: smallest eigenvalue of T by multisection ------------------
+  const double e_prev = dpp_d<DPP_WAVE_SHR1>(my_e);  // e_{i-1} (0 for lane 0)
+  double lo, hi, scale;
+  {
+    const double off = e_prev + my_e;
+    lo = wmin_d(lane < n ? my_d - off : 1e308);
+    hi = wmax_d(lane < n ? my_d + off : -1e308);
+    const double sc = wmax_d(lane < n ? fabs(my_d) + off : 0.0);
+    scale = sc > 0.0 ? sc : 1.0;
+    lo -= 1e-14 * scale;
+    hi += 1e-14 * scale;
+  }
+  const double pivmin = 1e-300 + scale * scale * 1e-290;
+  for (int round = 0; round < 6; ++round) {
+    if (hi - lo <= 2e-15 * scale) break;
+    const double x = lo + (hi - lo) * (double)(lane + 1) * (1.0 / 65.0);
+    // Sturm count of eigenvalues below x; d_i, e_{i-1}^2 broadcast by readlane
+    double q = rl_d(my_d, 0) - x;
+    int cnt = q < 0.0;
+    for (int i = 1; i < n; ++i) {
+      if (fabs(q) < pivmin) q = -pivmin;
+      double r = __builtin_amdgcn_rcp(q);
+      r = fma(r, fma(-q, r, 1.0), r);  // one Newton step on the reciprocal
+      q = (rl_d(my_d, i) - x) - rl_d(my_e2, i - 1) * r;
+      cnt += q < 0.0;
+    }
+    const unsigned long long m = __ballot(cnt >= 1);
+    if (m == 0ull) {
+      lo = rl_d(x, 63);
+    } else {
+      const int L = __ffsll((long long)m) - 1;
+      const double xl = rl_d(x, L);
+      const double xp = rl_d(x, L > 0 ? L - 1 : 0);
+      hi = xl;
+      if (L > 0) lo = xp;
+    }
+  }
+  const double lam = 0.5 * (lo + hi);
+
+  // ---- step 4b: eigenvector of T by a twisted factorisation ---------------
+  // D+_i = (d_i - lam) - e_{i-1}^2 / D+_{i-1},  D-_i = (d_i - lam) - e_i^2 / D-_{i+1},
+  // gamma_i = D+_i + D-_i - (d_i - lam); twist r = argmin |gamma|;
+  // z_r = 1, z_i = -(e_i / D+_i) z_{i+1} (i < r), z_{i+1} = -(e_i / D-_{i+1}) z_i (i >= r).
+  const double tiny = 2.2e-16 * scale + 1e-300;
+  double my_dp = 0.0, my_dm = 0.0, my_rhi = 0.0;
+  {
+    double dp = rl_d(my_d, 0) - lam;
+    if (fabs(dp) < tiny) dp = tiny;
+    if (lane == 0) my_dp = dp;
+    for (int i = 1; i < n; ++i) {
+      dp = (rl_d(my_d, i) - lam) - rl_d(my_e2, i - 1) / dp;
+      if (fabs(dp) < tiny) dp = tiny;
+      if (lane == i) my_dp = dp;
+    }
+    double dm = rl_d(my_d, n - 1) - lam;
+    if (fabs(dm) < tiny) dm = tiny;
+    if (lane == n - 1) my_dm = dm;
+    for (int i = n - 2; i >= 0; --i) {
+      const double ei = rl_d(my_e, i);
+      if (lane == i) my_rhi = -ei / dm;  // dm = D-_{i+1}
+      dm = (rl_d(my_d, i) - lam) - ei * ei / dm;
+      if (fabs(dm) < tiny) dm = tiny;
+      if (lane == i) my_dm = dm;
+    }
+  }
+  const double gam = lane < n ? fabs(my_dp + my_dm - (my_d - lam)) : 1e308;
+  const int rtw = wargmax_d(-gam, lane);
+  const double my_rlo = -my_e / my_dp;
+  double my_z = (lane == rtw) ? 1.0 : 0.0;
+  {
+    double z = 1.0;
+    for (int i = rtw - 1; i >= 0; --i) {
+      z *= rl_d(my_rlo, i);
+      if (lane == i) my_z = z;
+    }
+    z = 1.0;
+    for (int i = rtw + 1; i < n; ++i) {
+      z *= rl_d(my_rhi, i - 1);
+      if (lane == i) my_z = z;
+    }
+  }
+  my_z *= 1.0 / sqrt(wsum_d(my_z * my_z));
+  // Rayleigh quotient of T for the reported eigenvalue
+  const double z_prev = dpp_d<DPP_WAVE_SHR1>(my_z);
+  const double z_next = dpp_d<DPP_WAVE_SHL1>(my_z);
+  const double tz = lane < n ? my_d * my_z + e_prev * z_prev + my_e * z_next : 0.0;
+  rq = wsum_d(my_z * tz);
+
+  // ---- back-transformation v = H_0 ... H_{n-3} D z ------------------------
+  vr = 0.f;
+  vi = 0.f;
+  if (lane < n) {
+    const float zi = (float)my_z;
+    vr = my_dr * zi;
+    vi = my_di * zi;
+  }
+  for (int k = n - 3; k >= 0; --k) {
+    const float xr = reg_get<float, NM>(mr_, k);
+    const float xi = reg_get<float, NM>(mi_, k);
+    const bool below = lane > k && lane < n;
+    const float alr = rl_f(my_ar, k), ali = rl_f(my_ai, k), hv = rl_f(my_hv, k);
+    const float ar = below ? ((lane == k + 1) ? xr - alr : xr) * hv : 0.f;
+    const float ai = below ? ((lane == k + 1) ? xi - ali : xi) * hv : 0.f;
+    // s = u^H v
+    const float sr = wsum_f(ar * vr + ai * vi);
+    const float si = wsum_f(ar * vi - ai * vr);
+    vr -= 2.f * (ar * sr - ai * si);
+    vi -= 2.f * (ar * si + ai * sr);
+  }
+}
+
+// ph_i = v_i conj(v_ref) / |v_i conj(v_ref)|
+__device__ __forceinline__ float2 emi_link_phase(float vr, float vi, int ref) {
+  const float rr = rl_f(vr, ref), ri = rl_f(vi, ref);
+  const float pr = vr * rr + vi * ri;
+  const float pim = vi * rr - vr * ri;
+  const float am = sqrtf(pr * pr + pim * pim);
+  return make_float2(pr / am, pim / am);
+}
+
+struct CohPacked {  // compressed coherence row (triu pair order) in LDS
+  const float2* cs;
+  int n;
+  __device__ __forceinline__ float2 operator()(int a, int b) const { return cs[pair_index(a, b, n)]; }
+};
+
+}  // namespace mr

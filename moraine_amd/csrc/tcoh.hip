@@ -8,7 +8,7 @@
 // part), gathers the phase terms from an L1-resident N-vector, and reduces with
 // cross-lane shuffles.  Inputs are read only (the numba temp_coh normalises its
 // inputs in place; this one does not).
-#include "common.hpp"
+#include "tcoh.hpp"
 
 namespace mr {
 
@@ -22,25 +22,9 @@ __global__ __launch_bounds__(256) void tcoh_kernel(const float2* __restrict__ c,
   for (long long pt = w0; pt < npts; pt += nw) {
     const float2* cp = c + pt * npairs;
     const float2* vp = v + pt * n;
-    float sr = 0.f, si = 0.f;
-    for (int p = lane; p < npairs; p += 64) {
-      const int a = pairs[2 * p], b = pairs[2 * p + 1];
-      const float2 x = cp[p], va = vp[a], vb = vp[b];
-      // conj(va) * vb
-      float tr = va.x * vb.x + va.y * vb.y;
-      float ti = va.x * vb.y - va.y * vb.x;
-      float sc;
-      if (DS) {
-        sc = 1.f / sqrtf(x.x * x.x + x.y * x.y);
-      } else {
-        sc = 1.f / sqrtf((x.x * x.x + x.y * x.y) * (va.x * va.x + va.y * va.y) * (vb.x * vb.x + vb.y * vb.y));
-      }
-      sr += (x.x * tr - x.y * ti) * sc;
-      si += (x.x * ti + x.y * tr) * sc;
-    }
-    sr = wave_sum(sr);
-    si = wave_sum(si);
-    if (lane == 0) out[pt] = sqrtf(sr * sr + si * si) / (float)npairs;
+    const float t = tcoh_point<DS>([&](int p, int, int) { return cp[p]; }, [&](int i) { return vp[i]; }, pairs,
+                                   npairs, lane);
+    if (lane == 0) out[pt] = t;
   }
 }
 

@@ -122,3 +122,85 @@ def closure_free_coh(n_pts: int, n: int, seed: int = 0, ref: int = 0):
         e = th * np.conj(th[ref])
         exp[i] = e / np.abs(e)
     return coh.astype(np.complex64), exp
+
+
+def ds_samples(n_px: int, n_choices=(30, 60, 90, 150), k_range=(20, 315), g0_range=(0.3, 0.9),
+               tau_range=(12.0, 96.0), seed: int = 0):
+    """Per-pixel DS sample matrices for the batched form (numpy, small cases).
+
+    Pixel i: N_i drawn from ``n_choices``, K_i ~ U[k_range] (inclusive), and K_i
+    independent samples z = (L w) exp(j phi) of the exponential-decay model
+    Gamma_ik = g0 exp(-|t_i - t_k| / tau) + 0.1 with g0 ~ U[g0_range],
+    tau ~ U[tau_range] days.  Returns a list of (K_i, N_i) complex64 arrays.
+    """
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_px):
+        n = int(rng.choice(n_choices))
+        k = int(rng.integers(k_range[0], k_range[1] + 1))
+        L = np.linalg.cholesky(coherence_model(n, rng.uniform(*g0_range), rng.uniform(*tau_range)))
+        wv = (rng.standard_normal((k, n)) + 1j * rng.standard_normal((k, n))) / math.sqrt(2.0)
+        ph = _phase(np.array(rng.uniform(0.0, 10.0)), n)
+        out.append(((wv @ L.T) * np.exp(1j * ph)).astype(np.complex64))
+    return out
+
+
+def ds_samples_torch(n_px: int, n_choices=(30, 60, 90, 150), k_range=(20, 315), g0_range=(0.3, 0.9),
+                     tau_range=(12.0, 96.0), seed: int = 0, device="cuda", chunk: int = 8192):
+    """The ``ds_samples`` model generated on a torch device in ragged form.
+
+    Returns (samples, row_off, n_images, n_samples): samples is a flat complex64
+    tensor holding pixel i's (K_i, N_i) row-major matrix at complex element
+    row_off[i] (int64 tensor); n_images is a host int32 array, n_samples an
+    int32 tensor.  Pixels are stored in index order.
+    """
+    import torch
+
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    rng = np.random.default_rng(seed)
+    n_img = rng.choice(np.asarray(n_choices, np.int32), n_px).astype(np.int32)
+    n_smp = rng.integers(k_range[0], k_range[1] + 1, n_px).astype(np.int32)
+    sizes = n_img.astype(np.int64) * n_smp
+    row_off = np.zeros(n_px, np.int64)
+    row_off[1:] = np.cumsum(sizes)[:-1]
+    samples = torch.empty(int(sizes.sum()), dtype=torch.complex64, device=device)
+    for n in sorted(set(n_img.tolist())):
+        ids = np.nonzero(n_img == n)[0]
+        t = torch.arange(n, device=device, dtype=torch.float64) * DT_DAYS
+        dt = (t[:, None] - t[None, :]).abs()
+        for c0 in range(0, ids.size, chunk):
+            sel = ids[c0:c0 + chunk]
+            m = sel.size
+            kmax = int(n_smp[sel].max())
+            g0 = torch.rand(m, generator=g, device=device, dtype=torch.float64) * (g0_range[1] - g0_range[0]) + g0_range[0]
+            tau = torch.rand(m, generator=g, device=device, dtype=torch.float64) * (tau_range[1] - tau_range[0]) + tau_range[0]
+            G = g0[:, None, None] * torch.exp(-dt[None] / tau[:, None, None]) + 0.1
+            G.diagonal(dim1=1, dim2=2).fill_(1.0)
+            L = torch.linalg.cholesky(G).to(torch.float32)
+            wr = torch.randn((m, kmax, n), generator=g, device=device) / math.sqrt(2.0)
+            wi = torch.randn((m, kmax, n), generator=g, device=device) / math.sqrt(2.0)
+            zr = wr @ L.transpose(1, 2)
+            zi = wi @ L.transpose(1, 2)
+            v = torch.rand(m, generator=g, device=device) * 10.0
+            ph = (4 * math.pi / WAVELENGTH_M) * (v / 1000.0 / 365.25)[:, None] * t.float()
+            c, s = torch.cos(ph)[:, None, :], torch.sin(ph)[:, None, :]
+            z = torch.complex(zr * c - zi * s, zr * s + zi * c)
+            keep = torch.arange(kmax, device=device)[None, :] < torch.from_numpy(n_smp[sel]).to(device)[:, None]
+            rows = z[keep]  # (sum K, n) in pixel order
+            # pixels of one N group are not contiguous in `samples`: scatter row blocks
+            dst = torch.from_numpy(np.repeat(row_off[sel], n_smp[sel] * n)
+                                   + _ragged_arange(n_smp[sel] * n)).to(device)
+            samples[dst] = rows.reshape(-1)
+    return samples, torch.from_numpy(row_off).to(device), n_img, torch.from_numpy(n_smp).to(device)
+
+
+def _ragged_arange(lengths: np.ndarray) -> np.ndarray:
+    """Concatenation of arange(l) for each l in lengths (int64)."""
+    lengths = np.asarray(lengths, np.int64)
+    tot = int(lengths.sum())
+    if tot == 0:
+        return np.zeros(0, np.int64)
+    starts = np.zeros(lengths.size, np.int64)
+    starts[1:] = np.cumsum(lengths)[:-1]
+    return np.arange(tot, dtype=np.int64) - np.repeat(starts, lengths)
