@@ -164,9 +164,13 @@ class DSBatchRunner:
     after new data has been written into ``batch.samples`` in place."""
 
     def __init__(self, batch: DSBatch, ref: int = 0, tuner: AutoTuner | None = None,
-                 max_workspace_bytes: int = 8 << 30):
+                 max_workspace_bytes: int = 8 << 30, per_stage: bool = False):
         self.batch = batch
         self.ref = int(ref)
+        # per_stage: coherence, EMI and temporal coherence as separate launches
+        # through HBM for every group (the drop-in sequence), instead of the
+        # fused launch for N <= 64
+        self.per_stage = bool(per_stage)
         self.tuner = tuner
         dev = batch.samples.device
         _, ph_off = DSBatch.layout(batch.n_images, np.zeros(batch.n_px))
@@ -196,6 +200,9 @@ class DSBatchRunner:
                   _dev.ptr(self.t_coh), int(blocks), int(chunk), _dev.ptr(ws),
                   0 if ws is None else ws.numel(), _dev.stream())
 
+    def _staged(self, g: _Group) -> bool:
+        return self.per_stage or g.n > 64
+
     def _chunk_fits(self, n: int, chunk: int) -> bool:
         return _lib.query("mr_ds_batch_workspace_bytes", chunk, n) <= self.max_ws
 
@@ -205,7 +212,7 @@ class DSBatchRunner:
         dev = self.batch.samples.device
         need = 0
         for g in self.groups:
-            if g.n > 64:
+            if self._staged(g):
                 cands = [c for c in AutoTuner.CHUNKS if self._chunk_fits(g.n, c)] or [1024]
                 g.chunk = min(cands[-1], max(g.count, 1))
                 need = max(need, _lib.query("mr_ds_batch_workspace_bytes", g.chunk, g.n))
@@ -214,13 +221,13 @@ class DSBatchRunner:
             for g in self.groups:
                 if g.count == 0:
                     continue
-                if g.n <= 64:
+                if not self._staged(g):
                     g.blocks = self.tuner.choose(AutoTuner.key("fused", g.n, g.count), AutoTuner.FUSED_BLOCKS,
                                                  lambda b, g=g: self._call(g, b, 0))
                 else:
                     cands = [c for c in AutoTuner.CHUNKS if self._chunk_fits(g.n, c) and c <= g.chunk] or [g.chunk]
                     g.chunk = self.tuner.choose(AutoTuner.key("chunk", g.n, g.count), cands,
-                                                lambda c, g=g: self._call(g, 0, c))
+                                                lambda c, g=g: self._call(g, -1, c))
         self._tuned = True
 
     def launch(self):
@@ -229,7 +236,7 @@ class DSBatchRunner:
             self.tune()
         for g in self.groups:
             if g.count:
-                self._call(g, g.blocks, g.chunk)
+                self._call(g, -1 if self._staged(g) else g.blocks, g.chunk)
 
     def run(self):
         """Eager run; returns (ph, ph_off, quality, t_coh)."""

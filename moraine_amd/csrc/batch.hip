@@ -11,9 +11,12 @@
 // coherence matrix never reaches HBM.  Every stage shares its code with the
 // drop-in kernels, so for a sample matrix made of a pixel's SHP stacks in
 // window order the outputs equal emperical_co_pc -> emi -> ds_temp_coh exactly.
-// n > 64: per chunk of pixels, the VALU coherence kernel in sample-matrix mode,
-// the register-resident EMI and the temporal-coherence kernel, then a scatter
-// into the ragged outputs.
+// n > 64 (or any n with blocks < 0, the per-stage mode): per chunk of pixels,
+// the coherence kernel in sample-matrix mode (MFMA for n <= 64, VALU above),
+// EMI and the temporal-coherence kernel as separate launches through an HBM
+// coherence buffer, then a scatter into the ragged outputs.  The per-stage
+// mode is what the drop-in functions do for one group; it gives the same
+// results as the fused launch.
 #include "co_mfma.hpp"
 #include "emi_fast.hpp"
 #include "tcoh.hpp"
@@ -23,6 +26,9 @@ namespace mr {
 int co_samples_launch(const float2* samples, const long long* row_off, const int32_t* n_samp, const int32_t* pix,
                       long long pix0, long long cn, int n, const int32_t* pairs, int npairs, float2* coh,
                       hipStream_t st);
+int co_mfma_samples_launch(const float2* samples, const long long* row_off, const int32_t* n_samp,
+                           const int32_t* pix, long long pix0, long long cn, int n, const int32_t* pairs,
+                           int npairs, float2* coh, hipStream_t st);
 constexpr int BATCH_DEFAULT_BLOCKS = 1 << 15;
 
 __host__ __device__ inline size_t batch_fused_lds(bool two) {
@@ -91,7 +97,7 @@ static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 extern "C" {
 
 size_t mr_ds_batch_workspace_bytes(long long chunk, int n) {
-  if (n <= 64 || chunk <= 0) return 0;
+  if (n < 2 || chunk <= 0) return 0;
   const long long P = (long long)n * (n - 1) / 2;
   return mr::align256((size_t)chunk * P * 8) + mr::align256((size_t)chunk * n * 8) +
          2 * mr::align256((size_t)chunk * 4) + mr::align256(mr_emi_workspace_bytes(chunk, n));
@@ -112,7 +118,7 @@ int mr_ds_batch(const float* samples, const long long* row_off, const int32_t* n
   hipStream_t st = mr::as_stream(stream);
   const float2* smp = reinterpret_cast<const float2*>(samples);
   float2* phc = reinterpret_cast<float2*>(ph);
-  if (n <= 64) {
+  if (n <= 64 && blocks >= 0) {
     const int nb = (int)std::min<long long>(n_pix, blocks > 0 ? blocks : mr::BATCH_DEFAULT_BLOCKS);
     if (n <= 32)
       hipLaunchKernelGGL((mr::ds_batch_fused_kernel<32, false>), dim3(nb), dim3(64), mr::batch_fused_lds(false), st,
@@ -122,7 +128,7 @@ int mr_ds_batch(const float* samples, const long long* row_off, const int32_t* n
                          smp, row_off, n_samples, ph_off, pix, n_pix, n, ref, pairs, n_pairs, phc, quality, t_coh);
     return mr::launch_status("ds_batch");
   }
-  MR_CHECK_ARG(chunk > 0, "ds_batch: chunk must be positive for n > 64");
+  MR_CHECK_ARG(chunk > 0, "ds_batch: chunk must be positive for n > 64 or per-stage mode");
   const size_t need = mr_ds_batch_workspace_bytes(chunk, n);
   MR_CHECK_ARG(workspace && workspace_bytes >= need, "ds_batch: workspace %zu < %zu bytes", workspace_bytes, need);
   const long long P = (long long)n * (n - 1) / 2;
@@ -139,7 +145,9 @@ int mr_ds_batch(const float* samples, const long long* row_off, const int32_t* n
   const size_t emi_ws_bytes = mr::align256(mr_emi_workspace_bytes(chunk, n));
   for (long long c0 = 0; c0 < n_pix; c0 += chunk) {
     const long long cn = std::min(chunk, n_pix - c0);
-    if (int rc = mr::co_samples_launch(smp, row_off, n_samples, pix, c0, cn, n, pairs, n_pairs, coh, st)) return rc;
+    const int rc0 = n <= 64 ? mr::co_mfma_samples_launch(smp, row_off, n_samples, pix, c0, cn, n, pairs, n_pairs, coh, st)
+                            : mr::co_samples_launch(smp, row_off, n_samples, pix, c0, cn, n, pairs, n_pairs, coh, st);
+    if (rc0) return rc0;
     if (int rc = mr_emi(reinterpret_cast<const float*>(coh), cn, n, ref, reinterpret_cast<float*>(ph_t), q_t, emi_ws,
                         emi_ws_bytes, stream))
       return rc;

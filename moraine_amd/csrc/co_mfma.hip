@@ -88,6 +88,44 @@ __global__ __launch_bounds__(64 * COM_WAVES) void co_mfma_kernel(
   }
 }
 
+// Sample-matrix form (batched path, per-stage mode): point pt is pixel
+// pix[pix0 + pt] with K = n_samp[id] rows at samples + row_off[id]; writes the
+// triu coherence row of the point, as the point-cloud form does.
+template <bool TWO>
+__global__ __launch_bounds__(64 * COM_WAVES) void co_mfma_samples_kernel(
+    const float2* __restrict__ samples, const long long* __restrict__ row_off, const int32_t* __restrict__ n_samp,
+    const int32_t* __restrict__ pix, long long pix0, long long npts, int n, const int32_t* __restrict__ pairs,
+    int npairs, float2* __restrict__ coh) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const size_t wb = (size_t)com_tile_floats(TWO) * 4 + 2 * 64 * 4;
+  float2* tiles = reinterpret_cast<float2*>(smem + wv * wb);
+  float* sc = reinterpret_cast<float*>(smem + wv * wb + (size_t)com_tile_floats(TWO) * 4);
+  for (long long pt = (long long)blockIdx.x * COM_WAVES + wv; pt < npts; pt += (long long)gridDim.x * COM_WAVES) {
+    const int id = pix[pix0 + pt];
+    co_mfma_point<TWO>(samples + row_off[id], RowIdentity{}, n_samp[id], n, lane, tiles, sc);
+    const CohTiles<TWO> ct{tiles, sc + 64};
+    for (int p = lane; p < npairs; p += 64) coh[(size_t)pt * npairs + p] = ct(pairs[2 * p], pairs[2 * p + 1]);
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+int co_mfma_samples_launch(const float2* samples, const long long* row_off, const int32_t* n_samp,
+                           const int32_t* pix, long long pix0, long long cn, int n, const int32_t* pairs,
+                           int npairs, float2* coh, hipStream_t st) {
+  const bool two = n > 32;
+  const size_t lds = ((size_t)com_tile_floats(two) * 4 + 2 * 64 * 4) * COM_WAVES;
+  const long long nb = std::min<long long>((cn + COM_WAVES - 1) / COM_WAVES, 1 << 20);
+  if (two)
+    hipLaunchKernelGGL((co_mfma_samples_kernel<true>), dim3((unsigned)nb), dim3(64 * COM_WAVES), lds, st, samples,
+                       row_off, n_samp, pix, pix0, cn, n, pairs, npairs, coh);
+  else
+    hipLaunchKernelGGL((co_mfma_samples_kernel<false>), dim3((unsigned)nb), dim3(64 * COM_WAVES), lds, st, samples,
+                       row_off, n_samp, pix, pix0, cn, n, pairs, npairs, coh);
+  return launch_status("ds_batch coherence (MFMA)");
+}
+
 int co_mfma_launch(bool raster, const float2* rslc, int nl, int w, int n, const int32_t* idx, const uint8_t* shp,
                    long long npts, int aw, int rw, long long pix0, const int32_t* pairs, int npairs, float2* cov,
                    float2* coh, hipStream_t st) {

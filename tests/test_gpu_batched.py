@@ -120,6 +120,25 @@ def test_batched_launch_shapes_do_not_change_results():
     assert torch.equal(out[3].view(torch.int32), ref[3].view(torch.int32))
 
 
+def test_fused_equals_per_stage():
+    """The fused one-wave-per-pixel launch and the per-stage sequence
+    (coherence to HBM, EMI, temporal coherence) give identical outputs."""
+    import torch
+
+    from moraine_amd.batched import DSBatch, DSBatchRunner
+
+    s, ro, ni, ns = synth.ds_samples_torch(4000, n_choices=(17, 30, 45, 60, 90), k_range=(20, 315), seed=5,
+                                           device="cuda")
+    b = DSBatch.from_tensors(s, ro, ni, ns)
+    _case("ds_batch fused vs per-stage, 4000 pixels")
+    a = [x.clone() for x in DSBatchRunner(b).run()]
+    c = DSBatchRunner(b, per_stage=True).run()
+    torch.cuda.synchronize()
+    assert torch.equal(a[0].view(torch.int64), c[0].view(torch.int64))
+    assert torch.equal(a[2].view(torch.int32), c[2].view(torch.int32))
+    assert torch.equal(a[3].view(torch.int32), c[3].view(torch.int32))
+
+
 def _coh64(z):
     z = z.astype(np.complex128)
     s = z.T @ z.conj()
