@@ -15,6 +15,11 @@
 //     dist and the SHP bit are table lookups of k, so they are bit-exact with
 //     the reference CPU path (p = dtype(f64 series), dist = dtype(k/N)).
 //     Several walks are interleaved per thread (ILP) to hide LDS latency.
+//     The walk is symmetric in its two stacks, so only the window offsets
+//     with l > 0, or l = 0 and m >= 0, are walked (half the window plus the
+//     pixel itself); each result is also written, mirrored, to the partner
+//     pixel's opposite offset, and SHP counts of mirrored entries accumulate
+//     with global atomics.
 #include "common.hpp"
 
 namespace mr {
@@ -189,11 +194,16 @@ __global__ __launch_bounds__(KS_THREADS) void ks_pairs_kernel(const typename Key
     refv[c] = c < tcols ? valid[(size_t)a * w + c0 + c] : 0;
     cnt[c] = 0;
   }
-  const int npairs = tcols * rw;
-  for (int l = -az; l <= az; ++l) {
+  // The KS distance of two pixels does not depend on which one is the
+  // reference (the walk is symmetric), so each unordered pair is walked once:
+  // offsets (l, m - r) with l > 0, or l = 0 and m >= r (m = r is the pixel
+  // itself).  The result goes to this pixel's entry and, mirrored, to the
+  // partner's entry for the opposite offset; a mirrored entry whose source
+  // pixel lies outside the raster gets NaN from this block.
+  for (int l = 0; l <= az; ++l) {
     __syncthreads();
     const int sa = a + l;
-    const bool row_ok = sa >= 0 && sa < nl;
+    const bool row_ok = sa < nl;
     if (row_ok) {
       const int g0 = max(0, c0 - r);
       const int g1 = min(w, c0 + tcols + r);
@@ -207,6 +217,9 @@ __global__ __launch_bounds__(KS_THREADS) void ks_pairs_kernel(const typename Key
       for (int s = threadIdx.x; s < scols; s += KS_THREADS) secv[s] = 0;
     }
     __syncthreads();
+    const int m0 = l == 0 ? r : 0;
+    const int mc = rw - m0;
+    const int npairs = tcols * mc;
     for (int t0 = threadIdx.x; t0 < npairs; t0 += KS_THREADS * KS_ILP) {
       int ia[KS_ILP], ib[KS_ILP], ra[KS_ILP], rb[KS_ILP], km[KS_ILP], cc[KS_ILP], mm[KS_ILP];
       bool ok[KS_ILP], in[KS_ILP];
@@ -215,8 +228,8 @@ __global__ __launch_bounds__(KS_THREADS) void ks_pairs_kernel(const typename Key
       for (int q = 0; q < KS_ILP; ++q) {
         const int t = t0 + q * KS_THREADS;
         in[q] = t < npairs;
-        const int c = in[q] ? t / rw : 0;
-        const int m = in[q] ? t - c * rw : 0;
+        const int c = in[q] ? t / mc : 0;
+        const int m = in[q] ? m0 + (t - c * mc) : 0;
         cc[q] = c;
         mm[q] = m;
         ok[q] = in[q] && refv[c] && secv[c + m];
@@ -252,19 +265,44 @@ __global__ __launch_bounds__(KS_THREADS) void ks_pairs_kernel(const typename Key
 #pragma unroll
       for (int q = 0; q < KS_ILP; ++q) {
         if (!in[q]) continue;
-        const size_t o = (((size_t)a * w + c0 + cc[q]) * aw + (l + az)) * rw + mm[q];
         const int k = km[q];
-        if (p_out) p_out[o] = ok[q] ? tab.p[k] : (T)NAN;
-        if (d_out) d_out[o] = ok[q] ? tab.d[k] : (T)NAN;
+        const int col = c0 + cc[q];
+        const T pv = ok[q] ? tab.p[k] : (T)NAN;
+        const T dv = ok[q] ? tab.d[k] : (T)NAN;
         const bool s = ok[q] && ((tab.shp[k >> 5] >> (k & 31)) & 1u);
+        const size_t o = (((size_t)a * w + col) * aw + (l + az)) * rw + mm[q];
+        if (p_out) p_out[o] = pv;
+        if (d_out) d_out[o] = dv;
         if (m_out) m_out[o] = s ? 1 : 0;
         if (s) atomicAdd(&cnt[cc[q]], 1);
+        if (l == 0 && mm[q] == r) continue;  // the pixel itself
+        const int mo = rw - 1 - mm[q];      // index of the opposite offset
+        const int col2 = col + mm[q] - r;   // partner pixel (a + l, col2)
+        if (sa < nl && col2 >= 0 && col2 < w) {
+          const size_t o2 = (((size_t)sa * w + col2) * aw + (az - l)) * rw + mo;
+          if (p_out) p_out[o2] = pv;
+          if (d_out) d_out[o2] = dv;
+          if (m_out) m_out[o2] = s ? 1 : 0;
+          if (s && shp_num) atomicAdd(&shp_num[(size_t)sa * w + col2], 1);
+        }
+        const int cs = col - (mm[q] - r);  // source pixel of this pixel's opposite offset: (a - l, cs)
+        if (a - l < 0 || cs < 0 || cs >= w) {
+          const size_t o3 = (((size_t)a * w + col) * aw + (az - l)) * rw + mo;
+          if (p_out) p_out[o3] = (T)NAN;
+          if (d_out) d_out[o3] = (T)NAN;
+          if (m_out) m_out[o3] = 0;
+        }
       }
     }
   }
   __syncthreads();
   if (shp_num)
-    for (int c = threadIdx.x; c < tcols; c += KS_THREADS) shp_num[(size_t)a * w + c0 + c] = cnt[c];
+    for (int c = threadIdx.x; c < tcols; c += KS_THREADS) atomicAdd(&shp_num[(size_t)a * w + c0 + c], cnt[c]);
+}
+
+__global__ void fill_i32_kernel(int32_t* __restrict__ x, long long n, int32_t v) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    x[i] = v;
 }
 
 // select_shp on an existing p array: one wave per pixel.
@@ -331,6 +369,11 @@ static int ks_test_impl(const T* rmli, int nl, int w, int n, int az, int r, doub
   }
   KsTables<T> tab;
   build_tables<T>(n, p_max, tab);
+  if (shp_num) {  // counts accumulate with atomics (a fill kernel, not a memset: safe under graph capture)
+    hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)std::min<size_t>(div_up(npx, 256), 65536)), dim3(256), 0, st,
+                       shp_num, (long long)npx, 0);
+    if (int rc2 = launch_status("ks_test count init")) return rc2;
+  }
   dim3 grid(div_up(w, tw), nl);
   hipLaunchKernelGGL((ks_pairs_kernel<T>), grid, dim3(KS_THREADS), lds, st, keys, valid, nl, w, n, az, r, tw, tab,
                      p_out, d_out, m_out, shp_num);
