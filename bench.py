@@ -43,6 +43,11 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=160, help="CPU baseline: interior sub-tile edge")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--err-sample", type=int, default=2048)
+    ap.add_argument("--forms", default="plain,batched,serving",
+                    help="comma list: plain (drop-in pipeline, the headline), batched, serving")
+    ap.add_argument("--batched-px", type=int, default=200000, help="pixels of the ragged batch")
+    ap.add_argument("--serve-requests", type=int, default=16)
+    ap.add_argument("--serve-edge", type=int, default=512, help="largest tile edge of the request stream")
     return ap.parse_args()
 
 
@@ -104,6 +109,91 @@ def cpu_baseline(args):
         "value": npx / dt, "unit": "DS pixels/s", "cores": cores, "kind": "port",
         "sample": f"{s}x{s} interior pixels (+{az}/{r} halo) of the N={n}, {2*az+1}x{2*r+1}, {int(args.ps*100)}% PS "
                   f"workload: C/OpenMP KS+select+coherence+tcoh, numpy LAPACK EMI (f32 inv, c64 eigh); {dt:.1f} s",
+    }
+
+
+# ---------------------------------------------------------------------------- other forms
+def batched_leg(args, torch):
+    """Ragged batch: N in {30, 60, 90, 150}, K in [20, 315], one sample matrix per
+    pixel.  Fused + autotuned + HIP-graph replay against the per-stage launches
+    (coherence through HBM, EMI, temporal coherence per image-count group)."""
+    from moraine_amd import synth
+    from moraine_amd.batched import AutoTuner, DSBatch, DSBatchRunner
+
+    s, ro, ni, ns = synth.ds_samples_torch(args.batched_px, seed=0, device="cuda")
+    b = DSBatch.from_tensors(s, ro, ni, ns)
+    torch.cuda.synchronize()
+
+    def best(fn, reps=3):
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return min(ts)
+
+    tuner = AutoTuner()
+    fused = DSBatchRunner(b, tuner=tuner)
+    fused.capture()
+    t_graph = best(fused.replay)
+    per_stage = DSBatchRunner(b, per_stage=True)
+    t_stage = best(per_stage.run)
+    groups = {}
+    for g in fused.groups:
+        groups[str(g.n)] = {"pixels": g.count, "s": best(lambda g=g: fused._call(g, -1 if fused._staged(g) else
+                                                                               g.blocks, g.chunk), 2)}
+    # phase error vs FP64 (complex128 coherence, inverse and eigh) on a sample per group
+    ph, ph_off, q, _ = fused.replay()
+    torch.cuda.synchronize()
+    off = ph_off.cpu().numpy()
+    rms_all = []
+    import numpy as np
+
+    for g in fused.groups:
+        ids = g.pix[:64].cpu().numpy()
+        for i in ids:
+            k, n = int(ns[i]), int(ni[i])
+            Z = s[int(ro[i]):int(ro[i]) + k * n].view(k, n).to(torch.complex128)
+            S = Z.T @ Z.conj()
+            d = torch.sqrt(torch.diagonal(S).real)
+            G = S / torch.outer(d, d)
+            M = torch.linalg.inv(G.abs()).to(torch.complex128) * G
+            _, V = torch.linalg.eigh(M)
+            v = V[:, 0] * V[0, 0].conj()
+            e = torch.angle(ph[off[i]:off[i] + n].to(torch.complex128) * (v / v.abs()).conj())
+            rms_all.append(float(torch.sqrt(torch.mean(e ** 2))))
+    rms_all = np.array(rms_all)
+    return {
+        "workload": f"{args.batched_px} pixels, N in {{30,60,90,150}}, K in [20,315], gamma0~U[0.3,0.9], "
+                    f"tau~U[12,96] d",
+        "value": args.batched_px / t_graph, "unit": "DS pixels/s", "s_per_batch": t_graph,
+        "per_stage_value": args.batched_px / t_stage, "fused_vs_per_stage": t_stage / t_graph,
+        "tuner_picks": tuner.picks, "groups": groups,
+        "phase_rms_vs_fp64": {"median": float(np.median(rms_all)), "p99": float(np.quantile(rms_all, 0.99)),
+                              "max": float(rms_all.max()), "pixels": int(rms_all.size)},
+    }
+
+
+def serving_leg(args, torch):
+    """Poisson stream (50 req/s) of patch requests, N in {30, 60, 120, 200}, square
+    tiles 128..serve_edge, windows 11x15..21x41, through the continuous-batching
+    server and through the plain functions one request at a time."""
+    from moraine_amd.serve import DSServer, run_serial, run_stream, stream_specs, stream_stats
+
+    warm = stream_specs(4, seed=99, edge_range=(64, 64))
+    run_stream(warm, DSServer(keep_outputs=False))
+    run_serial(warm)
+    specs = stream_specs(args.serve_requests, rate=50.0, seed=11, edge_range=(128, args.serve_edge))
+    srv = run_stream([type(x)(**x.__dict__) for x in specs], DSServer(keep_outputs=False))
+    ser = run_serial(specs)
+    a, b = stream_stats(srv, specs), stream_stats(ser, specs)
+    return {
+        "workload": f"{args.serve_requests} requests at 50 req/s, N in {{30,60,120,200}}, tiles 128^2.."
+                    f"{args.serve_edge}^2, windows 11x15..21x41, seed per request",
+        "served": a, "serial_plain": b, "throughput_vs_serial": a["px_per_s"] / b["px_per_s"],
     }
 
 
@@ -256,6 +346,16 @@ def main():
              "rms_max_rad": float(rms[~torch.isnan(rms)].max()), "points": m}
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(args)
+    forms = {}
+    wanted = {f.strip() for f in args.forms.split(",") if f.strip()}
+    del state, rslc, idx, ph, q, tc, coh
+    torch.cuda.empty_cache()
+    if "batched" in wanted:
+        forms["batched"] = batched_leg(args, torch)
+        torch.cuda.empty_cache()
+    if "serving" in wanted:
+        forms["serving"] = serving_leg(args, torch)
+        torch.cuda.empty_cache()
     value = npx / dt
     out = {
         "metric": "DS pixels/s (KS+coh+EMI+tcoh) at N=60; p50/p99 latency; phase err vs FP64",
@@ -265,12 +365,14 @@ def main():
         "config": {"workload": f"plain drop-in pipeline, N={N}, {S}x{S} tile, {aw}x{rw} window, "
                                f"{int(args.ps*100)}% PS, p_max=0.05, all pixels", "pixels": npx,
                    "n_images": N, "window": [aw, rw], "batch": args.batch},
-        "latency_ms": {"p50": float(np.percentile(times, 50) * 1e3), "p99": float(np.percentile(times, 99) * 1e3)},
+        "latency_ms": {"p50": float(np.percentile(times, 50) * 1e3), "p99": float(np.percentile(times, 99) * 1e3),
+                       "of": "plain-form tile steps; request latency of the serving form is under forms.serving"},
         "phase_err_vs_fp64": phase,
         "mean_shp_count": kmean,
         "stages": stages,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "forms": forms,
     }
     print(json.dumps(out))
 

@@ -8,10 +8,12 @@ namespace mr {
 
 constexpr int COM_WAVES = 2;  // points per block (one per wave); 2 x ~25 KiB LDS per block
 
-__host__ __device__ inline size_t com_wave_bytes(int win, bool two) {
+// LDS per wave: the window list (while accumulating), then one 32x32 complex
+// tile at a time for the output pass, plus scales and 1/sqrt(diag).
+__host__ __device__ inline size_t com_wave_bytes(int win, bool) {
   const size_t list = ((size_t)win * 4 + 15) & ~(size_t)15;
-  const size_t tile = (size_t)com_tile_floats(two) * 4;
-  return (list > tile ? list : tile) + 2 * 64 * 4 /* scales, 1/sqrt(diag) */;
+  const size_t tile = 32 * 32 * 8;
+  return (list > tile ? list : tile) + 2 * 64 * 4;
 }
 
 template <bool RASTER, bool TWO>
@@ -66,25 +68,45 @@ __global__ __launch_bounds__(64 * COM_WAVES) void co_mfma_kernel(
       K += __popcll(b);
     }
     __builtin_amdgcn_wave_barrier();
-    co_mfma_point<TWO>(rslc, RowList{list}, K, n, lane, tiles, sc);
-    if (RASTER) {
-      const float fk = (float)K;
-      for (int e = lane; e < n * n; e += 64) {
-        const int m = e / n, j = e - (e / n) * n;
-        const float2 v = co_tile_get<TWO>(tiles, m, j);
-        const float f = rs[m] * rs[j];
-        const size_t o = (size_t)pt * n * n + e;
-        coh[o] = make_float2(v.x * f, v.y * f);
-        if (cov) {
-          const float us = 1.f / (sc[m] * sc[j]);  // exact: powers of two
-          cov[o] = make_float2(v.x * us / fk, v.y * us / fk);
+    const CoAcc acc = co_mfma_accumulate<TWO>(rslc, RowList{list}, K, n, lane);
+    __builtin_amdgcn_wave_barrier();  // the tile below overwrites the list
+    co_store_scales<TWO>(acc, n, lane, sc);
+    // output pass, one upper tile at a time: (0,0), then (0,1) and (1,1)
+#pragma unroll
+    for (int t = 0; t < (TWO ? 3 : 1); ++t) {
+      co_store_tile(tiles, t == 0 ? acc.r00 : t == 1 ? acc.r01 : acc.r11, t == 0 ? acc.i00 : t == 1 ? acc.i01 : acc.i11,
+                    lane);
+      __builtin_amdgcn_wave_barrier();
+      auto in_tile = [&](int lo, int hi) { return !TWO || ((lo >= 32) ? 2 : (hi >= 32 ? 1 : 0)) == t; };
+      if (RASTER) {
+        const float fk = (float)K;
+        for (int e = lane; e < n * n; e += 64) {
+          const int m = e / n, j = e - (e / n) * n;
+          const int lo = min(m, j), hi = max(m, j);
+          if (!in_tile(lo, hi)) continue;
+          float2 v = tiles[(lo & 31) * 32 + (hi & 31)];
+          if (m > j) v.y = -v.y;
+          const float f = rs[m] * rs[j];
+          const size_t o = (size_t)pt * n * n + e;
+          coh[o] = make_float2(v.x * f, v.y * f);
+          if (cov) {
+            const float us = 1.f / (sc[m] * sc[j]);  // exact: powers of two
+            cov[o] = make_float2(v.x * us / fk, v.y * us / fk);
+          }
+        }
+      } else {
+        for (int p = lane; p < npairs; p += 64) {
+          const int a = pairs[2 * p], b = pairs[2 * p + 1];
+          const int lo = min(a, b), hi = max(a, b);
+          if (!in_tile(lo, hi)) continue;
+          float2 v = tiles[(lo & 31) * 32 + (hi & 31)];
+          if (a > b) v.y = -v.y;
+          const float f = rs[a] * rs[b];
+          coh[(size_t)pt * npairs + p] = make_float2(v.x * f, v.y * f);
         }
       }
-    } else {
-      const CohTiles<TWO> ct{tiles, rs};
-      for (int p = lane; p < npairs; p += 64) coh[(size_t)pt * npairs + p] = ct(pairs[2 * p], pairs[2 * p + 1]);
+      __builtin_amdgcn_wave_barrier();
     }
-    __builtin_amdgcn_wave_barrier();
   }
 }
 

@@ -72,13 +72,17 @@ __device__ __forceinline__ f16x_t mfma3(const half8_t& ah, const half8_t& al, co
   return c;
 }
 
-// One wave: S' = sum_k (D z_k)(D z_k)^H with D the per-image power-of-two
-// scaling.  On return (after a wave barrier) `tiles` holds the upper tiles of
-// S' (row-major 32x32 complex each), sc[0..64) the scales and sc[64..128)
-// 1/sqrt(S'_mm), so that coh_mj = S'_mj sc[64+m] sc[64+j].
+// Accumulators of one wave: the upper tiles of S' = sum_k (D z_k)(D z_k)^H
+// (real and imaginary parts of tiles (0,0), (0,1), (1,1)), D the per-image
+// power-of-two scaling; s = this lane's image scale (lane = image).
+struct CoAcc {
+  f16x_t r00, i00, r01, i01, r11, i11;
+  float s;
+};
+
 template <bool TWO, class Row>
-__device__ __forceinline__ void co_mfma_point(const float2* __restrict__ base, const Row& row, int K, int n, int lane,
-                                              float2* tiles, float* sc) {
+__device__ __forceinline__ CoAcc co_mfma_accumulate(const float2* __restrict__ base, const Row& row, int K, int n,
+                                                    int lane) {
   const int hf = lane >> 5;  // which 8-slot half of a k-chunk this lane feeds
   const int r0 = lane & 31;  // row of block 0 (= column of block 0 as B operand)
   const int r1 = 32 + (lane & 31);
@@ -155,28 +159,48 @@ __device__ __forceinline__ void co_mfma_point(const float2* __restrict__ base, c
       }
     }
   }
-  __builtin_amdgcn_wave_barrier();
-  // ---- tiles -> LDS (row-major 32x32 complex per tile) -----------------------
+  return CoAcc{cr00, ci00, cr01, ci01, cr11, ci11, s};
+}
+
+// Accumulator layout of v_mfma_f32_32x32x16: lane l, register q holds row
+// 8 (q / 4) + 4 (l / 32) + q % 4, column l % 32 of the tile.
+__device__ __forceinline__ void co_store_tile(float2* tile, const f16x_t& re, const f16x_t& im, int lane) {
+  const int hf = lane >> 5;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) tile[(8 * (q >> 2) + 4 * hf + (q & 3)) * 32 + (lane & 31)] = make_float2(re[q], im[q]);
+}
+
+// sc[0..64) = scales, sc[64..128) = 1/sqrt(S'_mm), taken from the diagonal
+// entries in the accumulators (no tile in LDS needed).
+template <bool TWO>
+__device__ __forceinline__ void co_store_scales(const CoAcc& acc, int n, int lane, float* sc) {
+  sc[lane] = acc.s;
+  const int hf = lane >> 5, c = lane & 31;
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
-    const int rw = 8 * (q >> 2) + 4 * hf + (q & 3);
-    const int col = lane & 31;
-    tiles[rw * 32 + col] = make_float2(cr00[q], ci00[q]);
-    if (TWO) {
-      tiles[1024 + rw * 32 + col] = make_float2(cr01[q], ci01[q]);
-      tiles[2048 + rw * 32 + col] = make_float2(cr11[q], ci11[q]);
+    if (8 * (q >> 2) + 4 * hf + (q & 3) == c) {
+      sc[64 + c] = c < n ? 1.f / sqrtf(acc.r00[q]) : 1.f / sqrtf(0.f);
+      if (TWO) sc[96 + c] = 32 + c < n ? 1.f / sqrtf(acc.r11[q]) : 1.f / sqrtf(0.f);
     }
   }
-  sc[lane] = s;
+  if (!TWO && lane >= 32) sc[64 + lane] = 1.f / sqrtf(0.f);
   __builtin_amdgcn_wave_barrier();
-  // 1/sqrt(S'_mm) per image, so that coh_mj = S'_mj * rs_m * rs_j
-  {
-    const int m = lane;
-    float dm = 0.f;
-    if (m < n) dm = tiles[(TWO && m >= 32 ? 2048 : 0) + (m & 31) * 33].x;
-    sc[64 + lane] = 1.f / sqrtf(dm);
+}
+
+// One wave: accumulate, then (after a wave barrier) `tiles` holds the upper
+// tiles of S' (row-major 32x32 complex each), sc[0..64) the scales and
+// sc[64..128) 1/sqrt(S'_mm), so that coh_mj = S'_mj sc[64+m] sc[64+j].
+template <bool TWO, class Row>
+__device__ __forceinline__ void co_mfma_point(const float2* __restrict__ base, const Row& row, int K, int n, int lane,
+                                              float2* tiles, float* sc) {
+  const CoAcc acc = co_mfma_accumulate<TWO>(base, row, K, n, lane);
+  __builtin_amdgcn_wave_barrier();
+  co_store_tile(tiles, acc.r00, acc.i00, lane);
+  if (TWO) {
+    co_store_tile(tiles + 1024, acc.r01, acc.i01, lane);
+    co_store_tile(tiles + 2048, acc.r11, acc.i11, lane);
   }
-  __builtin_amdgcn_wave_barrier();
+  co_store_scales<TWO>(acc, n, lane, sc);
 }
 
 // S'(a, b) from the upper tiles (conjugated for a > b)

@@ -204,3 +204,30 @@ def _ragged_arange(lengths: np.ndarray) -> np.ndarray:
     starts = np.zeros(lengths.size, np.int64)
     starts[1:] = np.cumsum(lengths)[:-1]
     return np.arange(tot, dtype=np.int64) - np.repeat(starts, lengths)
+
+
+def ds_stack_mixed(nl: int, w: int, n: int, seed: int = 0, g0_range=(0.2, 0.9), near_singular_frac: float = 0.02,
+                   tau: float = 24.0):
+    """(nl, w, n) complex64 stack with per-pixel coherence levels (precision form).
+
+    Pixel coherence Gamma_ik = g0 exp(-|t_i - t_k| / tau) + 0.1 with g0 ~ U[g0_range];
+    a fraction ``near_singular_frac`` of pixels is near-singular instead:
+    Gamma_ik = g0 with g0 ~ U[0.98, 0.999] for all i != k (almost rank one).
+    Returns (stack, near_singular bool (nl, w)).
+    """
+    rng = np.random.default_rng(seed)
+    ns = rng.random((nl, w)) < near_singular_frac
+    z = np.empty((nl, w, n), np.complex128)
+    v = rng.uniform(0.0, 10.0, (nl, w))
+    ph = _phase(v, n)
+    for i in range(nl):
+        for j in range(w):
+            if ns[i, j]:
+                G = np.full((n, n), rng.uniform(0.98, 0.999))
+                np.fill_diagonal(G, 1.0)
+            else:
+                G = coherence_model(n, rng.uniform(*g0_range), tau)
+            L = np.linalg.cholesky(G)
+            wv = (rng.standard_normal(n) + 1j * rng.standard_normal(n)) / math.sqrt(2.0)
+            z[i, j] = (L @ wv) * np.exp(1j * ph[i, j])
+    return z.astype(np.complex64), ns

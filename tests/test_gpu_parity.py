@@ -291,3 +291,29 @@ def test_emi_on_ks_selected_shps():
     ph64, q64 = R.emi64(coh)
     rms = np.sqrt(np.nanmean(np.angle(ph * np.conj(ph64)) ** 2, axis=1))
     assert np.nanmax(rms) < 1e-3, np.nanmax(rms)
+
+
+def test_precision_form_n120_large_window_mixed_coherence():
+    """Precision-form conditions at test size: N=120, 21x41 windows, gamma0 ~ U[0.2, 0.9]
+    plus near-singular pixels.  Coherence within 1e-5 relative of FP64 on the same SHP
+    masks; EMI within 1e-3 rad RMS of FP64 EMI on the same coherence (per-pixel errors
+    printed, near-singular pixels listed separately)."""
+    mr = _mr()
+    z, ns = synth.ds_stack_mixed(14, 20, 120, seed=31, near_singular_frac=0.1)
+    rmli = (np.abs(z) ** 2).astype(np.float32)
+    _case("precision form N=120 14x20 window 21x41")
+    _, is_shp, shp_num = mr.ks_test_select_shp(rmli, 10, 20, 0.05, return_p=False)
+    idx = np.stack(np.nonzero(np.ones((14, 20), bool)), -1).astype(np.int32)
+    pc = is_shp.reshape(-1, 21, 41)
+    c = mr.emperical_co_pc(z, idx, pc)
+    c64 = R.emperical_co_pc64(z, idx, pc)
+    scale = np.abs(c64).max(axis=1, keepdims=True)
+    assert np.nanmax(np.abs(c - c64) / scale) < 1e-5
+    ph, q = mr.emi(c)
+    ph64, q64 = R.emi64(c.astype(np.complex128))
+    rms = np.sqrt(np.nanmean(np.angle(ph * np.conj(ph64)) ** 2, axis=1))
+    nsf = ns.ravel()
+    print(f"phase RMS vs FP64: regular max {np.nanmax(rms[~nsf]):.2e} median {np.nanmedian(rms[~nsf]):.2e}; "
+          f"near-singular ({nsf.sum()}) max {np.nanmax(rms[nsf]) if nsf.any() else 0:.2e}; "
+          f"SHP count {shp_num.min()}..{shp_num.max()}")
+    assert np.nanmax(rms) < 1e-3
