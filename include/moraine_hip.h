@@ -135,10 +135,10 @@ int mr_temp_coh(const float* intf, const float* rslc, long long n_pts, int n, co
  *   pairs      the full triu network (n_pairs = n(n-1)/2);
  *   ph         complex64, pixel id's n phases at complex element ph_off[id];
  *   quality, t_coh  float32 indexed by pixel id.
- * n <= 64 runs one fused launch (coherence stays on chip; `blocks` sets its
- * grid, 0 = default); n > 64, or blocks < 0 (per-stage mode: coherence, EMI
- * and temporal coherence as separate launches through HBM, the drop-in
- * sequence), runs per chunk of `chunk` pixels with
+ * 32 < n <= 64 runs one fused launch (coherence stays on chip; `blocks` sets
+ * its grid, 0 = default); n <= 32, n > 64, or blocks < 0 (per-stage mode:
+ * coherence, EMI and temporal coherence as separate launches through HBM, the
+ * drop-in sequence), runs per chunk of `chunk` pixels with
  * workspace >= mr_ds_batch_workspace_bytes(chunk, n).  For a sample matrix
  * made of a pixel's SHP stacks in window order the outputs equal the drop-in
  * composition exactly.

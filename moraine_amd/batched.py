@@ -5,8 +5,8 @@ images, row-major K_i x N_i) and goes through coherence -> EMI -> temporal
 coherence, as moraine's batch driver does for a point cloud
 (moraine/pl.py:148-168).  Pixels are grouped by image count; each group is one
 call of ``mr_ds_batch`` (csrc/batch.hip): a fused one-wave-per-pixel launch for
-N <= 64 (the coherence matrix stays in LDS), chunked coherence / EMI /
-temporal-coherence launches for N > 64.  Launch shapes (grid size of the fused
+32 < N <= 64 (the coherence matrix stays in LDS), chunked coherence / EMI /
+temporal-coherence launches otherwise.  Launch shapes (grid size of the fused
 kernel, chunk size of the large-N path) come from :class:`AutoTuner`, and a
 :class:`DSBatchRunner` can capture all group launches in one HIP graph and
 replay it on new data written into the same buffers.
@@ -184,7 +184,7 @@ class DSBatchRunner:
             if n < 2 or n > 1024:
                 raise ValueError(f"ds_batch: N={n} outside [2, 1024]")
             ids = np.nonzero(batch.n_images == n)[0].astype(np.int32)
-            if n > 64 or per_stage:
+            if n > 64 or n <= 32 or per_stage:
                 kmax = int(batch.n_samples[torch.from_numpy(ids).to(dev).long()].max()) if ids.size else 0
                 if kmax > 4096:
                     raise ValueError(f"ds_batch: N={n} pixels hold up to {kmax} sample rows; at most 4096 "
@@ -206,7 +206,7 @@ class DSBatchRunner:
                   0 if ws is None else ws.numel(), _dev.stream())
 
     def _staged(self, g: _Group) -> bool:
-        return self.per_stage or g.n > 64
+        return self.per_stage or g.n > 64 or g.n <= 32
 
     def _chunk_fits(self, n: int, chunk: int) -> bool:
         return _lib.query("mr_ds_batch_workspace_bytes", chunk, n) <= self.max_ws

@@ -5,18 +5,20 @@
 // ds_temp_coh per batch) over ragged per-pixel inputs, grouped by N by the
 // caller (one call per image count).
 //
-// n <= 64: one fused launch.  One wave per pixel accumulates the coherence on
+// 32 < n <= 64: one fused launch.  One wave per pixel accumulates the coherence on
 // MFMA into LDS tiles (co_mfma.hpp), runs EMI reading the tiles (emi_fast.hpp),
 // and forms the temporal coherence from the same tiles (tcoh.hpp); the
 // coherence matrix never reaches HBM.  Every stage shares its code with the
 // drop-in kernels, so for a sample matrix made of a pixel's SHP stacks in
 // window order the outputs equal emperical_co_pc -> emi -> ds_temp_coh exactly.
-// n > 64 (or any n with blocks < 0, the per-stage mode): per chunk of pixels,
+// n <= 32, n > 64 (or any n with blocks < 0, the per-stage mode): per chunk of pixels,
 // the coherence kernel in sample-matrix mode (MFMA for n <= 64, VALU above),
 // EMI and the temporal-coherence kernel as separate launches through an HBM
 // coherence buffer, then a scatter into the ragged outputs.  The per-stage
 // mode is what the drop-in functions do for one group; it gives the same
-// results as the fused launch.
+// results as the fused launch.  (For n <= 32 EMI runs the single-wave
+// two-lanes-per-row kernel, faster than the one-lane-per-row body the fused
+// launch is built on, so those groups always take the per-stage path.)
 #include "co_mfma.hpp"
 #include "emi_fast.hpp"
 #include "tcoh.hpp"
@@ -118,17 +120,13 @@ int mr_ds_batch(const float* samples, const long long* row_off, const int32_t* n
   hipStream_t st = mr::as_stream(stream);
   const float2* smp = reinterpret_cast<const float2*>(samples);
   float2* phc = reinterpret_cast<float2*>(ph);
-  if (n <= 64 && blocks >= 0) {
+  if (n > 32 && n <= 64 && blocks >= 0) {
     const int nb = (int)std::min<long long>(n_pix, blocks > 0 ? blocks : mr::BATCH_DEFAULT_BLOCKS);
-    if (n <= 32)
-      hipLaunchKernelGGL((mr::ds_batch_fused_kernel<32, false>), dim3(nb), dim3(64), mr::batch_fused_lds(false), st,
-                         smp, row_off, n_samples, ph_off, pix, n_pix, n, ref, pairs, n_pairs, phc, quality, t_coh);
-    else
-      hipLaunchKernelGGL((mr::ds_batch_fused_kernel<64, true>), dim3(nb), dim3(64), mr::batch_fused_lds(true), st,
-                         smp, row_off, n_samples, ph_off, pix, n_pix, n, ref, pairs, n_pairs, phc, quality, t_coh);
+    hipLaunchKernelGGL((mr::ds_batch_fused_kernel<64, true>), dim3(nb), dim3(64), mr::batch_fused_lds(true), st, smp,
+                       row_off, n_samples, ph_off, pix, n_pix, n, ref, pairs, n_pairs, phc, quality, t_coh);
     return mr::launch_status("ds_batch");
   }
-  MR_CHECK_ARG(chunk > 0, "ds_batch: chunk must be positive for n > 64 or per-stage mode");
+  MR_CHECK_ARG(chunk > 0, "ds_batch: chunk must be positive for n <= 32, n > 64 or the per-stage mode");
   const size_t need = mr_ds_batch_workspace_bytes(chunk, n);
   MR_CHECK_ARG(workspace && workspace_bytes >= need, "ds_batch: workspace %zu < %zu bytes", workspace_bytes, need);
   const long long P = (long long)n * (n - 1) / 2;

@@ -18,12 +18,13 @@
 //      shifts per round, one per lane), eigenvector of T by inverse iteration
 //      with a pivoted tridiagonal LU (FP64), back-transformation through the
 //      stored reflectors.
-// Fast path (n <= 64): one wave per point, lane i owns row i of the matrix in
-// VGPRs (FP64 for step 1, FP32 complex for steps 2-4); a uniform runtime
-// column index is served by a switch (scalar branch tree + v_mov), never by
-// scratch memory.  64 < n <= 200: the same algorithm with two lanes per row in
-// one workgroup (emi_wide.hip).  General path (n > 200): one 256-thread block
-// per point, everything FP64 in a per-block global workspace.
+// Fast path (32 < n <= 64): one wave per point, lane i owns row i of the
+// matrix in VGPRs (FP64 for step 1, FP32 complex for steps 2-4); a uniform
+// runtime column index is served by a switch (scalar branch tree + v_mov),
+// never by scratch memory.  n <= 32 and 64 < n <= 200: the same algorithm with
+// two lanes per row (emi_wide.hip; a single wave for n <= 32, where one lane
+// per row would leave half the wave idle).  General path (n > 200): one
+// 256-thread block per point, everything FP64 in a per-block global workspace.
 #include "emi_fast.hpp"
 
 namespace mr {
@@ -442,16 +443,11 @@ int mr_emi(const float* coh, long long n_pts, int n, int ref, float* ph, float* 
   hipStream_t st = mr::as_stream(stream);
   const float2* c = reinterpret_cast<const float2*>(coh);
   float2* p = reinterpret_cast<float2*>(ph);
-  if (n <= 64) {
-    const int nm = n <= 32 ? 32 : 64;
-    const size_t lds = mr::emi_fast_lds(n, nm);
+  if (n > 32 && n <= 64) {
+    const size_t lds = mr::emi_fast_lds(n, 64);
     const int nb = (int)std::min<long long>(n_pts, 1 << 20);
-    if (nm == 32)
-      hipLaunchKernelGGL((mr::emi_fast_kernel<32>), dim3(nb), dim3(mr::EMI_FAST_THREADS), lds, st, c, n_pts, n, ref,
-                         p, quality);
-    else
-      hipLaunchKernelGGL((mr::emi_fast_kernel<64>), dim3(nb), dim3(mr::EMI_FAST_THREADS), lds, st, c, n_pts, n, ref,
-                         p, quality);
+    hipLaunchKernelGGL((mr::emi_fast_kernel<64>), dim3(nb), dim3(mr::EMI_FAST_THREADS), lds, st, c, n_pts, n, ref, p,
+                       quality);
     return mr::launch_status("emi_fast");
   }
   if (n <= mr::EMI_WIDE_MAX_N) return mr::emi_wide_launch(c, n_pts, n, ref, p, quality, st);

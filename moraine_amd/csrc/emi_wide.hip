@@ -1,5 +1,5 @@
-// EMI phase linking for 64 < n <= 200 on gfx950: one workgroup per point, two
-// lanes per matrix row.  Same algorithm and precision split as the one-wave
+// EMI phase linking for n <= 32 and 64 < n <= 200 on gfx950: one workgroup
+// per point, two lanes per matrix row.  Same algorithm and precision split as the one-wave
 // kernel (moraine/pl.py:45-65 restated: inv(|Gamma|) by the FP64 symmetric
 // sweep with largest-|diagonal| pivoting, M = inv(|Gamma|) o Gamma, FP32
 // Householder tridiagonalisation, FP64 multisection + twisted factorisation
@@ -7,8 +7,9 @@
 //
 // Layout: thread t holds row i = t / 2 and the interleaved columns
 // j = 2 s + (t % 2), s = 0..E-1, in VGPRs (FP64 during the sweep, FP32 complex
-// afterwards), so a workgroup of 2n lanes (3-7 waves) keeps the whole matrix in
-// registers (E = 48, 64, 80, 100 entries per lane for n <= 96, 128, 160, 200).
+// afterwards), so a workgroup of 2n lanes (1 or 3-7 waves) keeps the whole matrix in
+// registers (E = 16, 48, 64, 80, 100 entries per lane for n <= 32, 96, 128,
+// 160, 200; n <= 32 is a single wave).
 // The two halves of a row are adjacent lanes: row partial sums combine with one
 // DPP exchange; column vectors are broadcast through LDS; reductions over rows
 // are wave DPP sums plus one LDS slot per wave.  Two workgroup barriers per
@@ -414,6 +415,8 @@ __global__ __launch_bounds__(64 * EW_MAX_WAVES) void emi_wide_kernel(const float
 
     // ---- step 4b: eigenvector of T by a twisted factorisation ---------------
     const double tiny = 2.2e-16 * scale + 1e-300;
+    // the two serial recurrences run on lanes of different waves (same wave if W = 1)
+    const int t2 = W > 1 ? 64 : 32;
     if (threadIdx.x == 0) {
       double dp = td[0] - lam;
       if (fabs(dp) < tiny) dp = tiny;
@@ -423,7 +426,7 @@ __global__ __launch_bounds__(64 * EW_MAX_WAVES) void emi_wide_kernel(const float
         if (fabs(dp) < tiny) dp = tiny;
         dpl[i] = dp;
       }
-    } else if (threadIdx.x == 64) {
+    } else if (threadIdx.x == t2) {
       double dm = td[n - 1] - lam;
       if (fabs(dm) < tiny) dm = tiny;
       dmi[n - 1] = dm;
@@ -461,7 +464,7 @@ __global__ __launch_bounds__(64 * EW_MAX_WAVES) void emi_wide_kernel(const float
         z *= -te[i] / dpl[i];
         zz[i] = z;
       }
-    } else if (threadIdx.x == 64) {
+    } else if (threadIdx.x == t2) {
       double z = 1.0;
       for (int i = rtw + 1; i < n; ++i) {
         z *= -te[i - 1] / dmi[i];
@@ -561,7 +564,9 @@ int emi_wide_launch(const float2* coh, long long npts, int n, int ref, float2* p
 #define MR_EW(E, GS, GH)                                                                                   \
   hipLaunchKernelGGL((emi_wide_kernel<E, GS, GH>), dim3(nb), dim3(threads), ew_lds_bytes(2 * (E)), st, coh, npts, \
                      n, ref, ph, quality)
-  if (n <= 96)
+  if (n <= 32)
+    MR_EW(16, 8, 4);
+  else if (n <= 96)
     MR_EW(48, 8, 4);
   else if (n <= 128)
     MR_EW(64, 8, 4);

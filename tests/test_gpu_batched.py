@@ -100,7 +100,7 @@ def test_batched_launch_shapes_do_not_change_results():
 
     from moraine_amd.batched import AutoTuner, DSBatch, DSBatchRunner
 
-    s, ro, ni, ns = synth.ds_samples_torch(3000, n_choices=(30, 90), k_range=(20, 120), seed=4, device="cuda")
+    s, ro, ni, ns = synth.ds_samples_torch(3000, n_choices=(30, 45, 90), k_range=(20, 120), seed=4, device="cuda")
     b = DSBatch.from_tensors(s, ro, ni, ns)
     _case("ds_batch default shapes")
     ref = [x.clone() for x in DSBatchRunner(b).run()]
@@ -109,9 +109,9 @@ def test_batched_launch_shapes_do_not_change_results():
     r = DSBatchRunner(b, tuner=tuner)
     out = r.run()
     torch.cuda.synchronize()
-    assert len(tuner.picks) == 2
+    assert len(tuner.picks) == 3
     for g in r.groups:
-        if g.n <= 64:
+        if not r._staged(g):
             assert g.blocks in AutoTuner.FUSED_BLOCKS
         else:
             assert g.chunk in AutoTuner.CHUNKS or g.chunk == g.count
