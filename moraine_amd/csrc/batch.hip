@@ -12,7 +12,7 @@
 // drop-in kernels, so for a sample matrix made of a pixel's SHP stacks in
 // window order the outputs equal emperical_co_pc -> emi -> ds_temp_coh exactly.
 // n <= 32, n > 64 (or any n with blocks < 0, the per-stage mode): per chunk of pixels,
-// the coherence kernel in sample-matrix mode (MFMA for n <= 64, VALU above),
+// the coherence kernel in sample-matrix mode (the drop-in's choice for n),
 // EMI and the temporal-coherence kernel as separate launches through an HBM
 // coherence buffer, then a scatter into the ragged outputs.  The per-stage
 // mode is what the drop-in functions do for one group; it gives the same
@@ -31,6 +31,9 @@ int co_samples_launch(const float2* samples, const long long* row_off, const int
 int co_mfma_samples_launch(const float2* samples, const long long* row_off, const int32_t* n_samp,
                            const int32_t* pix, long long pix0, long long cn, int n, const int32_t* pairs,
                            int npairs, float2* coh, hipStream_t st);
+int co_mfma_wide_samples_launch(const float2* samples, const long long* row_off, const int32_t* n_samp,
+                                const int32_t* pix, long long pix0, long long cn, int n, const int32_t* pairs,
+                                int npairs, float2* coh, hipStream_t st);
 constexpr int BATCH_DEFAULT_BLOCKS = 1 << 15;
 
 __host__ __device__ inline size_t batch_fused_lds(bool two) {
@@ -143,8 +146,13 @@ int mr_ds_batch(const float* samples, const long long* row_off, const int32_t* n
   const size_t emi_ws_bytes = mr::align256(mr_emi_workspace_bytes(chunk, n));
   for (long long c0 = 0; c0 < n_pix; c0 += chunk) {
     const long long cn = std::min(chunk, n_pix - c0);
-    const int rc0 = n <= 64 ? mr::co_mfma_samples_launch(smp, row_off, n_samples, pix, c0, cn, n, pairs, n_pairs, coh, st)
-                            : mr::co_samples_launch(smp, row_off, n_samples, pix, c0, cn, n, pairs, n_pairs, coh, st);
+    // the coherence kernel the drop-in uses for this n (MFMA up to 64 and for
+    // 97..128, VALU otherwise)
+    const bool wide = n > 96 && n <= 128;
+    const int rc0 =
+        n <= 64 ? mr::co_mfma_samples_launch(smp, row_off, n_samples, pix, c0, cn, n, pairs, n_pairs, coh, st)
+        : wide  ? mr::co_mfma_wide_samples_launch(smp, row_off, n_samples, pix, c0, cn, n, pairs, n_pairs, coh, st)
+                : mr::co_samples_launch(smp, row_off, n_samples, pix, c0, cn, n, pairs, n_pairs, coh, st);
     if (rc0) return rc0;
     if (int rc = mr_emi(reinterpret_cast<const float*>(coh), cn, n, ref, reinterpret_cast<float*>(ph_t), q_t, emi_ws,
                         emi_ws_bytes, stream))

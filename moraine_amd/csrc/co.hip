@@ -15,6 +15,10 @@ int co_mfma_launch(bool raster, const float2* rslc, int nl, int w, int n, const 
                    long long npts, int aw, int rw, long long pix0, const int32_t* pairs, int npairs, float2* cov,
                    float2* coh, hipStream_t st);
 constexpr int CO_MFMA_MAX_N = 64;
+int co_mfma_wide_launch(const float2* rslc, int nl, int w, int n, const int32_t* idx, const uint8_t* shp,
+                        long long npts, int aw, int rw, const int32_t* pairs, int npairs, float2* coh, hipStream_t st);
+constexpr int CO_MFMA_WIDE_MIN_N = 97;   // point-cloud form on the 4-wave MFMA kernel
+constexpr int CO_MFMA_WIDE_MAX_N = 128;  // for n in [97, 128]
 
 constexpr int CO_THREADS = 256;
 constexpr int CO_QB = 8;             // pairs per thread per batch
@@ -253,6 +257,9 @@ int mr_emperical_co_pc(const float* rslc, int nl, int w, int n, const int32_t* i
   if (n <= mr::CO_MFMA_MAX_N)
     return mr::co_mfma_launch(false, reinterpret_cast<const float2*>(rslc), nl, w, n, idx, pc_is_shp, n_pc, aw, rw, 0,
                               pairs, n_pairs, nullptr, reinterpret_cast<float2*>(coh), mr::as_stream(stream));
+  if (n >= mr::CO_MFMA_WIDE_MIN_N && n <= mr::CO_MFMA_WIDE_MAX_N)
+    return mr::co_mfma_wide_launch(reinterpret_cast<const float2*>(rslc), nl, w, n, idx, pc_is_shp, n_pc, aw, rw,
+                                   pairs, n_pairs, reinterpret_cast<float2*>(coh), mr::as_stream(stream));
   const int nb = std::min(n_pc, 1 << 16);
   const size_t lds = mr::CO_STAGE_BYTES + (size_t)n * 4;
   hipLaunchKernelGGL((mr::co_kernel<false>), dim3(nb), dim3(mr::CO_THREADS), lds, mr::as_stream(stream),

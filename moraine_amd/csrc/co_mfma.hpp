@@ -187,6 +187,40 @@ __device__ __forceinline__ void co_store_scales(const CoAcc& acc, int n, int lan
   __builtin_amdgcn_wave_barrier();
 }
 
+// index of pair (a, b), a < b, in triu order (moraine/tnet.py:15-20)
+__device__ __forceinline__ int co_pair_index(int a, int b, int n) { return a * (2 * n - a - 1) / 2 + (b - a - 1); }
+
+// True on every thread of the block if `pairs` is the full network in triu
+// order (the layout moraine uses when image_pairs is None); checked once per
+// block, it lets tiles be written straight from the accumulators.
+__device__ __forceinline__ bool co_pairs_canonical(const int32_t* __restrict__ pairs, int npairs, int n) {
+  bool bad = npairs != n * (n - 1) / 2;
+  if (!bad)
+    for (int p = threadIdx.x; p < npairs; p += blockDim.x) {
+      const int a = pairs[2 * p], b = pairs[2 * p + 1];
+      bad |= !(a >= 0 && a < b && b < n && co_pair_index(a, b, n) == p);
+    }
+  return !__syncthreads_or(bad);
+}
+
+// Coherence of the canonical pairs inside upper tile (ta, tb), written from the
+// accumulators: register q of lane l is element (32 ta + row(q, l), 32 tb + l % 32),
+// so for each q the 32 lanes of a half-wave write 32 consecutive pairs.
+__device__ __forceinline__ void co_write_tile_direct(const f16x_t& re, const f16x_t& im, int ta, int tb, int n,
+                                                     int lane, const float* rs, float2* __restrict__ out) {
+  const int hf = lane >> 5;
+  const int B = 32 * tb + (lane & 31);
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int A = 32 * ta + 8 * (q >> 2) + 4 * hf + (q & 3);
+    if (A < B && B < n) {
+      const float f = rs[A] * rs[B];
+      out[co_pair_index(A, B, n)] = make_float2(re[q] * f, im[q] * f);
+    }
+    if ((q & 3) == 3) __builtin_amdgcn_sched_barrier(0);  // bounds the addresses in flight
+  }
+}
+
 // One wave: accumulate, then (after a wave barrier) `tiles` holds the upper
 // tiles of S' (row-major 32x32 complex each), sc[0..64) the scales and
 // sc[64..128) 1/sqrt(S'_mm), so that coh_mj = S'_mj sc[64+m] sc[64+j].
