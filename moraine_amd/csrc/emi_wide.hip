@@ -5,37 +5,76 @@
 // Householder tridiagonalisation, FP64 multisection + twisted factorisation
 // for the smallest eigenpair, back-transformation, ph = v conj(v_ref)/|.|).
 //
-// Layout: thread t holds row i = t / 2 and the interleaved columns
-// j = 2 s + (t % 2), s = 0..E-1, in VGPRs (FP64 during the sweep, FP32 complex
-// afterwards), so a workgroup of 2n lanes (1 or 3-7 waves) keeps the whole matrix in
-// registers (E = 16, 48, 64, 80, 100 entries per lane for n <= 32, 96, 128,
-// 160, 200; n <= 32 is a single wave).
-// The two halves of a row are adjacent lanes: row partial sums combine with one
-// DPP exchange; column vectors are broadcast through LDS; reductions over rows
+// Layout: thread t holds row i = t / R and the interleaved columns
+// j = R s + (t % R), s = 0..E-1, in VGPRs (FP64 during the sweep, FP32 complex
+// afterwards), so a workgroup of R n lanes keeps the whole matrix in registers:
+// R = 2 with E = 16, 48, 64, 80, 100 for n <= 32, 96, 128, 160, 200 (n <= 32 is a
+// single wave, the others 3-7 waves).  R = 4 (E = 40, 10 waves at 156 VGPRs,
+// three per SIMD) was measured at n = 150: 6% slower than R = 2 (one 5-wave
+// block per CU), the extra barriers costing more than the occupancy gained, so
+// it is not dispatched.
+// The lanes of a row are adjacent (one DPP quad): row partial sums combine with
+// DPP exchanges; column vectors are broadcast through LDS; reductions over rows
 // are wave DPP sums plus one LDS slot per wave.  Two workgroup barriers per
 // sweep step and per Householder step, one per back-transformation step.
 #include "emi_common.hpp"
 
 namespace mr {
 
-constexpr int EW_MAX_WAVES = 8;
+constexpr int EW_MAX_WAVES = 10;
+
+// Rows are held by R adjacent lanes (R = 2 or 4, inside one DPP quad).
+// rowsum: sum over the R lanes of a row; rowpick: the value held by lane h of
+// the row (h uniform; every lane active -- a DPP read of a masked-off lane
+// returns 0).  quad_perm [h,h,h,h] = 0x55 h, [h,h,2+h,2+h] = 0xA0 / 0xF5.
+template <int R>
+__device__ __forceinline__ float rowsum_f(float v) {
+  v += dpp_f<DPP_XOR1>(v);
+  if (R == 4) v += dpp_f<DPP_XOR2>(v);
+  return v;
+}
+template <int R>
+__device__ __forceinline__ int rowpick_i(int v, int h) {
+  if (R == 2) return h == 0 ? dpp_i<0xA0>(v) : dpp_i<0xF5>(v);
+  switch (h) {
+    case 0:
+      return dpp_i<0x00>(v);
+    case 1:
+      return dpp_i<0x55>(v);
+    case 2:
+      return dpp_i<0xAA>(v);
+    default:
+      return dpp_i<0xFF>(v);
+  }
+}
+template <int R>
+__device__ __forceinline__ float rowpick_f(float v, int h) {
+  return __int_as_float(rowpick_i<R>(__float_as_int(v), h));
+}
+template <int R>
+__device__ __forceinline__ double rowpick_d(double v, int h) {
+  const long long b = __double_as_longlong(v);
+  const int lo = rowpick_i<R>((int)(b & 0xffffffffll), h);
+  const int hi = rowpick_i<R>((int)(b >> 32), h);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 
 // LDS layout for NM = 2E columns (bytes):
 //   doubles: colb[NM] td[NM] te[NM] dpl[NM] dmi[NM] zz[NM] red[2*16] misc[8]
 //   float2 : xb[NM] ub[NM] pb[NM] dph[NM] hal[NM]
-//   float  : hinv[NM] redf[2*16]
+//   float  : hinv[NM] redf[2*32]
 //   int    : redi[2*16]
 __host__ __device__ inline size_t ew_lds_bytes(int nm) {
-  return (size_t)8 * (6 * nm + 32 + 8) + (size_t)8 * 5 * nm + (size_t)4 * (nm + 32) + 4 * 32;
+  return (size_t)8 * (6 * nm + 32 + 8) + (size_t)8 * 5 * nm + (size_t)4 * (nm + 64) + 4 * 32;
 }
 
 // GS / GH: columns per software-pipelined LDS group in the sweep / Householder
 // updates (smaller groups trade latency hiding for registers at large E).
-template <int E, int GS, int GH>
-__global__ __launch_bounds__(64 * EW_MAX_WAVES) void emi_wide_kernel(const float2* __restrict__ coh, long long npts,
+template <int R, int E, int GS, int GH>
+__global__ __launch_bounds__(R == 2 ? 512 : 64 * EW_MAX_WAVES) void emi_wide_kernel(const float2* __restrict__ coh, long long npts,
                                                                      int n, int ref, float2* __restrict__ ph,
                                                                      float* __restrict__ quality) {
-  constexpr int NM = 2 * E;
+  constexpr int NM = R * E;
   static_assert(E % GS == 0 && E % GH == 0, "groups must tile the row");
   extern __shared__ __align__(16) unsigned char smem[];
   double* colb = reinterpret_cast<double*>(smem);
@@ -52,8 +91,8 @@ __global__ __launch_bounds__(64 * EW_MAX_WAVES) void emi_wide_kernel(const float
   float2* dph = pb + NM;
   float2* hal = dph + NM;
   float* hinv = reinterpret_cast<float*>(hal + NM);
-  float* redf = hinv + NM;  // [2][16]
-  int* redi = reinterpret_cast<int*>(redf + 32);  // [2][16]
+  float* redf = hinv + NM;  // [2][32]
+  int* redi = reinterpret_cast<int*>(redf + 64);  // [2][16]
 
   const int W = blockDim.x >> 6;
   const int P = n * (n - 1) / 2;
@@ -71,7 +110,7 @@ __global__ __launch_bounds__(64 * EW_MAX_WAVES) void emi_wide_kernel(const float
     int tid = threadIdx.x;
     __asm__ volatile("" : "+v"(tid));  // keeps per-lane addresses from being hoisted out of the point loop
     const int lane = tid & 63;
-    const int row = tid >> 1, c = tid & 1;
+    const int row = tid / R, c = tid % R;
     const bool live = row < n;
     const float2* src = coh + pt * (long long)P;
 
@@ -79,7 +118,7 @@ __global__ __launch_bounds__(64 * EW_MAX_WAVES) void emi_wide_kernel(const float
     double a[E];
 #pragma unroll
     for (int s = 0; s < E; ++s) {
-      const int j = 2 * s + c;
+      const int j = R * s + c;
       double v = 0.0;
       if (live && j < n) {
         if (j == row) {
@@ -92,7 +131,7 @@ __global__ __launch_bounds__(64 * EW_MAX_WAVES) void emi_wide_kernel(const float
       a[s] = v;
       if ((s & 7) == 7) __builtin_amdgcn_sched_barrier(0);  // bounds the loads in flight
     }
-    const bool dlane = live && (c == (row & 1));  // this lane holds the diagonal entry of its row
+    const bool dlane = live && (c == row % R);  // this lane holds the diagonal entry of its row
     double dg = dlane ? 1.0 : 0.0;
     bool swept = !dlane;
     for (int st = 0; st < n; ++st) {
@@ -103,7 +142,7 @@ __global__ __launch_bounds__(64 * EW_MAX_WAVES) void emi_wide_kernel(const float
         const double bk = rl_d(key, bl);
         if (lane == 0) {
           red[wv] = bk;
-          redi[wv] = (wv * 64 + bl) >> 1;
+          redi[wv] = (wv * 64 + bl) / R;
         }
       }
       __syncthreads();
@@ -117,12 +156,9 @@ __global__ __launch_bounds__(64 * EW_MAX_WAVES) void emi_wide_kernel(const float
         }
       }
       k = min(__builtin_amdgcn_readfirstlane(k), n - 1);
-      const bool holdk = c == (k & 1);
-      const double own = reg_get<double, E>(a, k >> 1);
-      // the exchange runs with every lane active (a DPP read of a lane that is
-      // masked off returns 0), then each lane selects
-      const double oth = dpp_d<DPP_XOR1>(own);
-      const double aik = holdk ? own : oth;  // A[row][k] on both lanes of the row
+      const bool holdk = c == k % R;
+      const double own = reg_get<double, E>(a, k / R);
+      const double aik = rowpick_d<R>(own, k % R);  // A[row][k] on every lane of the row
       // d-1 in place of the pivot: the rank-1 update then writes column k itself
       // (c - (c/d)(d-1) = c/d); the pivot's own diagonal becomes 2 - 1/d, which
       // the sweep never reads again and which is corrected when M is formed.
@@ -134,12 +170,12 @@ __global__ __launch_bounds__(64 * EW_MAX_WAVES) void emi_wide_kernel(const float
       {
         double cur[GS], nxt[GS];
 #pragma unroll
-        for (int t = 0; t < GS; ++t) cur[t] = colb[2 * t + c];
+        for (int t = 0; t < GS; ++t) cur[t] = colb[R * t + c];
 #pragma unroll
         for (int g = 0; g < E / GS; ++g) {
           if (g + 1 < E / GS) {
 #pragma unroll
-            for (int t = 0; t < GS; ++t) nxt[t] = colb[2 * ((g + 1) * GS + t) + c];
+            for (int t = 0; t < GS; ++t) nxt[t] = colb[R * ((g + 1) * GS + t) + c];
           }
 #pragma unroll
           for (int t = 0; t < GS; ++t) a[g * GS + t] = fma(-f, cur[t], a[g * GS + t]);
@@ -163,7 +199,7 @@ __global__ __launch_bounds__(64 * EW_MAX_WAVES) void emi_wide_kernel(const float
     __asm__ volatile("" : "+v"(row2), "+v"(c2));
 #pragma unroll
     for (int s = 0; s < E; ++s) {
-      const int j = 2 * s + c2;
+      const int j = R * s + c2;
       float re = 0.f, im = 0.f;
       if (live && j < n) {
         const double ai = (j == row2) ? 2.0 - a[s] : -a[s];
@@ -198,10 +234,8 @@ __global__ __launch_bounds__(64 * EW_MAX_WAVES) void emi_wide_kernel(const float
       // the loop as E/GH live masks
       int ns = n;
       __asm__ volatile("" : "+s"(ns));
-      const bool holdk = c == (k & 1);
-      const float ownr = reg_get<float, E>(mr_, k >> 1), owni = reg_get<float, E>(mi_, k >> 1);
-      const float pxr = dpp_f<DPP_XOR1>(ownr), pxi = dpp_f<DPP_XOR1>(owni);
-      const float mrk = holdk ? ownr : pxr, mik = holdk ? owni : pxi;  // M[row][k]
+      const float ownr = reg_get<float, E>(mr_, k / R), owni = reg_get<float, E>(mi_, k / R);
+      const float mrk = rowpick_f<R>(ownr, k % R), mik = rowpick_f<R>(owni, k % R);  // M[row][k]
       const bool below = live && row > k;
       const float xr = below ? mrk : 0.f, xi = below ? mik : 0.f;
       {
@@ -245,13 +279,13 @@ __global__ __launch_bounds__(64 * EW_MAX_WAVES) void emi_wide_kernel(const float
       {
         float2 cu[GH], nu[GH];
 #pragma unroll
-        for (int t = 0; t < GH; ++t) cu[t] = xb[2 * t + c];
+        for (int t = 0; t < GH; ++t) cu[t] = xb[R * t + c];
 #pragma unroll
         for (int g = 0; g < E / GH; ++g) {
-          const bool act = (2 * GH * g + 2 * GH - 1 > k) && (2 * GH * g < ns);
+          const bool act = (R * GH * g + R * GH - 1 > k) && (R * GH * g < ns);
           if (g + 1 < E / GH) {
 #pragma unroll
-            for (int t = 0; t < GH; ++t) nu[t] = xb[2 * ((g + 1) * GH + t) + c];
+            for (int t = 0; t < GH; ++t) nu[t] = xb[R * ((g + 1) * GH + t) + c];
           }
           if (act) {
 #pragma unroll
@@ -268,13 +302,10 @@ __global__ __launch_bounds__(64 * EW_MAX_WAVES) void emi_wide_kernel(const float
           for (int t = 0; t < GH; ++t) cu[t] = nu[t];
         }
       }
-      const bool hold1 = c == ((k + 1) & 1);
-      const float m1o_r = reg_get<float, E>(mr_, (k + 1) >> 1), m1o_i = reg_get<float, E>(mi_, (k + 1) >> 1);
-      float m1r = hold1 ? m1o_r : 0.f, m1i = hold1 ? m1o_i : 0.f;
-      sr += dpp_f<DPP_XOR1>(sr);
-      si += dpp_f<DPP_XOR1>(si);
-      m1r += dpp_f<DPP_XOR1>(m1r);
-      m1i += dpp_f<DPP_XOR1>(m1i);
+      const float m1o_r = reg_get<float, E>(mr_, (k + 1) / R), m1o_i = reg_get<float, E>(mi_, (k + 1) / R);
+      const float m1r = rowpick_f<R>(m1o_r, (k + 1) % R), m1i = rowpick_f<R>(m1o_i, (k + 1) % R);
+      sr = rowsum_f<R>(sr);
+      si = rowsum_f<R>(si);
       float pr = inv * (sr - (alr * m1r - ali * m1i));
       float pi = inv * (si - (alr * m1i + ali * m1r));
       float ur = (row == k + 1) ? (xr - alr) : xr;
@@ -303,17 +334,17 @@ __global__ __launch_bounds__(64 * EW_MAX_WAVES) void emi_wide_kernel(const float
         float2 cu[GH], cp[GH], nu[GH], np[GH];
 #pragma unroll
         for (int t = 0; t < GH; ++t) {
-          cu[t] = ub[2 * t + c];
-          cp[t] = pb[2 * t + c];
+          cu[t] = ub[R * t + c];
+          cp[t] = pb[R * t + c];
         }
 #pragma unroll
         for (int g = 0; g < E / GH; ++g) {
-          const bool act = (2 * GH * g + 2 * GH - 1 > k) && (2 * GH * g < ns);
+          const bool act = (R * GH * g + R * GH - 1 > k) && (R * GH * g < ns);
           if (g + 1 < E / GH) {
 #pragma unroll
             for (int t = 0; t < GH; ++t) {
-              nu[t] = ub[2 * ((g + 1) * GH + t) + c];
-              np[t] = pb[2 * ((g + 1) * GH + t) + c];
+              nu[t] = ub[R * ((g + 1) * GH + t) + c];
+              np[t] = pb[R * ((g + 1) * GH + t) + c];
             }
           }
           if (act) {
@@ -347,12 +378,12 @@ __global__ __launch_bounds__(64 * EW_MAX_WAVES) void emi_wide_kernel(const float
     // trailing 2x2 block: rows n-2, n-1
     {
       const int ka = n - 2, kb = n - 1;
-      const float va = reg_get<float, E>(mr_, ka >> 1);  // M[row][n-2] (real part) on its holder
-      const float vai = reg_get<float, E>(mi_, ka >> 1);
-      const float vb = reg_get<float, E>(mr_, kb >> 1);  // M[row][n-1] on its holder
-      if (row == ka && c == (ka & 1)) td[ka] = va;
-      if (row == kb && c == (kb & 1)) td[kb] = vb;
-      if (row == kb && c == (ka & 1)) {
+      const float va = reg_get<float, E>(mr_, ka / R);  // M[row][n-2] (real part) on its holder
+      const float vai = reg_get<float, E>(mi_, ka / R);
+      const float vb = reg_get<float, E>(mr_, kb / R);  // M[row][n-1] on its holder
+      if (row == ka && c == ka % R) td[ka] = va;
+      if (row == kb && c == kb % R) td[kb] = vb;
+      if (row == kb && c == ka % R) {
         const float ae = sqrtf(va * va + vai * vai);
         const float sr = ae > 0.f ? va / ae : 1.f, si = ae > 0.f ? vai / ae : 0.f;
         te[ka] = ae;
@@ -509,13 +540,11 @@ __global__ __launch_bounds__(64 * EW_MAX_WAVES) void emi_wide_kernel(const float
       vi = d.y * zi;
     }
     for (int k = n - 3; k >= 0; --k) {
-      const bool holdk = c == (k & 1);
-      const float ownr = reg_get<float, E>(mr_, k >> 1), owni = reg_get<float, E>(mi_, k >> 1);
-      const float pxr = dpp_f<DPP_XOR1>(ownr), pxi = dpp_f<DPP_XOR1>(owni);
+      const float ownr = reg_get<float, E>(mr_, k / R), owni = reg_get<float, E>(mi_, k / R);
       const bool below = live && row > k;
       const float2 al = hal[k];
       const float hv = hinv[k];
-      float ar = holdk ? ownr : pxr, ai = holdk ? owni : pxi;
+      float ar = rowpick_f<R>(ownr, k % R), ai = rowpick_f<R>(owni, k % R);
       if (row == k + 1) {
         ar -= al.x;
         ai -= al.y;
@@ -527,15 +556,15 @@ __global__ __launch_bounds__(64 * EW_MAX_WAVES) void emi_wide_kernel(const float
         const float s1 = wsum_f(c == 0 ? ar * vr + ai * vi : 0.f);
         const float s2 = wsum_f(c == 0 ? ar * vi - ai * vr : 0.f);
         if (lane == 0) {
-          redf[b * 16 + wv] = s1;
-          redf[b * 16 + 8 + wv] = s2;
+          redf[b * 32 + wv] = s1;
+          redf[b * 32 + 16 + wv] = s2;
         }
       }
       __syncthreads();
-      float sr = redf[b * 16], si = redf[b * 16 + 8];
+      float sr = redf[b * 32], si = redf[b * 32 + 16];
       for (int w2 = 1; w2 < W; ++w2) {
-        sr += redf[b * 16 + w2];
-        si += redf[b * 16 + 8 + w2];
+        sr += redf[b * 32 + w2];
+        si += redf[b * 32 + 16 + w2];
       }
       vr -= 2.f * (ar * sr - ai * si);
       vi -= 2.f * (ar * si + ai * sr);
@@ -559,21 +588,20 @@ __global__ __launch_bounds__(64 * EW_MAX_WAVES) void emi_wide_kernel(const float
 }
 
 int emi_wide_launch(const float2* coh, long long npts, int n, int ref, float2* ph, float* quality, hipStream_t st) {
-  const int threads = (2 * n + 63) / 64 * 64;
   const int nb = (int)std::min<long long>(npts, 1 << 20);
-#define MR_EW(E, GS, GH)                                                                                   \
-  hipLaunchKernelGGL((emi_wide_kernel<E, GS, GH>), dim3(nb), dim3(threads), ew_lds_bytes(2 * (E)), st, coh, npts, \
-                     n, ref, ph, quality)
+#define MR_EW(R, E, GS, GH)                                                                                     \
+  hipLaunchKernelGGL((emi_wide_kernel<R, E, GS, GH>), dim3(nb), dim3(((R) * n + 63) / 64 * 64),                \
+                     ew_lds_bytes((R) * (E)), st, coh, npts, n, ref, ph, quality)
   if (n <= 32)
-    MR_EW(16, 8, 4);
+    MR_EW(2, 16, 8, 4);
   else if (n <= 96)
-    MR_EW(48, 8, 4);
+    MR_EW(2, 48, 8, 4);
   else if (n <= 128)
-    MR_EW(64, 8, 4);
+    MR_EW(2, 64, 8, 4);
   else if (n <= 160)
-    MR_EW(80, 8, 4);
+    MR_EW(2, 80, 8, 4);
   else
-    MR_EW(100, 4, 2);
+    MR_EW(2, 100, 4, 2);
 #undef MR_EW
   return launch_status("emi_wide");
 }
