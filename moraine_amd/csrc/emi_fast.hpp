@@ -14,7 +14,7 @@ template <int NM, class Src>
 __device__ __forceinline__ void emi_fast_point(const Src& coh_at, int n, int lane, double* colb, float2* ub,
                                                float2* qb, float& vr, float& vi, double& rq) {
   // ---- step 1: A = |Gamma| (row per lane, FP64), sweep inverse ----------
-  double a[NM];
+  RegArr<double, NM> a;
 #pragma unroll
   for (int j = 0; j < NM; ++j) {
     double v = 0.0;
@@ -26,14 +26,14 @@ __device__ __forceinline__ void emi_fast_point(const Src& coh_at, int n, int lan
         v = sqrt((double)g.x * g.x + (double)g.y * g.y);
       }
     }
-    a[j] = v;
+    a.set(j, v);
     if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
   }
   double dg = lane < n ? 1.0 : 0.0;  // own diagonal entry, tracked through the sweep
   bool swept = lane >= n;
   for (int s = 0; s < n; ++s) {
     const int k = wargmax_d(swept ? -1.0 : fabs(dg), lane);
-    const double c = reg_get<double, NM>(a, k);
+    const double c = a.dyn(k);
     // Broadcasting d-1 in place of the pivot d lets the rank-1 update itself
     // write column k:  c - (c/d)(d-1) = c/d.  The pivot lane's own diagonal
     // comes out as 2 - 1/d instead of -1/d; that entry is never read again
@@ -56,7 +56,7 @@ __device__ __forceinline__ void emi_fast_point(const Src& coh_at, int n, int lan
           for (int t = 0; t < 8; ++t) nxt[t] = colb[(g + 1) * 8 + t];
         }
 #pragma unroll
-        for (int t = 0; t < 8; ++t) a[g * 8 + t] = fma(-f, cur[t], a[g * 8 + t]);
+        for (int t = 0; t < 8; ++t) a.set(g * 8 + t, fma(-f, cur[t], a.get(g * 8 + t)));
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int t = 0; t < 8; ++t) cur[t] = nxt[t];
@@ -69,12 +69,12 @@ __device__ __forceinline__ void emi_fast_point(const Src& coh_at, int n, int lan
   // a now holds -inv(A) row `lane` (diagonal entry offset by +2)
 
   // ---- step 2: M = inv(A) o Gamma (FP32 complex row per lane) ------------
-  float mr_[NM], mi_[NM];
+  RegArr<float, NM> mr_, mi_;
 #pragma unroll
   for (int j = 0; j < NM; ++j) {
     float re = 0.f, im = 0.f;
     if (lane < n && j < n) {
-      const double ai = (j == lane) ? 2.0 - a[j] : -a[j];  // undo the sweep's +2 on the diagonal
+      const double ai = (j == lane) ? 2.0 - a.get(j) : -a.get(j);  // undo the sweep's +2 on the diagonal
       if (j == lane) {
         re = (float)ai;
       } else if (j > lane) {
@@ -87,8 +87,8 @@ __device__ __forceinline__ void emi_fast_point(const Src& coh_at, int n, int lan
         im = (float)(-ai * g.y);
       }
     }
-    mr_[j] = re;
-    mi_[j] = im;
+    mr_.set(j, re);
+    mi_.set(j, im);
     if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
   }
 
@@ -100,8 +100,8 @@ __device__ __forceinline__ void emi_fast_point(const Src& coh_at, int n, int lan
   float my_dr = 1.f, my_di = 0.f;
   float dcr = 1.f, dci = 0.f;  // running phase product D_k (uniform)
   for (int k = 0; k + 2 < n; ++k) {
-    const float xr_raw = reg_get<float, NM>(mr_, k);
-    const float xi_raw = reg_get<float, NM>(mi_, k);
+    const float xr_raw = mr_.dyn(k);
+    const float xi_raw = mi_.dyn(k);
     const bool below = lane > k && lane < n;
     const float xr = below ? xr_raw : 0.f, xi = below ? xi_raw : 0.f;
     const float nrm2 = wsum_f(xr * xr + xi * xi);
@@ -156,10 +156,10 @@ __device__ __forceinline__ void emi_fast_point(const Src& coh_at, int n, int lan
 #pragma unroll
           for (int t = 0; t < 4; ++t) {
             const int j = g * 4 + t;
-            pr = fmaf(mr_[j], cu[t].x, pr);
-            pr = fmaf(-mi_[j], cu[t].y, pr);
-            pi = fmaf(mr_[j], cu[t].y, pi);
-            pi = fmaf(mi_[j], cu[t].x, pi);
+            pr = fmaf(mr_.get(j), cu[t].x, pr);
+            pr = fmaf(-mi_.get(j), cu[t].y, pr);
+            pi = fmaf(mr_.get(j), cu[t].y, pi);
+            pi = fmaf(mi_.get(j), cu[t].x, pi);
           }
         }
         __builtin_amdgcn_sched_barrier(0);
@@ -194,7 +194,7 @@ __device__ __forceinline__ void emi_fast_point(const Src& coh_at, int n, int lan
 #pragma unroll
           for (int t = 0; t < 4; ++t) {
             const int j = g * 4 + t;
-            float r = mr_[j], i = mi_[j];
+            float r = mr_.get(j), i = mi_.get(j);
             r = fmaf(-ur, cq[t].x, r);
             r = fmaf(-ui, cq[t].y, r);
             i = fmaf(ur, cq[t].y, i);
@@ -203,8 +203,8 @@ __device__ __forceinline__ void emi_fast_point(const Src& coh_at, int n, int lan
             r = fmaf(-qi, cu[t].y, r);
             i = fmaf(qr, cu[t].y, i);
             i = fmaf(-qi, cu[t].x, i);
-            mr_[j] = r;
-            mi_[j] = i;
+            mr_.set(j, r);
+            mi_.set(j, i);
           }
         }
         __builtin_amdgcn_sched_barrier(0);
@@ -219,9 +219,9 @@ __device__ __forceinline__ void emi_fast_point(const Src& coh_at, int n, int lan
   }
   {
     // trailing 2x2 block
-    const float r0 = reg_get<float, NM>(mr_, n - 2);
-    const float i0 = reg_get<float, NM>(mi_, n - 2);
-    const float r1 = reg_get<float, NM>(mr_, n - 1);
+    const float r0 = mr_.dyn(n - 2);
+    const float i0 = mi_.dyn(n - 2);
+    const float r1 = mr_.dyn(n - 1);
     const float dm2 = rl_f(r0, n - 2);
     const float dm1 = rl_f(r1, n - 1);
     const float er = rl_f(r0, n - 1), ei = rl_f(i0, n - 1);
@@ -337,8 +337,8 @@ __device__ __forceinline__ void emi_fast_point(const Src& coh_at, int n, int lan
     vi = my_di * zi;
   }
   for (int k = n - 3; k >= 0; --k) {
-    const float xr = reg_get<float, NM>(mr_, k);
-    const float xi = reg_get<float, NM>(mi_, k);
+    const float xr = mr_.dyn(k);
+    const float xi = mi_.dyn(k);
     const bool below = lane > k && lane < n;
     const float alr = rl_f(my_ar, k), ali = rl_f(my_ai, k), hv = rl_f(my_hv, k);
     const float ar = below ? ((lane == k + 1) ? xr - alr : xr) * hv : 0.f;

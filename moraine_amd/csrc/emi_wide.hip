@@ -115,7 +115,7 @@ __global__ __launch_bounds__(R == 2 ? 512 : 64 * EW_MAX_WAVES) void emi_wide_ker
     const float2* src = coh + pt * (long long)P;
 
     // ---- step 1: A = |Gamma| (FP64), sweep inverse --------------------------
-    double a[E];
+    RegArr<double, E> a;
 #pragma unroll
     for (int s = 0; s < E; ++s) {
       const int j = R * s + c;
@@ -128,7 +128,7 @@ __global__ __launch_bounds__(R == 2 ? 512 : 64 * EW_MAX_WAVES) void emi_wide_ker
           v = sqrt((double)g.x * g.x + (double)g.y * g.y);
         }
       }
-      a[s] = v;
+      a.set(s, v);
       if ((s & 7) == 7) __builtin_amdgcn_sched_barrier(0);  // bounds the loads in flight
     }
     const bool dlane = live && (c == row % R);  // this lane holds the diagonal entry of its row
@@ -157,7 +157,7 @@ __global__ __launch_bounds__(R == 2 ? 512 : 64 * EW_MAX_WAVES) void emi_wide_ker
       }
       k = min(__builtin_amdgcn_readfirstlane(k), n - 1);
       const bool holdk = c == k % R;
-      const double own = reg_get<double, E>(a, k / R);
+      const double own = a.dyn(k / R);
       const double aik = rowpick_d<R>(own, k % R);  // A[row][k] on every lane of the row
       // d-1 in place of the pivot: the rank-1 update then writes column k itself
       // (c - (c/d)(d-1) = c/d); the pivot's own diagonal becomes 2 - 1/d, which
@@ -178,7 +178,7 @@ __global__ __launch_bounds__(R == 2 ? 512 : 64 * EW_MAX_WAVES) void emi_wide_ker
             for (int t = 0; t < GS; ++t) nxt[t] = colb[R * ((g + 1) * GS + t) + c];
           }
 #pragma unroll
-          for (int t = 0; t < GS; ++t) a[g * GS + t] = fma(-f, cur[t], a[g * GS + t]);
+          for (int t = 0; t < GS; ++t) a.set(g * GS + t, fma(-f, cur[t], a.get(g * GS + t)));
           __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
           for (int t = 0; t < GS; ++t) cur[t] = nxt[t];
@@ -190,7 +190,7 @@ __global__ __launch_bounds__(R == 2 ? 512 : 64 * EW_MAX_WAVES) void emi_wide_ker
     // a holds -inv(A) (diagonal entries offset by +2)
 
     // ---- step 2: M = inv(A) o Gamma (FP32 complex) ---------------------------
-    float mr_[E], mi_[E];
+    RegArr<float, E> mr_, mi_;
     {
     // fresh opaque copies of row and column parity: the E column indices and
     // pair offsets of step 1 are recomputed here instead of being kept live in
@@ -202,7 +202,7 @@ __global__ __launch_bounds__(R == 2 ? 512 : 64 * EW_MAX_WAVES) void emi_wide_ker
       const int j = R * s + c2;
       float re = 0.f, im = 0.f;
       if (live && j < n) {
-        const double ai = (j == row2) ? 2.0 - a[s] : -a[s];
+        const double ai = (j == row2) ? 2.0 - a.get(s) : -a.get(s);
         if (j == row2) {
           re = (float)ai;
         } else if (j > row2) {
@@ -215,8 +215,8 @@ __global__ __launch_bounds__(R == 2 ? 512 : 64 * EW_MAX_WAVES) void emi_wide_ker
           im = (float)(-ai * g.y);
         }
       }
-      mr_[s] = re;
-      mi_[s] = im;
+      mr_.set(s, re);
+      mi_.set(s, im);
       if ((s & 7) == 7) __builtin_amdgcn_sched_barrier(0);
     }
     }
@@ -234,7 +234,7 @@ __global__ __launch_bounds__(R == 2 ? 512 : 64 * EW_MAX_WAVES) void emi_wide_ker
       // the loop as E/GH live masks
       int ns = n;
       __asm__ volatile("" : "+s"(ns));
-      const float ownr = reg_get<float, E>(mr_, k / R), owni = reg_get<float, E>(mi_, k / R);
+      const float ownr = mr_.dyn(k / R), owni = mi_.dyn(k / R);
       const float mrk = rowpick_f<R>(ownr, k % R), mik = rowpick_f<R>(owni, k % R);  // M[row][k]
       const bool below = live && row > k;
       const float xr = below ? mrk : 0.f, xi = below ? mik : 0.f;
@@ -291,10 +291,10 @@ __global__ __launch_bounds__(R == 2 ? 512 : 64 * EW_MAX_WAVES) void emi_wide_ker
 #pragma unroll
             for (int t = 0; t < GH; ++t) {
               const int s = g * GH + t;
-              sr = fmaf(mr_[s], cu[t].x, sr);
-              sr = fmaf(-mi_[s], cu[t].y, sr);
-              si = fmaf(mr_[s], cu[t].y, si);
-              si = fmaf(mi_[s], cu[t].x, si);
+              sr = fmaf(mr_.get(s), cu[t].x, sr);
+              sr = fmaf(-mi_.get(s), cu[t].y, sr);
+              si = fmaf(mr_.get(s), cu[t].y, si);
+              si = fmaf(mi_.get(s), cu[t].x, si);
             }
           }
           __builtin_amdgcn_sched_barrier(0);
@@ -302,7 +302,7 @@ __global__ __launch_bounds__(R == 2 ? 512 : 64 * EW_MAX_WAVES) void emi_wide_ker
           for (int t = 0; t < GH; ++t) cu[t] = nu[t];
         }
       }
-      const float m1o_r = reg_get<float, E>(mr_, (k + 1) / R), m1o_i = reg_get<float, E>(mi_, (k + 1) / R);
+      const float m1o_r = mr_.dyn((k + 1) / R), m1o_i = mi_.dyn((k + 1) / R);
       const float m1r = rowpick_f<R>(m1o_r, (k + 1) % R), m1i = rowpick_f<R>(m1o_i, (k + 1) % R);
       sr = rowsum_f<R>(sr);
       si = rowsum_f<R>(si);
@@ -353,7 +353,7 @@ __global__ __launch_bounds__(R == 2 ? 512 : 64 * EW_MAX_WAVES) void emi_wide_ker
               const int s = g * GH + t;
               const float cqr = fmaf(-c2, cu[t].x, cp[t].x);
               const float cqi = fmaf(-c2, cu[t].y, cp[t].y);
-              float r = mr_[s], i = mi_[s];
+              float r = mr_.get(s), i = mi_.get(s);
               r = fmaf(-ur, cqr, r);
               r = fmaf(-ui, cqi, r);
               i = fmaf(ur, cqi, i);
@@ -362,8 +362,8 @@ __global__ __launch_bounds__(R == 2 ? 512 : 64 * EW_MAX_WAVES) void emi_wide_ker
               r = fmaf(-qi, cu[t].y, r);
               i = fmaf(qr, cu[t].y, i);
               i = fmaf(-qi, cu[t].x, i);
-              mr_[s] = r;
-              mi_[s] = i;
+              mr_.set(s, r);
+              mi_.set(s, i);
             }
           }
           __builtin_amdgcn_sched_barrier(0);
@@ -378,9 +378,9 @@ __global__ __launch_bounds__(R == 2 ? 512 : 64 * EW_MAX_WAVES) void emi_wide_ker
     // trailing 2x2 block: rows n-2, n-1
     {
       const int ka = n - 2, kb = n - 1;
-      const float va = reg_get<float, E>(mr_, ka / R);  // M[row][n-2] (real part) on its holder
-      const float vai = reg_get<float, E>(mi_, ka / R);
-      const float vb = reg_get<float, E>(mr_, kb / R);  // M[row][n-1] on its holder
+      const float va = mr_.dyn(ka / R);  // M[row][n-2] (real part) on its holder
+      const float vai = mi_.dyn(ka / R);
+      const float vb = mr_.dyn(kb / R);  // M[row][n-1] on its holder
       if (row == ka && c == ka % R) td[ka] = va;
       if (row == kb && c == kb % R) td[kb] = vb;
       if (row == kb && c == ka % R) {
@@ -540,7 +540,7 @@ __global__ __launch_bounds__(R == 2 ? 512 : 64 * EW_MAX_WAVES) void emi_wide_ker
       vi = d.y * zi;
     }
     for (int k = n - 3; k >= 0; --k) {
-      const float ownr = reg_get<float, E>(mr_, k / R), owni = reg_get<float, E>(mi_, k / R);
+      const float ownr = mr_.dyn(k / R), owni = mi_.dyn(k / R);
       const bool below = live && row > k;
       const float2 al = hal[k];
       const float hv = hinv[k];
