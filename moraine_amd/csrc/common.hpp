@@ -159,6 +159,25 @@ __device__ __forceinline__ int wargmax_d(double key, int lane) {
   return __builtin_amdgcn_readfirstlane(bi);
 }
 
+// Wave max of an int, uniform result.
+__device__ __forceinline__ int wmax_i(int v) {
+  v = max(v, dpp_i<DPP_XOR1>(v));
+  v = max(v, dpp_i<DPP_XOR2>(v));
+  v = max(v, dpp_i<DPP_HALF_MIRROR>(v));
+  v = max(v, dpp_i<DPP_MIRROR>(v));
+  return __builtin_amdgcn_readfirstlane(max(max(rl_i(v, 0), rl_i(v, 16)), max(rl_i(v, 32), rl_i(v, 48))));
+}
+// Pivot key for an integer max: the high word of a non-negative double (whose
+// bit patterns order as the values do) with its low BITS bits replaced by
+// (2^BITS - 1 - id), so the max picks the largest value to within a relative
+// 2^-(20 - BITS) and, among equal keys, the smallest id; ineligible entries
+// get -1.  id = (2^BITS - 1) - (max & (2^BITS - 1)).
+template <int BITS>
+__device__ __forceinline__ int pivot_key(double mag, bool eligible, int id) {
+  constexpr int m = (1 << BITS) - 1;
+  return eligible ? ((__double2hiint(mag) & ~m) | (m - id)) : -1;
+}
+
 // FP64 reciprocal: hardware approximation refined by two Newton steps (~1 ulp).
 __device__ __forceinline__ double rcp_d(double x) {
   double r = __builtin_amdgcn_rcp(x);

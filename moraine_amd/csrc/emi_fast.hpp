@@ -32,7 +32,8 @@ __device__ __forceinline__ void emi_fast_point(const Src& coh_at, int n, int lan
   double dg = lane < n ? 1.0 : 0.0;  // own diagonal entry, tracked through the sweep
   bool swept = lane >= n;
   for (int s = 0; s < n; ++s) {
-    const int k = wargmax_d(swept ? -1.0 : fabs(dg), lane);
+    int k = 63 - (wmax_i(pivot_key<6>(fabs(dg), !swept, lane)) & 63);  // unswept row, largest |diagonal|
+    __asm__ volatile("" : "+s"(k));  // an opaque index keeps the row in VGPRs (M0-relative read, no scratch copy)
     const double c = a.dyn(k);
     // Broadcasting d-1 in place of the pivot d lets the rank-1 update itself
     // write column k:  c - (c/d)(d-1) = c/d.  The pivot lane's own diagonal

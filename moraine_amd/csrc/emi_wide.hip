@@ -22,6 +22,7 @@
 namespace mr {
 
 constexpr int EW_MAX_WAVES = 10;
+static_assert(EMI_WIDE_MAX_N <= 256, "pivot keys carry the row index in 8 bits");
 
 // Rows are held by R adjacent lanes (R = 2 or 4, inside one DPP quad).
 // rowsum: sum over the R lanes of a row; rowpick: the value held by lane h of
@@ -137,24 +138,13 @@ __global__ __launch_bounds__(R == 2 ? 512 : 64 * EW_MAX_WAVES) void emi_wide_ker
     for (int st = 0; st < n; ++st) {
       // pivot: unswept row with the largest |diagonal| (ties -> smallest row)
       {
-        const double key = swept ? -1.0 : fabs(dg);
-        const int bl = wargmax_d(key, lane);
-        const double bk = rl_d(key, bl);
-        if (lane == 0) {
-          red[wv] = bk;
-          redi[wv] = (wv * 64 + bl) / R;
-        }
+        const int wk = wmax_i(pivot_key<8>(fabs(dg), !swept, row));
+        if (lane == 0) redi[wv] = wk;
       }
       __syncthreads();
-      double kk = red[0];
-      int k = redi[0];
-      for (int w2 = 1; w2 < W; ++w2) {
-        const double k2 = red[w2];
-        if (k2 > kk) {
-          kk = k2;
-          k = redi[w2];
-        }
-      }
+      int kb = redi[0];
+      for (int w2 = 1; w2 < W; ++w2) kb = max(kb, redi[w2]);
+      int k = 255 - (kb & 255);
       k = min(__builtin_amdgcn_readfirstlane(k), n - 1);
       const bool holdk = c == k % R;
       const double own = a.dyn(k / R);
