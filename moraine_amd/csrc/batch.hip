@@ -37,7 +37,7 @@ int co_mfma_wide_samples_launch(const float2* samples, const long long* row_off,
 constexpr int BATCH_DEFAULT_BLOCKS = 1 << 15;
 
 __host__ __device__ inline size_t batch_fused_lds(bool two) {
-  return (size_t)com_tile_floats(two) * 4 + 128 * 4 /* scales */ + 64 * 8 /* colb */ + 2 * 64 * 8 /* ub, qb */;
+  return (size_t)com_tile_floats(two) * 4 + 128 * 4 /* scales */ + EMI_FAST_LDS_BYTES;
 }
 
 template <int NM, bool TWO>
@@ -52,9 +52,8 @@ __global__ __launch_bounds__(64) void ds_batch_fused_kernel(const float2* __rest
   extern __shared__ __align__(16) unsigned char smem[];
   float2* tiles = reinterpret_cast<float2*>(smem);
   float* sc = reinterpret_cast<float*>(smem + (size_t)com_tile_floats(TWO) * 4);  // scales, 1/sqrt(diag)
-  double* colb = reinterpret_cast<double*>(sc + 128);
-  float2* ub = reinterpret_cast<float2*>(colb + 64);
-  float2* qb = ub + 64;
+  unsigned char* work = reinterpret_cast<unsigned char*>(sc + 128);  // EMI_FAST_LDS_BYTES
+  float2* ub = reinterpret_cast<float2*>(work + 512);               // linked phases after EMI
   for (long long t = blockIdx.x; t < npix; t += gridDim.x) {
     int lane = threadIdx.x;
     __asm__ volatile("" : "+v"(lane));  // per-pixel opaque lane (see emi.hip)
@@ -63,7 +62,7 @@ __global__ __launch_bounds__(64) void ds_batch_fused_kernel(const float2* __rest
     const CohTiles<TWO> ct{tiles, sc + 64};
     float vr, vi;
     double rq;
-    emi_fast_point<NM>(ct, n, lane, colb, ub, qb, vr, vi, rq);
+    emi_fast_point<NM>(ct, n, lane, work, vr, vi, rq);
     const float2 pv = emi_link_phase(vr, vi, ref);
     __builtin_amdgcn_wave_barrier();
     ub[lane] = pv;  // linked phases for the temporal coherence

@@ -126,10 +126,7 @@ __global__ __launch_bounds__(EMI_FAST_THREADS) void emi_fast_kernel(const float2
   const int P = n * (n - 1) / 2;
   const int lane0 = threadIdx.x;
   float2* cs = reinterpret_cast<float2*>(smem);                          // staged coh, P entries
-  // per-lane broadcast buffers hold 64 entries (every lane writes its slot)
-  double* colb = reinterpret_cast<double*>(smem + ((P * 8 + 15) & ~15));  // 64 doubles (sweep broadcast)
-  float2* ub = reinterpret_cast<float2*>(colb + 64);                      // 64 (reflector broadcast)
-  float2* qb = ub + 64;                                                   // 64
+  unsigned char* work = smem + ((P * 8 + 15) & ~15);                      // EMI_FAST_LDS_BYTES
 
   for (long long pt = blockIdx.x; pt < npts; pt += gridDim.x) {
     // Opaque per-point copy of the lane id: keeps the compiler from hoisting
@@ -142,7 +139,7 @@ __global__ __launch_bounds__(EMI_FAST_THREADS) void emi_fast_kernel(const float2
     __builtin_amdgcn_wave_barrier();
     float vr, vi;
     double rq;
-    emi_fast_point<NM>(CohPacked{cs, n}, n, lane, colb, ub, qb, vr, vi, rq);
+    emi_fast_point<NM>(CohPacked{cs, n}, n, lane, work, vr, vi, rq);
     // ---- output: ph = v conj(v_ref) / |.|, quality --------------------------
     const float2 pv = emi_link_phase(vr, vi, ref);
     if (lane < n) ph[pt * n + lane] = pv;
@@ -420,7 +417,7 @@ int emi_wide_launch(const float2* coh, long long npts, int n, int ref, float2* p
 
 static size_t emi_fast_lds(int n, int nm) {
   const int P = n * (n - 1) / 2;
-  return (((size_t)P * 8 + 15) & ~(size_t)15) + 64 * (8 + 8 + 8) + 64;
+  return (((size_t)P * 8 + 15) & ~(size_t)15) + EMI_FAST_LDS_BYTES;
 }
 
 }  // namespace mr

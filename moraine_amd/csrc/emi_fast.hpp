@@ -8,11 +8,19 @@ namespace mr {
 // One wave, n <= NM <= 64: the smallest eigenpair of M = inv(|Gamma|) o Gamma
 // for one point.  coh_at(a, b), a < b, returns the coherence of pair (a, b).
 // On return lane i holds (vr, vi) = v_i, the eigenvector before the reference
-// rotation, and rq the Rayleigh quotient (smallest eigenvalue).  colb/ub/qb:
-// 64-entry LDS buffers.
+// rotation, and rq the Rayleigh quotient (smallest eigenvalue).  lds:
+// EMI_FAST_LDS_BYTES of 16-byte aligned LDS owned by this wave.
+constexpr int EMI_FAST_LDS_BYTES = 1536;
+
 template <int NM, class Src>
-__device__ __forceinline__ void emi_fast_point(const Src& coh_at, int n, int lane, double* colb, float2* ub,
-                                               float2* qb, float& vr, float& vi, double& rq) {
+__device__ __forceinline__ void emi_fast_point(const Src& coh_at, int n, int lane, unsigned char* lds, float& vr,
+                                               float& vi, double& rq) {
+  double* colb = reinterpret_cast<double*>(lds);        // 64: sweep column broadcast
+  float2* ub = reinterpret_cast<float2*>(lds + 512);    // 64: reflector u
+  float2* qb = ub + 64;                                 // 64: q
+  // after the reduction the same bytes serve the uniform reads of T
+  double2* tde = reinterpret_cast<double2*>(lds);       // 64: (d_i, e_{i-1}^2)
+  double* tee = reinterpret_cast<double*>(lds + 1024);  // 64: e_i
   // ---- step 1: A = |Gamma| (row per lane, FP64), sweep inverse ----------
   RegArr<double, NM> a;
 #pragma unroll
@@ -239,10 +247,15 @@ __device__ __forceinline__ void emi_fast_point(const Src& coh_at, int n, int lan
       my_di = dcr * si + dci * sr;
     }
   }
-  const double my_e2 = my_e * my_e;
 
   // ---- step 4a: smallest eigenvalue of T by multisection ------------------
   const double e_prev = dpp_d<DPP_WAVE_SHR1>(my_e);  // e_{i-1} (0 for lane 0)
+  // T to LDS: the serial recurrences below read d_i, e_{i-1}^2 and e_i as one
+  // uniform-address LDS load per step instead of two readlanes per double
+  __builtin_amdgcn_wave_barrier();  // every lane is past its last read of ub/qb
+  tde[lane] = make_double2(my_d, e_prev * e_prev);
+  tee[lane] = my_e;
+  __builtin_amdgcn_wave_barrier();
   double lo, hi, scale;
   {
     const double off = e_prev + my_e;
@@ -257,14 +270,15 @@ __device__ __forceinline__ void emi_fast_point(const Src& coh_at, int n, int lan
   for (int round = 0; round < 6; ++round) {
     if (hi - lo <= 2e-15 * scale) break;
     const double x = lo + (hi - lo) * (double)(lane + 1) * (1.0 / 65.0);
-    // Sturm count of eigenvalues below x; d_i, e_{i-1}^2 broadcast by readlane
-    double q = rl_d(my_d, 0) - x;
+    // Sturm count of eigenvalues below x
+    double q = tde[0].x - x;
     int cnt = q < 0.0;
     for (int i = 1; i < n; ++i) {
       if (fabs(q) < pivmin) q = -pivmin;
       double r = __builtin_amdgcn_rcp(q);
       r = fma(r, fma(-q, r, 1.0), r);  // one Newton step on the reciprocal
-      q = (rl_d(my_d, i) - x) - rl_d(my_e2, i - 1) * r;
+      const double2 t = tde[i];
+      q = (t.x - x) - t.y * r;
       cnt += q < 0.0;
     }
     const unsigned long long m = __ballot(cnt >= 1);
@@ -287,21 +301,22 @@ __device__ __forceinline__ void emi_fast_point(const Src& coh_at, int n, int lan
   const double tiny = 2.2e-16 * scale + 1e-300;
   double my_dp = 0.0, my_dm = 0.0, my_rhi = 0.0;
   {
-    double dp = rl_d(my_d, 0) - lam;
+    double dp = tde[0].x - lam;
     if (fabs(dp) < tiny) dp = tiny;
     if (lane == 0) my_dp = dp;
     for (int i = 1; i < n; ++i) {
-      dp = (rl_d(my_d, i) - lam) - rl_d(my_e2, i - 1) / dp;
+      const double2 t = tde[i];
+      dp = (t.x - lam) - t.y / dp;
       if (fabs(dp) < tiny) dp = tiny;
       if (lane == i) my_dp = dp;
     }
-    double dm = rl_d(my_d, n - 1) - lam;
+    double dm = tde[n - 1].x - lam;
     if (fabs(dm) < tiny) dm = tiny;
     if (lane == n - 1) my_dm = dm;
     for (int i = n - 2; i >= 0; --i) {
-      const double ei = rl_d(my_e, i);
+      const double ei = tee[i];
       if (lane == i) my_rhi = -ei / dm;  // dm = D-_{i+1}
-      dm = (rl_d(my_d, i) - lam) - ei * ei / dm;
+      dm = (tde[i].x - lam) - ei * ei / dm;
       if (fabs(dm) < tiny) dm = tiny;
       if (lane == i) my_dm = dm;
     }
