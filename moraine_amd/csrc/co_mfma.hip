@@ -22,7 +22,7 @@ __global__ __launch_bounds__(64 * COM_WAVES) void co_mfma_kernel(
     const uint8_t* __restrict__ shp, long long npts, int aw, int rw, long long pix0,
     const int32_t* __restrict__ pairs, int npairs, float2* __restrict__ cov, float2* __restrict__ coh) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int wv = threadIdx.x >> 6;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: per-point pointers stay scalar
   const int lane = threadIdx.x & 63;
   const int win = aw * rw;
   const int az = (aw - 1) / 2, r = (rw - 1) / 2;
@@ -312,12 +312,17 @@ __global__ __launch_bounds__(64 * COW_WAVES) void co_mfma_wide_kernel(
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
       if (!on[s] || ta[s] != tb[s]) continue;
+      // select the diagonal entry this lane owns, then one reciprocal square root
+      bool own = false;
+      float d = 0.f;
 #pragma unroll
-      for (int q = 0; q < 16; ++q)
-        if (8 * (q >> 2) + 4 * hf + (q & 3) == c32) {
-          const int m = 32 * ta[s] + c32;
-          rs[m] = m < n ? 1.f / sqrtf(ar[s][q]) : 1.f / sqrtf(0.f);
-        }
+      for (int q = 0; q < 16; ++q) {
+        const bool hit = 8 * (q >> 2) + 4 * hf + (q & 3) == c32;
+        own = own || hit;
+        d = hit ? ar[s][q] : d;
+      }
+      const int m = 32 * ta[s] + c32;
+      if (own) rs[m] = 1.f / sqrtf(m < n ? d : 0.f);
     }
     __syncthreads();
     // ---- output ------------------------------------------------------------------

@@ -176,12 +176,20 @@ template <bool TWO>
 __device__ __forceinline__ void co_store_scales(const CoAcc& acc, int n, int lane, float* sc) {
   sc[lane] = acc.s;
   const int hf = lane >> 5, c = lane & 31;
+  // the lane of column c whose row group holds row c owns the diagonal entry:
+  // select it first, then one reciprocal square root per lane
+  bool own = false;
+  float d0 = 0.f, d1 = 0.f;
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
-    if (8 * (q >> 2) + 4 * hf + (q & 3) == c) {
-      sc[64 + c] = c < n ? 1.f / sqrtf(acc.r00[q]) : 1.f / sqrtf(0.f);
-      if (TWO) sc[96 + c] = 32 + c < n ? 1.f / sqrtf(acc.r11[q]) : 1.f / sqrtf(0.f);
-    }
+    const bool hit = 8 * (q >> 2) + 4 * hf + (q & 3) == c;
+    own = own || hit;
+    d0 = hit ? acc.r00[q] : d0;
+    if (TWO) d1 = hit ? acc.r11[q] : d1;
+  }
+  if (own) {
+    sc[64 + c] = 1.f / sqrtf(c < n ? d0 : 0.f);
+    if (TWO) sc[96 + c] = 1.f / sqrtf(32 + c < n ? d1 : 0.f);
   }
   if (!TWO && lane >= 32) sc[64 + lane] = 1.f / sqrtf(0.f);
   __builtin_amdgcn_wave_barrier();
@@ -210,12 +218,13 @@ __device__ __forceinline__ void co_write_tile_direct(const f16x_t& re, const f16
                                                      int lane, const float* rs, float2* __restrict__ out) {
   const int hf = lane >> 5;
   const int B = 32 * tb + (lane & 31);
+  const float rb = rs[B];
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
     const int A = 32 * ta + 8 * (q >> 2) + 4 * hf + (q & 3);
     if (A < B && B < n) {
-      const float f = rs[A] * rs[B];
-      out[co_pair_index(A, B, n)] = make_float2(re[q] * f, im[q] * f);
+      const float f = rs[A] * rb;
+      out[(unsigned)co_pair_index(A, B, n)] = make_float2(re[q] * f, im[q] * f);
     }
     if ((q & 3) == 3) __builtin_amdgcn_sched_barrier(0);  // bounds the addresses in flight
   }
