@@ -74,6 +74,8 @@ constexpr int DPP_HALF_MIRROR = 0x141;
 constexpr int DPP_MIRROR = 0x140;
 constexpr int DPP_WAVE_SHR1 = 0x138;  // lane i <- lane i-1 (lane 0 <- 0)
 constexpr int DPP_WAVE_SHL1 = 0x130;  // lane i <- lane i+1 (lane 63 <- 0)
+constexpr int DPP_ROW_BCAST15 = 0x142;  // row r <- lane 15 of row r-1 (row 0 reads 0)
+constexpr int DPP_ROW_BCAST31 = 0x143;  // rows 2, 3 <- lane 31 (rows 0, 1 read 0)
 
 template <int CTRL>
 __device__ __forceinline__ int dpp_i(int v) {
@@ -99,20 +101,27 @@ __device__ __forceinline__ double rl_d(double v, int l) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// Sum over all 64 lanes, result uniform.  Fixed association order (deterministic).
+// Sum over all 64 lanes, result uniform (every lane must be active).  Fixed
+// association order: after the in-row steps every lane holds its row sum r_i;
+// row_bcast:15 then gives row 3 r3 + r2 and row 1 r1 + r0, and row_bcast:31
+// adds row 1 into row 3, so lane 63 holds (r2 + r3) + (r0 + r1).
 __device__ __forceinline__ float wsum_f(float v) {
   v += dpp_f<DPP_XOR1>(v);
   v += dpp_f<DPP_XOR2>(v);
   v += dpp_f<DPP_HALF_MIRROR>(v);
   v += dpp_f<DPP_MIRROR>(v);
-  return (rl_f(v, 0) + rl_f(v, 16)) + (rl_f(v, 32) + rl_f(v, 48));
+  v += dpp_f<DPP_ROW_BCAST15>(v);
+  v += dpp_f<DPP_ROW_BCAST31>(v);
+  return rl_f(v, 63);
 }
 __device__ __forceinline__ double wsum_d(double v) {
   v += dpp_d<DPP_XOR1>(v);
   v += dpp_d<DPP_XOR2>(v);
   v += dpp_d<DPP_HALF_MIRROR>(v);
   v += dpp_d<DPP_MIRROR>(v);
-  return (rl_d(v, 0) + rl_d(v, 16)) + (rl_d(v, 32) + rl_d(v, 48));
+  v += dpp_d<DPP_ROW_BCAST15>(v);
+  v += dpp_d<DPP_ROW_BCAST31>(v);
+  return rl_d(v, 63);
 }
 __device__ __forceinline__ double wmin_d(double v) {
   v = fmin(v, dpp_d<DPP_XOR1>(v));
@@ -165,7 +174,9 @@ __device__ __forceinline__ int wmax_i(int v) {
   v = max(v, dpp_i<DPP_XOR2>(v));
   v = max(v, dpp_i<DPP_HALF_MIRROR>(v));
   v = max(v, dpp_i<DPP_MIRROR>(v));
-  return __builtin_amdgcn_readfirstlane(max(max(rl_i(v, 0), rl_i(v, 16)), max(rl_i(v, 32), rl_i(v, 48))));
+  v = max(v, dpp_i<DPP_ROW_BCAST15>(v));  // lane 63 ends with the max of all four rows
+  v = max(v, dpp_i<DPP_ROW_BCAST31>(v));
+  return rl_i(v, 63);
 }
 // Pivot key for an integer max: the high word of a non-negative double (whose
 // bit patterns order as the values do) with its low BITS bits replaced by
