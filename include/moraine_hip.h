@@ -124,6 +124,18 @@ int mr_ds_temp_coh(const float* coh, const float* ph, long long n_pts, int n, co
 int mr_temp_coh(const float* intf, const float* rslc, long long n_pts, int n, const int32_t* pairs,
                 int n_pairs, float* t_coh, mr_stream_t stream);
 
+/* EMI followed by the DS temporal coherence of its own phases: the inner two
+ * stages of moraine.emperical_co_emi_temp_coh_pc (moraine/pl.py:148-168),
+ * emi (moraine/pl.py:68-77) then ds_temp_coh (moraine/pl.py:126-145) on the
+ * same coherence rows.  Results equal mr_emi followed by mr_ds_temp_coh bit
+ * for bit.  n_pairs must be n(n-1)/2 (EMI reads full rows in triu order;
+ * pairs names the image pair of each row entry, as for mr_ds_temp_coh).  For
+ * 32 < n <= 64 one kernel does both, reading each row from HBM once.
+ * workspace >= mr_emi_workspace_bytes(n_pts, n). */
+int mr_emi_ds_temp_coh(const float* coh, long long n_pts, int n, int ref, const int32_t* pairs, int n_pairs,
+                       float* ph, float* quality, float* t_coh, void* workspace, size_t workspace_bytes,
+                       mr_stream_t stream);
+
 /* ---------------------------------------------------------------------------
  * Batched DS form.  No single reference function: it is the batch driver
  * moraine.emperical_co_emi_temp_coh_pc (moraine/pl.py:148-168, i.e.

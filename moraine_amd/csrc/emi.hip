@@ -26,6 +26,7 @@
 // per row would leave half the wave idle).  General path (n > 200): one
 // 256-thread block per point, everything FP64 in a per-block global workspace.
 #include "emi_fast.hpp"
+#include "tcoh.hpp"
 
 namespace mr {
 
@@ -118,10 +119,14 @@ __device__ void tridiag_min_eigpair(const double* d, const double* ae, const dou
 // ----------------------------------------------------------------------------
 constexpr int EMI_FAST_THREADS = 64;
 
-template <int NM>
+// TC: also the DS temporal coherence of each point from the staged row
+// (moraine/pl.py:126-145), as mr_ds_temp_coh computes it from (coh, ph, pairs).
+template <int NM, bool TC>
 __global__ __launch_bounds__(EMI_FAST_THREADS) void emi_fast_kernel(const float2* __restrict__ coh, long long npts,
                                                                     int n, int ref, float2* __restrict__ ph,
-                                                                    float* __restrict__ quality) {
+                                                                    float* __restrict__ quality,
+                                                                    const int32_t* __restrict__ pairs,
+                                                                    float* __restrict__ t_coh) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int P = n * (n - 1) / 2;
   const int lane0 = threadIdx.x;
@@ -144,6 +149,16 @@ __global__ __launch_bounds__(EMI_FAST_THREADS) void emi_fast_kernel(const float2
     const float2 pv = emi_link_phase(vr, vi, ref);
     if (lane < n) ph[pt * n + lane] = pv;
     if (lane == 0) quality[pt] = (float)rq;
+    if (TC) {
+      float2* phb = reinterpret_cast<float2*>(work);  // the EMI scratch is free again
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      phb[lane] = pv;
+      __builtin_amdgcn_wave_barrier();
+      const float tc = tcoh_point<true>([&](int p, int, int) { return cs[p]; }, [&](int i) { return phb[i]; }, pairs,
+                                        P, lane);
+      if (lane == 0) t_coh[pt] = tc;
+    }
     __builtin_amdgcn_wave_barrier();
   }
 }
@@ -443,8 +458,8 @@ int mr_emi(const float* coh, long long n_pts, int n, int ref, float* ph, float* 
   if (n > 32 && n <= 64) {
     const size_t lds = mr::emi_fast_lds(n, 64);
     const int nb = (int)std::min<long long>(n_pts, 1 << 20);
-    hipLaunchKernelGGL((mr::emi_fast_kernel<64>), dim3(nb), dim3(mr::EMI_FAST_THREADS), lds, st, c, n_pts, n, ref, p,
-                       quality);
+    hipLaunchKernelGGL((mr::emi_fast_kernel<64, false>), dim3(nb), dim3(mr::EMI_FAST_THREADS), lds, st, c, n_pts, n,
+                       ref, p, quality, nullptr, nullptr);
     return mr::launch_status("emi_fast");
   }
   if (n <= mr::EMI_WIDE_MAX_N) return mr::emi_wide_launch(c, n_pts, n, ref, p, quality, st);
@@ -455,6 +470,30 @@ int mr_emi(const float* coh, long long n_pts, int n, int ref, float* ph, float* 
   hipLaunchKernelGGL(mr::emi_general_kernel, dim3(nb), dim3(mr::EMI_GEN_THREADS), 0, st, c, n_pts, n, ref, p,
                      quality, reinterpret_cast<unsigned char*>(workspace));
   return mr::launch_status("emi_general");
+}
+
+int mr_emi_ds_temp_coh(const float* coh, long long n_pts, int n, int ref, const int32_t* pairs, int n_pairs,
+                       float* ph, float* quality, float* t_coh, void* workspace, size_t workspace_bytes,
+                       mr_stream_t stream) {
+  MR_CHECK_ARG(n >= 2, "emi_ds_temp_coh: need at least 2 images (n=%d)", n);
+  MR_CHECK_ARG(ref >= 0 && ref < n, "emi_ds_temp_coh: ref=%d outside [0,%d)", ref, n);
+  MR_CHECK_ARG(n_pts >= 0, "emi_ds_temp_coh: negative point count");
+  MR_CHECK_ARG(n_pairs == n * (n - 1) / 2, "emi_ds_temp_coh: coherence rows hold %d pairs, n=%d needs %d", n_pairs, n,
+               n * (n - 1) / 2);
+  if (n_pts == 0) return MR_OK;
+  MR_CHECK_ARG(coh && pairs && ph && quality && t_coh, "emi_ds_temp_coh: null pointer");
+  if (n > 32 && n <= 64) {
+    hipStream_t st = mr::as_stream(stream);
+    const size_t lds = mr::emi_fast_lds(n, 64);
+    const int nb = (int)std::min<long long>(n_pts, 1 << 20);
+    hipLaunchKernelGGL((mr::emi_fast_kernel<64, true>), dim3(nb), dim3(mr::EMI_FAST_THREADS), lds, st,
+                       reinterpret_cast<const float2*>(coh), n_pts, n, ref, reinterpret_cast<float2*>(ph), quality,
+                       pairs, t_coh);
+    return mr::launch_status("emi_ds_temp_coh");
+  }
+  const int rc = mr_emi(coh, n_pts, n, ref, ph, quality, workspace, workspace_bytes, stream);
+  if (rc != MR_OK) return rc;
+  return mr_ds_temp_coh(coh, ph, n_pts, n, pairs, n_pairs, t_coh, stream);
 }
 
 }  // extern "C"

@@ -59,6 +59,33 @@ def ds_temp_coh(coh, ph, image_pairs=None, block_size: int = 128):
     return _tcoh("mr_ds_temp_coh", coh, ph, image_pairs)
 
 
+def _emi_tcoh_dev(c: torch.Tensor, n: int, ref: int, ph: torch.Tensor, q: torch.Tensor, t: torch.Tensor):
+    """emi then ds_temp_coh on the same full-network rows, into ph, q, t (one
+    kernel for 32 < n <= 64; results equal the two calls bit for bit)."""
+    npts = c.shape[0]
+    pairs = _dev.to_dev(_pairs(None, n)).to(c.device)
+    wsb = _lib.query("mr_emi_workspace_bytes", npts, n)
+    ws = _dev.workspace(wsb, c.device)
+    _lib.call("mr_emi_ds_temp_coh", _dev.ptr(c), npts, n, int(ref), _dev.ptr(pairs), pairs.shape[0], _dev.ptr(ph),
+              _dev.ptr(q), _dev.ptr(t), _dev.ptr(ws), ws.numel(), _dev.stream())
+
+
+def emi_ds_temp_coh(coh, ref: int = 0):
+    """(ph, quality, t_coh) = emi(coh, ref) followed by ds_temp_coh(coh, ph):
+    the inner stages of moraine.emperical_co_emi_temp_coh_pc (moraine/pl.py:165-167)
+    for full-network coherence rows (n_pts, N(N-1)/2) in triu order."""
+    as_torch = _dev.is_torch(coh)
+    c = _dev.to_dev(coh, torch.complex64).reshape(-1, coh.shape[-1])
+    n = nimage_from_npair(c.shape[-1])
+    ph = torch.empty((c.shape[0], n), dtype=torch.complex64, device=c.device)
+    q = torch.empty(c.shape[0], dtype=torch.float32, device=c.device)
+    t = torch.empty(c.shape[0], dtype=torch.float32, device=c.device)
+    _emi_tcoh_dev(c, n, ref, ph, q, t)
+    lead = tuple(coh.shape[:-1])
+    return (_dev.back(ph.reshape(lead + (n,)), as_torch), _dev.back(q.reshape(lead), as_torch),
+            _dev.back(t.reshape(lead), as_torch))
+
+
 def emperical_co_emi_temp_coh_pc(rslc, idx, pc_is_shp, batch_size: int = 1000):
     """Coherence -> EMI -> temporal coherence for a point cloud, in batches.
 
@@ -79,6 +106,5 @@ def emperical_co_emi_temp_coh_pc(rslc, idx, pc_is_shp, batch_size: int = 1000):
     for s in range(0, n_pc, bs):
         e = min(n_pc, s + bs)
         c = emperical_co_pc(z, ix[s:e], m[s:e])
-        ph[s:e], q[s:e] = _emi_dev(c, n, 0)
-        t[s:e] = ds_temp_coh(c, ph[s:e])
+        _emi_tcoh_dev(c, n, 0, ph[s:e], q[s:e], t[s:e])
     return _dev.back(ph, as_torch), _dev.back(q, as_torch), _dev.back(t, as_torch)

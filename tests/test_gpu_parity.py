@@ -257,12 +257,14 @@ def test_tcoh_vs_oracle_and_no_mutation():
     np.testing.assert_allclose(ras.ravel(), t2, rtol=0, atol=0)
 
 
-def test_fused_batch_driver_equals_composition():
-    """nbs/API/pl.ipynb:704-714: the batch driver equals the separate calls exactly."""
+@pytest.mark.parametrize("n", [17, 45, 60, 64, 90])
+def test_fused_batch_driver_equals_composition(n):
+    """nbs/API/pl.ipynb:704-714: the batch driver equals the separate calls exactly
+    (for 32 < N <= 64 EMI and the temporal coherence run as one kernel)."""
     mr = _mr()
-    z = synth.ds_stack(12, 16, 17, seed=9, ps_frac=0.1)
+    z = synth.ds_stack(12, 16, n, seed=9, ps_frac=0.1)
     rmli = (np.abs(z) ** 2).astype(np.float32)
-    _case("fused batch driver")
+    _case(f"fused batch driver, N={n}")
     _, is_shp, shp_num = mr.ks_test_select_shp(rmli, 2, 3, 0.05, return_p=False)
     cand = shp_num >= 3
     idx = np.stack(np.nonzero(cand), -1).astype(np.int32)
@@ -274,6 +276,12 @@ def test_fused_batch_driver_equals_composition():
     np.testing.assert_array_equal(ph, ph2)
     np.testing.assert_array_equal(q, q2)
     np.testing.assert_array_equal(t, t2)
+    from moraine_amd.pl import emi_ds_temp_coh
+
+    ph3, q3, t3 = emi_ds_temp_coh(c)
+    np.testing.assert_array_equal(ph3, ph2)
+    np.testing.assert_array_equal(q3, q2)
+    np.testing.assert_array_equal(t3, t2)
 
 
 def test_emi_on_ks_selected_shps():
