@@ -7,7 +7,8 @@ pixel taken through the whole path.  One step = one pass over the tile:
 
     rmli = |rslc|^2
     p, is_shp, shp_num = ks_test + select_shp              (p materialised, as the drop-in returns it)
-    per batch of pixels: coh = emperical_co_pc -> (ph, quality) = emi -> t = ds_temp_coh
+    per batch of pixels: coh = emperical_co_pc -> (ph, quality, t) = emi then ds_temp_coh
+                         (the fused driver's inner step: one kernel for 32 < N <= 64)
 
 Inputs are generated on the device before timing (synthetic, seeded).  Prints
 one JSON line.  Algorithmic units per pixel: KS 4N + 5W + 4 bytes, coherence 4K N(N+1)
@@ -210,6 +211,7 @@ def main():
 
     import moraine_amd as mr
     from moraine_amd import synth
+    from moraine_amd.pl import _emi_tcoh_dev as emi_tcoh
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
@@ -257,10 +259,9 @@ def main():
             mark("co0")
             coh = mr.emperical_co_pc(rslc, idx[s:e], pcs[s:e])
             mark("co1")
-            ph[s:e], q[s:e] = mr.emi(coh)
+            # emperical_co_emi_temp_coh_pc's inner step, writing straight into ph, q, tc
+            emi_tcoh(coh, N, 0, ph[s:e], q[s:e], tc[s:e])
             mark("emi1")
-            tc[s:e] = mr.ds_temp_coh(coh, ph[s:e])
-            mark("tc1")
         state["p"], state["shp_num"], state["coh_last"] = p, shp_num, coh
 
     for _ in range(args.warmup):
@@ -288,7 +289,6 @@ def main():
     t_ks = span("rmli", "ks")
     t_co = span("co0", "co1")
     t_emi = span("co1", "emi1")
-    t_tc = span("emi1", "tc1")
     nb = len(batches)
 
     shp_num = state["shp_num"].float()
@@ -305,30 +305,26 @@ def main():
                                "hbm_frac": ks_bytes * npx / t_ks / 1e9 / HBM_PEAK_GBS},
         "emperical_co_pc": {"s": t_co, "TFLOP/s": co_flops * npx / t_co / 1e12,
                             "coh_write_GB/s": co_bytes * npx / t_co / 1e9},
-        "emi": {"s": t_emi, "TFLOP/s": emi_flops * npx / t_emi / 1e12},
-        "ds_temp_coh": {"s": t_tc, "GB/s": tc_bytes * npx / t_tc / 1e9,
-                        "hbm_frac": tc_bytes * npx / t_tc / 1e9 / HBM_PEAK_GBS},
+        "emi+ds_temp_coh": {"s": t_emi, "TFLOP/s": emi_flops * npx / t_emi / 1e12,
+                            "coh_read_GB/s": tc_bytes * npx / t_emi / 1e9},
     }
-    dom = max(["ks_test+select_shp", "emperical_co_pc", "emi", "ds_temp_coh"], key=lambda k: stages[k]["s"])
-    if dom == "emi":
+    dom = max(["ks_test+select_shp", "emperical_co_pc", "emi+ds_temp_coh"], key=lambda k: stages[k]["s"])
+    if dom == "emi+ds_temp_coh":
         ach = emi_flops * (npx / nb) / (t_emi / nb) / 1e12
-        roof = {"kernel": "emi_fast_kernel<64>", "bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS,
+        roof = {"kernel": "emi_fast_kernel<64, true>", "bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
-                "traffic": measured_traffic("emi_fast_kernel<64>", args),
-                "note": "VALU-bound (FP64 sweep + FP32 Householder); peak = MI355X FP64 = FP32 non-packed vector rate"}
+                "traffic": measured_traffic("emi_fast_kernel<64,tc>", args),
+                "note": "EMI with the DS temporal coherence fused; VALU-bound (FP64 sweep + FP32 Householder); "
+                        "peak = MI355X FP64 = FP32 non-packed vector rate; flops count EMI only"}
     elif dom == "ks_test+select_shp":
         ach = ks_bytes * npx / t_ks / 1e9
         roof = {"kernel": "ks_pairs_kernel<float>", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic("ks_pairs_kernel<float>", args)}
-    elif dom == "emperical_co_pc":
+    else:
         ach = co_flops * (npx / nb) / (t_co / nb) / 1e12
         roof = {"kernel": "co_mfma_kernel<false, true>", "bound": "mfma", "achieved": ach, "peak": 2500.0,
                 "unit": "TFLOP/s", "frac": ach / 2500.0,
                 "traffic": measured_traffic("co_mfma_kernel<pc,n<=64>", args)}
-    else:
-        ach = tc_bytes * npx / t_tc / 1e9
-        roof = {"kernel": "tcoh_kernel<true>", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic("tcoh_kernel<ds>", args)}
 
     # phase error of the last batch vs an FP64 solve (torch/rocSOLVER, complex128) on a sample
     coh = state["coh_last"]
