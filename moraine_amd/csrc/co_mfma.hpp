@@ -87,21 +87,23 @@ __device__ __forceinline__ CoAcc co_mfma_accumulate(const float2* __restrict__ b
   const int r0 = lane & 31;  // row of block 0 (= column of block 0 as B operand)
   const int r1 = 32 + (lane & 31);
   // ---- per-image power-of-two scale (lane = image) --------------------------
-  float mx0 = 0.f, mx1 = 0.f;
+  // eight SHPs per trip: the row lookups and gathers of a trip are all in
+  // flight before the first maximum needs them
+  float mx = 0.f;
   if (lane < n) {
     int k = 0;
-    for (; k + 1 < K; k += 2) {
-      const float2 a = base[(size_t)row(k) * n + lane];
-      const float2 c = base[(size_t)row(k + 1) * n + lane];
-      mx0 = fmaxf(mx0, fmaxf(fabsf(a.x), fabsf(a.y)));
-      mx1 = fmaxf(mx1, fmaxf(fabsf(c.x), fabsf(c.y)));
+    for (; k + 7 < K; k += 8) {
+      float2 a[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) a[u] = base[(size_t)row(k + u) * n + lane];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) mx = fmaxf(mx, fmaxf(fabsf(a[u].x), fabsf(a[u].y)));
     }
-    if (k < K) {
+    for (; k < K; ++k) {
       const float2 a = base[(size_t)row(k) * n + lane];
-      mx0 = fmaxf(mx0, fmaxf(fabsf(a.x), fabsf(a.y)));
+      mx = fmaxf(mx, fmaxf(fabsf(a.x), fabsf(a.y)));
     }
   }
-  const float mx = fmaxf(mx0, mx1);
   float s = 1.f;
   if (mx > 0.f && mx < INFINITY) {
     int e;

@@ -263,9 +263,9 @@ class DSServer:
             q = torch.empty(rows, dtype=torch.float32, device=self.dev)
             t = torch.empty(rows, dtype=torch.float32, device=self.dev)
             ws = self._emi_workspace(rows, n)
-            _lib.call("mr_emi", _dev.ptr(coh), rows, n, 0, _dev.ptr(ph), _dev.ptr(q), _dev.ptr(ws),
-                      0 if ws is None else ws.numel(), st)
-            _lib.call("mr_ds_temp_coh", _dev.ptr(coh), _dev.ptr(ph), rows, n, _dev.ptr(pairs), P, _dev.ptr(t), st)
+            # EMI then the DS temporal coherence (one kernel for 32 < n <= 64)
+            _lib.call("mr_emi_ds_temp_coh", _dev.ptr(coh), rows, n, 0, _dev.ptr(pairs), P, _dev.ptr(ph), _dev.ptr(q),
+                      _dev.ptr(t), _dev.ptr(ws), 0 if ws is None else ws.numel(), st)
             r0 = 0
             for a, take in chunks:
                 p0 = a.cursor
@@ -399,8 +399,7 @@ def run_serial(specs, coh_budget: int = 2 << 30, ps_frac: float = 0.1, keep_outp
         for b0 in range(0, s.npx, bs):
             b1 = min(s.npx, b0 + bs)
             c = emperical_co_pc(z, idx[b0:b1], m[b0:b1])
-            ph[b0:b1], q[b0:b1] = pl._emi_dev(c, s.n, 0)
-            t[b0:b1] = pl.ds_temp_coh(c, ph[b0:b1])
+            pl._emi_tcoh_dev(c, s.n, 0, ph[b0:b1], q[b0:b1], t[b0:b1])
         torch.cuda.synchronize()
         done = time.perf_counter() - t0
         out.append(PatchResult(s.rid, ph.view(s.h, s.w, s.n) if keep_outputs else None,
