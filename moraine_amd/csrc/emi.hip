@@ -15,13 +15,14 @@
 //   3. Householder reduction of the Hermitian M to real symmetric tridiagonal
 //      form T (unitary diagonal scaling absorbs the sub-diagonal phases).
 //   4. Smallest eigenvalue of T by multisection on Sturm counts (FP64, 64
-//      shifts per round, one per lane), eigenvector of T by inverse iteration
-//      with a pivoted tridiagonal LU (FP64), back-transformation through the
-//      stored reflectors.
+//      shifts per round, one per lane), eigenvector of T (register kernels: a
+//      twisted factorisation, emi_tri.hpp; general path: inverse iteration
+//      with a pivoted tridiagonal LU), back-transformation through the stored
+//      reflectors.
 // Fast path (32 < n <= 64): one wave per point, lane i owns row i of the
 // matrix in VGPRs (FP64 for step 1, FP32 complex for steps 2-4); a uniform
-// runtime column index is served by a switch (scalar branch tree + v_mov),
-// never by scratch memory.  n <= 32 and 64 < n <= 200: the same algorithm with
+// runtime column index is read with M0-relative moves (RegArr in
+// emi_common.hpp), never through scratch memory.  n <= 32 and 64 < n <= 200: the same algorithm with
 // two lanes per row (emi_wide.hip; a single wave for n <= 32, where one lane
 // per row would leave half the wave idle).  General path (n > 200): one
 // 256-thread block per point, everything FP64 in a per-block global workspace.
