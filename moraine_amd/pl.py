@@ -59,11 +59,23 @@ def ds_temp_coh(coh, ph, image_pairs=None, block_size: int = 128):
     return _tcoh("mr_ds_temp_coh", coh, ph, image_pairs)
 
 
+_FULL_PAIRS: dict = {}
+
+
+def _full_pairs_dev(n: int, device) -> torch.Tensor:
+    """The full pair network of n images in triu order, uploaded once per (n, device)
+    (a pageable upload per batch would block the host on the stream)."""
+    key = (n, str(device))
+    if key not in _FULL_PAIRS:
+        _FULL_PAIRS[key] = _dev.to_dev(_pairs(None, n)).to(device)
+    return _FULL_PAIRS[key]
+
+
 def _emi_tcoh_dev(c: torch.Tensor, n: int, ref: int, ph: torch.Tensor, q: torch.Tensor, t: torch.Tensor):
     """emi then ds_temp_coh on the same full-network rows, into ph, q, t (one
     kernel for 32 < n <= 64; results equal the two calls bit for bit)."""
     npts = c.shape[0]
-    pairs = _dev.to_dev(_pairs(None, n)).to(c.device)
+    pairs = _full_pairs_dev(n, c.device)
     wsb = _lib.query("mr_emi_workspace_bytes", npts, n)
     ws = _dev.workspace(wsb, c.device)
     _lib.call("mr_emi_ds_temp_coh", _dev.ptr(c), npts, n, int(ref), _dev.ptr(pairs), pairs.shape[0], _dev.ptr(ph),
