@@ -101,6 +101,9 @@ __device__ __forceinline__ double rl_d(double v, int l) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// value of lane l ^ 32 (exchange between the two half-waves)
+__device__ __forceinline__ float swap_half_f(float v) { return __shfl_xor(v, 32, 64); }
+
 // Sum over all 64 lanes, result uniform (every lane must be active).  Fixed
 // association order: after the in-row steps every lane holds its row sum r_i;
 // row_bcast:15 then gives row 3 r3 + r2 and row 1 r1 + r0, and row_bcast:31

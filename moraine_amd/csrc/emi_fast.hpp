@@ -108,7 +108,13 @@ __device__ __forceinline__ void emi_fast_point(const Src& coh_at, int n, int lan
   float my_ar = 0.f, my_ai = 0.f, my_hv = 0.f;
   float my_dr = 1.f, my_di = 0.f;
   float dcr = 1.f, dci = 0.f;  // running phase product D_k (uniform)
-  for (int k = 0; k + 2 < n; ++k) {
+  // Steps 0..30 run one lane per row.  From step 31 on the active block lies in
+  // rows/columns 32..63 and moves to two lanes per row (late_steps below), so
+  // the lanes of reduced rows stop issuing half of the remaining work.
+  constexpr bool kSplit = NM == 64;
+  const bool late = kSplit && n > 33;
+  const int k1 = late ? 31 : n - 2;
+  for (int k = 0; k < k1; ++k) {
     const float xr_raw = mr_.dyn(k);
     const float xi_raw = mi_.dyn(k);
     const bool below = lane > k && lane < n;
@@ -226,14 +232,151 @@ __device__ __forceinline__ void emi_fast_point(const Src& coh_at, int n, int lan
     }
     __builtin_amdgcn_wave_barrier();
   }
+  const int hh = lane >> 5, bb = lane & 31;  // late steps: block row 32 + bb, columns 32 + 2s + hh
+  if (late) {
+    // Step 31's column (rows 32..63) stays in register 31; then lane (bb, hh)
+    // takes M[32 + bb][32 + 2s + hh] into register 32 + s (s = 0..15): odd
+    // columns are the lane's own, even ones come from lane bb + 32.  Registers
+    // 32..47 of rows 0..31 are dead by now (only columns k of rows > k are
+    // read again, by the back-transformation).
+#pragma unroll
+    for (int s2 = 0; s2 < 16; ++s2) {
+      const float er = swap_half_f(mr_.get(32 + 2 * s2)), ei = swap_half_f(mi_.get(32 + 2 * s2));
+      const float nr = hh ? mr_.get(33 + 2 * s2) : er, ni = hh ? mi_.get(33 + 2 * s2) : ei;
+      mr_.set(32 + s2, nr);
+      mi_.set(32 + s2, ni);
+    }
+    float x31r = mr_.get(31), x31i = mi_.get(31);
+    {
+      const float pr = swap_half_f(x31r), pi = swap_half_f(x31i);  // lane bb takes row 32 + bb's
+      x31r = hh ? x31r : pr;
+      x31i = hh ? x31i : pi;
+    }
+    const float d31 = rl_f(mr_.get(31), 31);  // M[31][31] (row 31 never moved)
+    for (int k = 31; k + 2 < n; ++k) {
+      // column k, this lane's row R = 32 + bb (register 32 + (k-32)/2 on half (k-32)&1)
+      float xr_raw, xi_raw, dkk;
+      if (k == 31) {
+        xr_raw = x31r;
+        xi_raw = x31i;
+        dkk = d31;
+      } else {
+        const int s2 = (k - 32) >> 1, hk = (k - 32) & 1;
+        const float o_r = mr_.dyn(32 + s2), o_i = mi_.dyn(32 + s2);
+        const float p_r = swap_half_f(o_r), p_i = swap_half_f(o_i);
+        xr_raw = hh == hk ? o_r : p_r;
+        xi_raw = hh == hk ? o_i : p_i;
+        dkk = rl_f(xr_raw, k - 32);
+      }
+      const int R = 32 + bb;
+      const bool below = R > k && R < n;
+      const float xr = below ? xr_raw : 0.f, xi = below ? xi_raw : 0.f;
+      const float nrm2 = wsum_f(hh == 0 ? xr * xr + xi * xi : 0.f);
+      const float x0r = rl_f(xr, k + 1 - 32), x0i = rl_f(xi, k + 1 - 32);
+      const float nx = __builtin_amdgcn_sqrtf(nrm2);
+      const float ax0 = __builtin_amdgcn_sqrtf(x0r * x0r + x0i * x0i);
+      const float rax0 = ax0 > 0.f ? __builtin_amdgcn_rcpf(ax0) : 0.f;
+      const float phr = ax0 > 0.f ? x0r * rax0 : 1.f, phi = ax0 > 0.f ? x0i * rax0 : 0.f;
+      const float alr = -phr * nx, ali = -phi * nx;
+      const float den2 = 2.f * nx * (nx + ax0);
+      const float inv = den2 > 0.f ? __builtin_amdgcn_rsqf(den2) : 0.f;
+      float ur = (R == k + 1) ? (x0r - alr) : xr;
+      float ui = (R == k + 1) ? (x0i - ali) : xi;
+      ur *= inv;
+      ui *= inv;
+      if (lane == k) {
+        my_d = dkk;
+        my_e = nx;
+        my_ar = alr;
+        my_ai = ali;
+        my_hv = inv;
+      }
+      {
+        const float sr = nx > 0.f ? -phr : 1.f, si = nx > 0.f ? -phi : 0.f;
+        const float r = dcr * sr - dci * si, i = dcr * si + dci * sr;
+        dcr = r;
+        dci = i;
+      }
+      if (lane == k + 1) {
+        my_dr = dcr;
+        my_di = dci;
+      }
+      if (hh == 0) ub[R] = make_float2(ur, ui);
+      __builtin_amdgcn_wave_barrier();
+      // p = M22 u: each lane sums its eight interleaved columns per group, the
+      // two lanes of a row then add their partial sums
+      float pr = 0.f, pi = 0.f;
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const bool act = 32 + 2 * (4 * g + 3) + 1 > k;
+        if (act) {
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            const int s2 = 4 * g + t;
+            const float2 cu = ub[32 + 2 * s2 + hh];
+            pr = fmaf(mr_.get(32 + s2), cu.x, pr);
+            pr = fmaf(-mi_.get(32 + s2), cu.y, pr);
+            pi = fmaf(mr_.get(32 + s2), cu.y, pi);
+            pi = fmaf(mi_.get(32 + s2), cu.x, pi);
+          }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      pr += swap_half_f(pr);
+      pi += swap_half_f(pi);
+      const float cval = wsum_f(hh == 0 ? ur * pr + ui * pi : 0.f);  // real(u^H p)
+      const float qr = below ? 2.f * (pr - cval * ur) : 0.f;
+      const float qi = below ? 2.f * (pi - cval * ui) : 0.f;
+      if (hh == 0) qb[R] = make_float2(qr, qi);
+      __builtin_amdgcn_wave_barrier();
+      // M22 -= u q^H + q u^H over this lane's columns
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const bool act = 32 + 2 * (4 * g + 3) + 1 > k;
+        if (act) {
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            const int s2 = 4 * g + t;
+            const float2 cu = ub[32 + 2 * s2 + hh], cq = qb[32 + 2 * s2 + hh];
+            float r = mr_.get(32 + s2), i = mi_.get(32 + s2);
+            r = fmaf(-ur, cq.x, r);
+            r = fmaf(-ui, cq.y, r);
+            i = fmaf(ur, cq.y, i);
+            i = fmaf(-ui, cq.x, i);
+            r = fmaf(-qr, cu.x, r);
+            r = fmaf(-qi, cu.y, r);
+            i = fmaf(qr, cu.y, i);
+            i = fmaf(-qi, cu.x, i);
+            mr_.set(32 + s2, r);
+            mi_.set(32 + s2, i);
+          }
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
   {
     // trailing 2x2 block
-    const float r0 = mr_.dyn(n - 2);
-    const float i0 = mi_.dyn(n - 2);
-    const float r1 = mr_.dyn(n - 1);
-    const float dm2 = rl_f(r0, n - 2);
-    const float dm1 = rl_f(r1, n - 1);
-    const float er = rl_f(r0, n - 1), ei = rl_f(i0, n - 1);
+    float r0, i0, r1;
+    int l0, l1;  // lanes holding rows n-2 / n-1 of column n-2 (r0, i0) and of column n-1 (r1)
+    if (late) {
+      const int c0 = n - 2 - 32, c1 = n - 1 - 32;
+      r0 = mr_.dyn(32 + (c0 >> 1));
+      i0 = mi_.dyn(32 + (c0 >> 1));
+      r1 = mr_.dyn(32 + (c1 >> 1));
+      l0 = 32 * (c0 & 1) - 32;  // + row
+      l1 = 32 * (c1 & 1) - 32;
+    } else {
+      r0 = mr_.dyn(n - 2);
+      i0 = mi_.dyn(n - 2);
+      r1 = mr_.dyn(n - 1);
+      l0 = 0;
+      l1 = 0;
+    }
+    const float dm2 = rl_f(r0, n - 2 + l0);
+    const float dm1 = rl_f(r1, n - 1 + l1);
+    const float er = rl_f(r0, n - 1 + l0), ei = rl_f(i0, n - 1 + l0);
     const float ae = sqrtf(er * er + ei * ei);
     if (lane == n - 2) {
       my_d = dm2;
@@ -260,8 +403,19 @@ __device__ __forceinline__ void emi_fast_point(const Src& coh_at, int n, int lan
     vi = my_di * zi;
   }
   for (int k = n - 3; k >= 0; --k) {
-    const float xr = mr_.dyn(k);
-    const float xi = mi_.dyn(k);
+    float xr, xi;
+    if (late && k >= 32) {
+      // column k of the late block: lane i >= 32 (row i) owns it on half 1,
+      // else it sits on lane i - 32
+      const int s2 = (k - 32) >> 1;
+      const float o_r = mr_.dyn(32 + s2), o_i = mi_.dyn(32 + s2);
+      const float p_r = swap_half_f(o_r), p_i = swap_half_f(o_i);
+      xr = (k - 32) & 1 ? o_r : p_r;
+      xi = (k - 32) & 1 ? o_i : p_i;
+    } else {
+      xr = mr_.dyn(k);
+      xi = mi_.dyn(k);
+    }
     const bool below = lane > k && lane < n;
     const float alr = rl_f(my_ar, k), ali = rl_f(my_ai, k), hv = rl_f(my_hv, k);
     const float ar = below ? ((lane == k + 1) ? xr - alr : xr) * hv : 0.f;
