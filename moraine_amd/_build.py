@@ -21,11 +21,15 @@ ARCH = os.environ.get("MORAINE_HIP_ARCH", "gfx950")
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 
 # packed-FP32 instructions are disabled for all device code (the compiler would
-# otherwise form v_pk_* from plain float code).
+# otherwise form v_pk_* from plain float code).  MFMA accumulators are kept in
+# VGPRs rather than AGPRs: AGPRs add to the per-wave register total for the
+# whole kernel, so a kernel whose MFMA phase is followed by a register-heavy
+# VALU phase (the fused batch kernel) or whose tiles need many accumulators
+# (the 4-wave coherence kernel) would drop to one wave per SIMD.
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
     "-Wno-unused-variable", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-munsafe-fp-atomics",
+    "-munsafe-fp-atomics", "-mllvm", "-amdgpu-mfma-vgpr-form",
 ]
 
 
