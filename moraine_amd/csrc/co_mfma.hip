@@ -126,16 +126,17 @@ __global__ __launch_bounds__(64 * COM_WAVES) void co_mfma_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// 96 < n <= 128 (point-cloud form): one 4-wave workgroup per point.  The
+// 64 < n <= 128 (point-cloud form): one 4-wave workgroup per point.  The
 // upper tiles of the NB x NB grid of 32x32 blocks (NB = 4: 10 tiles) are split
 // over the waves (tile t = wave + 4 s, at most 3 per wave); each wave accumulates its tiles with the
 // same FP16 hi+lo MFMA arithmetic and per-image power-of-two scaling as the
 // one-wave kernel, the diagonal tiles give 1/sqrt(S'_mm) through LDS, and each
 // wave writes its tiles (straight from the accumulators for the canonical pair
-// list, through an LDS tile otherwise).  Measured against the VALU kernel on the
-// same SHP sets: 1.57x faster at n = 120 (21x41 windows), slightly slower at
-// n = 90 (3x3 grid), which therefore stays on the VALU kernel.  PF prefetches
-// one chunk of operands ahead (off: the registers cost more than it hides).
+// list, through an LDS tile otherwise).  NB = 3 (6 tiles) serves 65 <= n <= 96.
+// With the accumulators in VGPRs (two waves per SIMD at NB = 4, three at NB = 3)
+// it is 2.1x / 2.8x faster than the VALU kernel on the same SHP sets at n = 120
+// (21x41 windows) / n = 90 (15x21).  PF prefetches one chunk of operands ahead
+// (off: the registers cost more than it hides).
 // ---------------------------------------------------------------------------
 constexpr int COW_WAVES = 4;
 
@@ -355,8 +356,12 @@ __global__ __launch_bounds__(64 * COW_WAVES) void co_mfma_wide_kernel(
 int co_mfma_wide_launch(const float2* rslc, int nl, int w, int n, const int32_t* idx, const uint8_t* shp,
                         long long npts, int aw, int rw, const int32_t* pairs, int npairs, float2* coh, hipStream_t st) {
   const int nb = (int)std::min<long long>(npts, 1 << 20);
-  hipLaunchKernelGGL((co_mfma_wide_kernel<false, 4, false>), dim3(nb), dim3(64 * COW_WAVES), cow_lds_bytes(aw * rw),
-                     st, rslc, nl, w, n, idx, shp, npts, aw, rw, pairs, npairs, coh, nullptr, nullptr, nullptr, 0ll);
+  if (n <= 96)
+    hipLaunchKernelGGL((co_mfma_wide_kernel<false, 3, false>), dim3(nb), dim3(64 * COW_WAVES), cow_lds_bytes(aw * rw),
+                       st, rslc, nl, w, n, idx, shp, npts, aw, rw, pairs, npairs, coh, nullptr, nullptr, nullptr, 0ll);
+  else
+    hipLaunchKernelGGL((co_mfma_wide_kernel<false, 4, false>), dim3(nb), dim3(64 * COW_WAVES), cow_lds_bytes(aw * rw),
+                       st, rslc, nl, w, n, idx, shp, npts, aw, rw, pairs, npairs, coh, nullptr, nullptr, nullptr, 0ll);
   return launch_status("co_mfma_wide");
 }
 
@@ -364,8 +369,12 @@ int co_mfma_wide_samples_launch(const float2* samples, const long long* row_off,
                                 const int32_t* pix, long long pix0, long long cn, int n, const int32_t* pairs,
                                 int npairs, float2* coh, hipStream_t st) {
   const int nb = (int)std::min<long long>(cn, 1 << 20);
-  hipLaunchKernelGGL((co_mfma_wide_kernel<true, 4, false>), dim3(nb), dim3(64 * COW_WAVES), cow_lds_bytes(1), st,
-                     samples, 0, 0, n, nullptr, nullptr, cn, 1, 1, pairs, npairs, coh, row_off, n_samp, pix, pix0);
+  if (n <= 96)
+    hipLaunchKernelGGL((co_mfma_wide_kernel<true, 3, false>), dim3(nb), dim3(64 * COW_WAVES), cow_lds_bytes(1), st,
+                       samples, 0, 0, n, nullptr, nullptr, cn, 1, 1, pairs, npairs, coh, row_off, n_samp, pix, pix0);
+  else
+    hipLaunchKernelGGL((co_mfma_wide_kernel<true, 4, false>), dim3(nb), dim3(64 * COW_WAVES), cow_lds_bytes(1), st,
+                       samples, 0, 0, n, nullptr, nullptr, cn, 1, 1, pairs, npairs, coh, row_off, n_samp, pix, pix0);
   return launch_status("ds_batch coherence (MFMA, 4 waves)");
 }
 
