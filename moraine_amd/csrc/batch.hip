@@ -145,8 +145,8 @@ int mr_ds_batch(const float* samples, const long long* row_off, const int32_t* n
   const size_t emi_ws_bytes = mr::align256(mr_emi_workspace_bytes(chunk, n));
   for (long long c0 = 0; c0 < n_pix; c0 += chunk) {
     const long long cn = std::min(chunk, n_pix - c0);
-    // the coherence kernel the drop-in uses for this n (MFMA up to 128, VALU above)
-    const bool wide = n > 64 && n <= 128;
+    // the coherence kernel the drop-in uses for this n (MFMA up to 224, VALU above)
+    const bool wide = n > 64 && n <= 224;
     const int rc0 =
         n <= 64 ? mr::co_mfma_samples_launch(smp, row_off, n_samples, pix, c0, cn, n, pairs, n_pairs, coh, st)
         : wide  ? mr::co_mfma_wide_samples_launch(smp, row_off, n_samples, pix, c0, cn, n, pairs, n_pairs, coh, st)

@@ -18,7 +18,7 @@ constexpr int CO_MFMA_MAX_N = 64;
 int co_mfma_wide_launch(const float2* rslc, int nl, int w, int n, const int32_t* idx, const uint8_t* shp,
                         long long npts, int aw, int rw, const int32_t* pairs, int npairs, float2* coh, hipStream_t st);
 constexpr int CO_MFMA_WIDE_MIN_N = 65;   // point-cloud form on the 4-wave MFMA kernel
-constexpr int CO_MFMA_WIDE_MAX_N = 128;  // for n in [65, 128]
+constexpr int CO_MFMA_WIDE_MAX_N = 224;  // for n in [65, 224]
 
 constexpr int CO_THREADS = 256;
 constexpr int CO_QB = 8;             // pairs per thread per batch

@@ -126,7 +126,7 @@ __global__ __launch_bounds__(64 * COM_WAVES) void co_mfma_kernel(
 }
 
 // ---------------------------------------------------------------------------
-// 64 < n <= 128 (point-cloud form): one 4-wave workgroup per point.  The
+// 64 < n <= 224 (point-cloud form): one 4- or 8-wave workgroup per point.  The
 // upper tiles of the NB x NB grid of 32x32 blocks (NB = 4: 10 tiles) are split
 // over the waves (tile t = wave + 4 s, at most 3 per wave); each wave accumulates its tiles with the
 // same FP16 hi+lo MFMA arithmetic and per-image power-of-two scaling as the
@@ -138,28 +138,28 @@ __global__ __launch_bounds__(64 * COM_WAVES) void co_mfma_kernel(
 // (21x41 windows) / n = 90 (15x21).  PF prefetches one chunk of operands ahead
 // (off: the registers cost more than it hides).
 // ---------------------------------------------------------------------------
-constexpr int COW_WAVES = 4;
-
 // upper tiles of the NB x NB block grid in row-major order: t -> (a, b), a <= b
 template <int NB>
 __device__ __forceinline__ void cow_tile_ab(int t, int& a, int& b) {
-  if (NB == 3) {
-    a = t < 3 ? 0 : (t < 5 ? 1 : 2);
-    b = a + t - (a == 0 ? 0 : (a == 1 ? 3 : 5));
-  } else {
-    a = t < 4 ? 0 : (t < 7 ? 1 : (t < 9 ? 2 : 3));
-    b = a + t - (a == 0 ? 0 : (a == 1 ? 4 : (a == 2 ? 7 : 9)));
+  a = 0;
+#pragma unroll
+  for (int i = 0; i + 1 < NB; ++i) {
+    const bool past = a == i && t >= NB - i;  // beyond row i (NB - i tiles)
+    t = past ? t - (NB - i) : t;
+    a += past ? 1 : 0;
   }
+  b = a + t;
 }
 
-__host__ __device__ inline size_t cow_lds_bytes(int win) {
-  return (((size_t)win * 4 + 15) & ~(size_t)15) + 2 * 128 * 4 + (size_t)COW_WAVES * 32 * 32 * 8 + 16;
+// list | scales (32 NB) | 1/sqrt(diag) (32 NB) | one staged tile per wave | K
+__host__ __device__ inline size_t cow_lds_bytes(int win, int nb, int waves) {
+  return (((size_t)win * 4 + 15) & ~(size_t)15) + (size_t)2 * 32 * nb * 4 + (size_t)waves * 32 * 32 * 8 + 16;
 }
 
 // SAMP: point pt is pixel pix[pix0 + pt], its K = n_samp[id] rows are the
 // sample matrix at rslc + row_off[id] (batched form), no window.
-template <bool SAMP, int NB, bool PF>
-__global__ __launch_bounds__(64 * COW_WAVES) void co_mfma_wide_kernel(
+template <bool SAMP, int NB, int W, bool PF>
+__global__ __launch_bounds__(64 * W) void co_mfma_wide_kernel(
     const float2* __restrict__ rslc, int nl, int w, int n, const int32_t* __restrict__ idx,
     const uint8_t* __restrict__ shp, long long npts, int aw, int rw, const int32_t* __restrict__ pairs, int npairs,
     float2* __restrict__ coh, const long long* __restrict__ row_off, const int32_t* __restrict__ n_samp,
@@ -168,10 +168,10 @@ __global__ __launch_bounds__(64 * COW_WAVES) void co_mfma_wide_kernel(
   const int win = aw * rw;
   const int az = (aw - 1) / 2, r = (rw - 1) / 2;
   int* list = reinterpret_cast<int*>(smem);
-  float* sc = reinterpret_cast<float*>(smem + (((size_t)win * 4 + 15) & ~(size_t)15));  // 128 scales
-  float* rs = sc + 128;                                                                 // 128 1/sqrt(diag)
-  float2* stage = reinterpret_cast<float2*>(rs + 128);                                  // 4 x 32x32
-  int* sK = reinterpret_cast<int*>(stage + COW_WAVES * 1024);
+  float* sc = reinterpret_cast<float*>(smem + (((size_t)win * 4 + 15) & ~(size_t)15));  // 32 NB scales
+  float* rs = sc + 32 * NB;                                                             // 32 NB 1/sqrt(diag)
+  float2* stage = reinterpret_cast<float2*>(rs + 32 * NB);                              // W x 32x32
+  int* sK = reinterpret_cast<int*>(stage + W * 1024);
   const int wv = threadIdx.x >> 6;
   const int lane0 = threadIdx.x & 63;
   const bool canon = co_pairs_canonical(pairs, npairs, n);
@@ -207,9 +207,9 @@ __global__ __launch_bounds__(64 * COW_WAVES) void co_mfma_wide_kernel(
     }
     __syncthreads();
     const int K = sK[0];
-    // ---- per-image power-of-two scale: wave w, images 32w .. 32w+31 ------------
-    {
-      const int img = 32 * wv + c32;
+    // ---- per-image power-of-two scale: wave w, image blocks w, w + W, ... -------
+    for (int ib = wv; ib < NB; ib += W) {
+      const int img = 32 * ib + c32;
       float mx = 0.f;
       if (img < n) {
         // four independent loads in flight per lane
@@ -243,13 +243,13 @@ __global__ __launch_bounds__(64 * COW_WAVES) void co_mfma_wide_kernel(
     __syncthreads();
     // ---- accumulate this wave's tiles ------------------------------------------
     constexpr int NT = NB * (NB + 1) / 2;
-    constexpr int NS = (NT + COW_WAVES - 1) / COW_WAVES;  // tile slots per wave
+    constexpr int NS = (NT + W - 1) / W;  // tile slots per wave
     int ta[NS], tb[NS];
     bool on[NS];
     float sa[NS], sb[NS];
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-      const int t = wv + 4 * s;
+      const int t = wv + W * s;
       on[s] = t < NT;
       cow_tile_ab<NB>(on[s] ? t : 0, ta[s], tb[s]);
       sa[s] = sc[32 * ta[s] + c32];
@@ -353,15 +353,32 @@ __global__ __launch_bounds__(64 * COW_WAVES) void co_mfma_wide_kernel(
   }
 }
 
+// Tile grid and waves per workgroup for n: 3x3 / 4x4 grids on 4 waves, 5x5 to
+// 7x7 on 8 waves (at most four tiles per wave).
+#define MR_COW_DISPATCH(SAMP, LDS_WIN, ...)                                                                        \
+  do {                                                                                                          \
+    if (n <= 96)                                                                                                \
+      hipLaunchKernelGGL((co_mfma_wide_kernel<SAMP, 3, 4, false>), dim3(nb), dim3(64 * 4),                      \
+                         cow_lds_bytes(LDS_WIN, 3, 4), st, __VA_ARGS__);                                        \
+    else if (n <= 128)                                                                                          \
+      hipLaunchKernelGGL((co_mfma_wide_kernel<SAMP, 4, 4, false>), dim3(nb), dim3(64 * 4),                      \
+                         cow_lds_bytes(LDS_WIN, 4, 4), st, __VA_ARGS__);                                        \
+    else if (n <= 160)                                                                                          \
+      hipLaunchKernelGGL((co_mfma_wide_kernel<SAMP, 5, 8, false>), dim3(nb), dim3(64 * 8),                      \
+                         cow_lds_bytes(LDS_WIN, 5, 8), st, __VA_ARGS__);                                        \
+    else if (n <= 192)                                                                                          \
+      hipLaunchKernelGGL((co_mfma_wide_kernel<SAMP, 6, 8, false>), dim3(nb), dim3(64 * 8),                      \
+                         cow_lds_bytes(LDS_WIN, 6, 8), st, __VA_ARGS__);                                        \
+    else                                                                                                        \
+      hipLaunchKernelGGL((co_mfma_wide_kernel<SAMP, 7, 8, false>), dim3(nb), dim3(64 * 8),                      \
+                         cow_lds_bytes(LDS_WIN, 7, 8), st, __VA_ARGS__);                                        \
+  } while (0)
+
 int co_mfma_wide_launch(const float2* rslc, int nl, int w, int n, const int32_t* idx, const uint8_t* shp,
                         long long npts, int aw, int rw, const int32_t* pairs, int npairs, float2* coh, hipStream_t st) {
   const int nb = (int)std::min<long long>(npts, 1 << 20);
-  if (n <= 96)
-    hipLaunchKernelGGL((co_mfma_wide_kernel<false, 3, false>), dim3(nb), dim3(64 * COW_WAVES), cow_lds_bytes(aw * rw),
-                       st, rslc, nl, w, n, idx, shp, npts, aw, rw, pairs, npairs, coh, nullptr, nullptr, nullptr, 0ll);
-  else
-    hipLaunchKernelGGL((co_mfma_wide_kernel<false, 4, false>), dim3(nb), dim3(64 * COW_WAVES), cow_lds_bytes(aw * rw),
-                       st, rslc, nl, w, n, idx, shp, npts, aw, rw, pairs, npairs, coh, nullptr, nullptr, nullptr, 0ll);
+  MR_COW_DISPATCH(false, aw * rw, rslc, nl, w, n, idx, shp, npts, aw, rw, pairs, npairs, coh, nullptr, nullptr,
+                  nullptr, 0ll);
   return launch_status("co_mfma_wide");
 }
 
@@ -369,14 +386,11 @@ int co_mfma_wide_samples_launch(const float2* samples, const long long* row_off,
                                 const int32_t* pix, long long pix0, long long cn, int n, const int32_t* pairs,
                                 int npairs, float2* coh, hipStream_t st) {
   const int nb = (int)std::min<long long>(cn, 1 << 20);
-  if (n <= 96)
-    hipLaunchKernelGGL((co_mfma_wide_kernel<true, 3, false>), dim3(nb), dim3(64 * COW_WAVES), cow_lds_bytes(1), st,
-                       samples, 0, 0, n, nullptr, nullptr, cn, 1, 1, pairs, npairs, coh, row_off, n_samp, pix, pix0);
-  else
-    hipLaunchKernelGGL((co_mfma_wide_kernel<true, 4, false>), dim3(nb), dim3(64 * COW_WAVES), cow_lds_bytes(1), st,
-                       samples, 0, 0, n, nullptr, nullptr, cn, 1, 1, pairs, npairs, coh, row_off, n_samp, pix, pix0);
-  return launch_status("ds_batch coherence (MFMA, 4 waves)");
+  MR_COW_DISPATCH(true, 1, samples, 0, 0, n, nullptr, nullptr, cn, 1, 1, pairs, npairs, coh, row_off, n_samp, pix,
+                  pix0);
+  return launch_status("ds_batch coherence (MFMA, 4-8 waves)");
 }
+#undef MR_COW_DISPATCH
 
 // Sample-matrix form (batched path, per-stage mode): point pt is pixel
 // pix[pix0 + pt] with K = n_samp[id] rows at samples + row_off[id]; writes the

@@ -46,7 +46,7 @@ def test_batched_equals_plain_composition_mixed_n():
     from moraine_amd.batched import ds_batch
 
     cases = [(17, 20, 24, 3, 4, 11, 3), (60, 16, 20, 3, 5, 12, 4), (90, 12, 16, 4, 5, 13, 4),
-             (120, 10, 14, 3, 4, 14, 4)]
+             (120, 10, 14, 3, 4, 14, 4), (150, 8, 10, 3, 4, 15, 4)]
     mats, exp = [], []
     for n, nl, w, az, r, seed, mins in cases:
         _case(f"plain composition N={n} {nl}x{w} window {2*az+1}x{2*r+1}")

@@ -148,7 +148,8 @@ def test_co_small_vs_fp64():
     np.testing.assert_allclose(c_pc, g["coh_pc"], rtol=0, atol=1e-6)
 
 
-@pytest.mark.parametrize("n,az,r", [(60, 7, 10), (30, 5, 7), (90, 5, 7), (120, 4, 6), (128, 3, 4), (129, 3, 4)])
+@pytest.mark.parametrize("n,az,r", [(60, 7, 10), (30, 5, 7), (90, 5, 7), (120, 4, 6), (128, 3, 4), (129, 3, 4),
+                                    (150, 5, 7), (192, 3, 4), (200, 3, 4), (224, 2, 3), (225, 2, 3)])
 def test_co_pc_large_k_vs_fp64(n, az, r):
     mr = _mr()
     z = synth.ds_stack(20, 30, n, seed=n, ps_frac=0.1)
