@@ -218,6 +218,10 @@ __global__ __launch_bounds__(R == 2 ? 512 : 64 * EW_MAX_WAVES) void emi_wide_ker
     // u, 2p and the u^H p partials; q = 2p - 2 (u^H p) u is formed per column.
     float dcr = 1.f, dci = 0.f;  // running phase product D_k (uniform)
     if (threadIdx.x == 0) dph[0] = make_float2(1.f, 0.f);
+    // last matrix row held by this wave: once it is reduced (<= k) the wave's
+    // product and update terms are all zero, so it skips both loops and only
+    // takes part in the reductions and barriers
+    const int wave_last = __builtin_amdgcn_readfirstlane((64 * wv + 63) / R);
     for (int k = 0; k + 2 < n; ++k) {
       // per-step opaque copy of n: the column-group activity tests below are
       // recomputed each step (scalar compares) instead of being hoisted out of
@@ -266,7 +270,7 @@ __global__ __launch_bounds__(R == 2 ? 512 : 64 * EW_MAX_WAVES) void emi_wide_ker
       }
       // s = partial of M x over this lane's columns; m1 = M[row][k+1] (from its holder)
       float sr = 0.f, si = 0.f;
-      {
+      if (wave_last > k) {
         float2 cu[GH], nu[GH];
 #pragma unroll
         for (int t = 0; t < GH; ++t) cu[t] = xb[R * t + c];
@@ -320,7 +324,7 @@ __global__ __launch_bounds__(R == 2 ? 512 : 64 * EW_MAX_WAVES) void emi_wide_ker
       const float c2 = 2.f * cval;
       const float qr = 2.f * pr - c2 * ur, qi = 2.f * pi - c2 * ui;
       // M22 -= u q^H + q u^H  over this lane's active columns
-      {
+      if (wave_last > k) {
         float2 cu[GH], cp[GH], nu[GH], np[GH];
 #pragma unroll
         for (int t = 0; t < GH; ++t) {
