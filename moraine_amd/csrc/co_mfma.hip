@@ -53,21 +53,7 @@ __global__ __launch_bounds__(64 * COM_WAVES) void co_mfma_kernel(
       ms = shp + pt * win;
     }
     // ---- in-bounds SHPs, compacted in window order ---------------------------
-    int K = 0;
-    for (int t0 = 0; t0 < win; t0 += 64) {
-      const int t = t0 + lane;
-      bool v = false;
-      int lin = 0;
-      if (t < win) {
-        const int k = t / rw, l = t - (t / rw) * rw;
-        const int sa = ia - az + k, sr = ir - r + l;
-        v = ms[t] && sa >= 0 && sa < nl && sr >= 0 && sr < w;
-        lin = sa * w + sr;
-      }
-      const unsigned long long b = __ballot(v);
-      if (v) list[K + __popcll(b & ((1ull << lane) - 1ull))] = lin;
-      K += __popcll(b);
-    }
+    const int K = co_compact_shp(ms, win, rw, ia, ir, az, r, nl, w, lane, list);
     __builtin_amdgcn_wave_barrier();
     const CoAcc acc = co_mfma_accumulate<TWO>(rslc, RowList{list}, K, n, lane);
     __builtin_amdgcn_wave_barrier();  // the tile below overwrites the list
@@ -188,21 +174,7 @@ __global__ __launch_bounds__(64 * W) void co_mfma_wide_kernel(
     } else if (wv == 0) {
       const int ia = idx[2 * pt], ir = idx[2 * pt + 1];
       const uint8_t* ms = shp + pt * win;
-      int K = 0;
-      for (int t0 = 0; t0 < win; t0 += 64) {
-        const int t = t0 + lane;
-        bool v = false;
-        int lin = 0;
-        if (t < win) {
-          const int k = t / rw, l = t - (t / rw) * rw;
-          const int sa = ia - az + k, sr = ir - r + l;
-          v = ms[t] && sa >= 0 && sa < nl && sr >= 0 && sr < w;
-          lin = sa * w + sr;
-        }
-        const unsigned long long b = __ballot(v);
-        if (v) list[K + __popcll(b & ((1ull << lane) - 1ull))] = lin;
-        K += __popcll(b);
-      }
+      const int K = co_compact_shp(ms, win, rw, ia, ir, az, r, nl, w, lane, list);
       if (lane == 0) sK[0] = K;
     }
     __syncthreads();
@@ -212,24 +184,19 @@ __global__ __launch_bounds__(64 * W) void co_mfma_wide_kernel(
       const int img = 32 * ib + c32;
       float mx = 0.f;
       if (img < n) {
-        // four independent loads in flight per lane
-        float m1 = 0.f, m2 = 0.f, m3 = 0.f;
-        int k = hf;
-        for (; k + 6 < K; k += 8) {
-          const float2 z0 = base[(size_t)(SAMP ? k : list[k]) * n + img];
-          const float2 z1 = base[(size_t)(SAMP ? k + 2 : list[k + 2]) * n + img];
-          const float2 z2 = base[(size_t)(SAMP ? k + 4 : list[k + 4]) * n + img];
-          const float2 z3 = base[(size_t)(SAMP ? k + 6 : list[k + 6]) * n + img];
-          mx = fmaxf(mx, fmaxf(fabsf(z0.x), fabsf(z0.y)));
-          m1 = fmaxf(m1, fmaxf(fabsf(z1.x), fabsf(z1.y)));
-          m2 = fmaxf(m2, fmaxf(fabsf(z2.x), fabsf(z2.y)));
-          m3 = fmaxf(m3, fmaxf(fabsf(z3.x), fabsf(z3.y)));
+        // eight independent loads in flight per lane (the two half-waves take
+        // alternate SHPs); a short last trip re-reads SHP K-1, which leaves
+        // the maximum unchanged
+        for (int k0 = 0; k0 < K; k0 += 16) {
+          float2 z[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const int k = k0 + 2 * u + hf < K ? k0 + 2 * u + hf : K - 1;
+            z[u] = base[(size_t)(SAMP ? k : list[k]) * n + img];
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u) mx = fmaxf(mx, fmaxf(fabsf(z[u].x), fabsf(z[u].y)));
         }
-        for (; k < K; k += 2) {
-          const float2 z = base[(size_t)(SAMP ? k : list[k]) * n + img];
-          mx = fmaxf(mx, fmaxf(fabsf(z.x), fabsf(z.y)));
-        }
-        mx = fmaxf(fmaxf(mx, m1), fmaxf(m2, m3));
       }
       mx = fmaxf(mx, __shfl_xor(mx, 32, 64));
       float s = 1.f;

@@ -38,6 +38,34 @@ struct RowIdentity {  // k -> k (contiguous sample matrix)
   __device__ __forceinline__ int operator()(int k) const { return k; }
 };
 
+// In-bounds SHPs of a window in window order: list[0..K) = linear pixel index
+// (row * w + col) of every flagged window position inside the raster; returns
+// K (uniform).  The mask bytes of four 64-position groups are loaded before
+// the first ballot, so their latencies overlap.
+__device__ __forceinline__ int co_compact_shp(const uint8_t* __restrict__ ms, int win, int rw, int ia, int ir, int az,
+                                              int r, int nl, int w, int lane, int* list) {
+  int K = 0;
+  for (int t0 = 0; t0 < win; t0 += 256) {
+    uint8_t m[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int t = t0 + 64 * j + lane;
+      m[j] = t < win ? ms[t] : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int t = t0 + 64 * j + lane;
+      const int k = t / rw, l = t - (t / rw) * rw;
+      const int sa = ia - az + k, sr = ir - r + l;
+      const bool v = m[j] && sa >= 0 && sa < nl && sr >= 0 && sr < w;
+      const unsigned long long b = __ballot(v);
+      if (v) list[K + __popcll(b & ((1ull << lane) - 1ull))] = sa * w + sr;
+      K += __popcll(b);
+    }
+  }
+  return K;
+}
+
 // Split 4 complex values (8 floats: re0, im0, re1, im1, ...) into FP16 hi/lo
 // halves packed as the MFMA operand: hi = RTZ(x), lo = RTZ(x - hi) (exact
 // remainder, then rounded).  Also returns Y = (im, -re) pairs derived from the
@@ -87,21 +115,17 @@ __device__ __forceinline__ CoAcc co_mfma_accumulate(const float2* __restrict__ b
   const int r0 = lane & 31;  // row of block 0 (= column of block 0 as B operand)
   const int r1 = 32 + (lane & 31);
   // ---- per-image power-of-two scale (lane = image) --------------------------
-  // eight SHPs per trip: the row lookups and gathers of a trip are all in
-  // flight before the first maximum needs them
+  // sixteen SHPs per trip: the row lookups and gathers of a trip are all in
+  // flight before the first maximum needs them; a short last trip re-reads
+  // SHP K-1 (a repeated sample leaves the maximum unchanged)
   float mx = 0.f;
   if (lane < n) {
-    int k = 0;
-    for (; k + 7 < K; k += 8) {
-      float2 a[8];
+    for (int k = 0; k < K; k += 16) {
+      float2 a[16];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) a[u] = base[(size_t)row(k + u) * n + lane];
+      for (int u = 0; u < 16; ++u) a[u] = base[(size_t)row(k + u < K ? k + u : K - 1) * n + lane];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) mx = fmaxf(mx, fmaxf(fabsf(a[u].x), fabsf(a[u].y)));
-    }
-    for (; k < K; ++k) {
-      const float2 a = base[(size_t)row(k) * n + lane];
-      mx = fmaxf(mx, fmaxf(fabsf(a.x), fabsf(a.y)));
+      for (int u = 0; u < 16; ++u) mx = fmaxf(mx, fmaxf(fabsf(a[u].x), fabsf(a[u].y)));
     }
   }
   float s = 1.f;
