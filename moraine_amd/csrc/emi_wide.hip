@@ -101,6 +101,7 @@ __global__ __launch_bounds__(R == 2 ? 512 : 64 * EW_MAX_WAVES) void emi_wide_ker
   // column broadcast buffers read zero beyond n (rows >= n never write them)
   for (int t = threadIdx.x; t < NM; t += blockDim.x) {
     colb[t] = 0.0;
+    td[t] = 0.0;  // second column buffer of the sweep
     xb[t] = make_float2(0.f, 0.f);
     ub[t] = make_float2(0.f, 0.f);
     pb[t] = make_float2(0.f, 0.f);
@@ -134,38 +135,61 @@ __global__ __launch_bounds__(R == 2 ? 512 : 64 * EW_MAX_WAVES) void emi_wide_ker
     }
     const bool dlane = live && (c == row % R);  // this lane holds the diagonal entry of its row
     double dg = dlane ? 1.0 : 0.0;
-    bool swept = !dlane;
+    // Pivot: the unswept row with the largest |diagonal| (ties -> smallest
+    // row).  Every wave finds the next pivot itself: the diagonal after step
+    // st is d_i - (a_ik / a_kk) a_ik, computed from the broadcast column and
+    // the diagonal (both in LDS) with the same operations the diagonal's
+    // holder uses, so one workgroup barrier per step suffices.  The column,
+    // diagonal and pivot buffers alternate between two copies by step parity
+    // (td / te / dpl are free until the tridiagonal is formed); a wave can only
+    // overwrite a copy after every wave has passed the next step's barrier.
+    // The first pivot is row 0 (A's diagonal is all ones).
+    unsigned long long sw0 = 0, sw1 = 0, sw2 = 0, sw3 = 0;  // swept rows (uniform)
+    int k = 0;
     for (int st = 0; st < n; ++st) {
-      // pivot: unswept row with the largest |diagonal| (ties -> smallest row)
-      {
-        const int wk = wmax_i(pivot_key<8>(fabs(dg), !swept, row));
-        if (lane == 0) redi[wv] = wk;
-      }
-      __syncthreads();
-      int kb = redi[0];
-      for (int w2 = 1; w2 < W; ++w2) kb = max(kb, redi[w2]);
-      int k = 255 - (kb & 255);
-      k = min(__builtin_amdgcn_readfirstlane(k), n - 1);
+      const int b = st & 1;
+      double* cb = b ? td : colb;
+      double* dgl = b ? dpl : te;
       const bool holdk = c == k % R;
       const double own = a.dyn(k / R);
       const double aik = rowpick_d<R>(own, k % R);  // A[row][k] on every lane of the row
       // d-1 in place of the pivot: the rank-1 update then writes column k itself
       // (c - (c/d)(d-1) = c/d); the pivot's own diagonal becomes 2 - 1/d, which
       // the sweep never reads again and which is corrected when M is formed.
-      if (live && holdk) colb[row] = (row == k) ? aik - 1.0 : aik;
-      if (live && holdk && row == k) misc[0] = aik;
+      if (live && holdk) cb[row] = (row == k) ? aik - 1.0 : aik;
+      if (live && holdk && row == k) misc[6 + b] = aik;
+      if (dlane) dgl[row] = dg;
       __syncthreads();
-      const double rd = rcp_d(misc[0]);
+      const double rd = rcp_d(misc[6 + b]);
       const double f = !live ? 0.0 : (row == k) ? (1.0 - rd) : aik * rd;
+      {
+        const unsigned long long bit = 1ull << (k & 63);
+        const int wk = k >> 6;
+        sw0 |= wk == 0 ? bit : 0ull;
+        sw1 |= wk == 1 ? bit : 0ull;
+        sw2 |= wk == 2 ? bit : 0ull;
+        sw3 |= wk == 3 ? bit : 0ull;
+      }
+      int key = -1;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int i = lane + 64 * j;
+        const unsigned long long swj = j == 0 ? sw0 : j == 1 ? sw1 : j == 2 ? sw2 : sw3;
+        if (64 * j < n && i < n && !((swj >> lane) & 1ull)) {
+          const double ac = cb[i];
+          key = max(key, pivot_key<8>(fabs(fma(-(ac * rd), ac, dgl[i])), true, i));
+        }
+      }
+      const int kn = wmax_i(key);
       {
         double cur[GS], nxt[GS];
 #pragma unroll
-        for (int t = 0; t < GS; ++t) cur[t] = colb[R * t + c];
+        for (int t = 0; t < GS; ++t) cur[t] = cb[R * t + c];
 #pragma unroll
         for (int g = 0; g < E / GS; ++g) {
           if (g + 1 < E / GS) {
 #pragma unroll
-            for (int t = 0; t < GS; ++t) nxt[t] = colb[R * ((g + 1) * GS + t) + c];
+            for (int t = 0; t < GS; ++t) nxt[t] = cb[R * ((g + 1) * GS + t) + c];
           }
 #pragma unroll
           for (int t = 0; t < GS; ++t) a.set(g * GS + t, fma(-f, cur[t], a.get(g * GS + t)));
@@ -175,7 +199,7 @@ __global__ __launch_bounds__(R == 2 ? 512 : 64 * EW_MAX_WAVES) void emi_wide_ker
         }
       }
       dg = fma(-f, aik, dg);
-      swept = swept || (row == k);
+      k = min(__builtin_amdgcn_readfirstlane(255 - (kn & 255)), n - 1);
     }
     // a holds -inv(A) (diagonal entries offset by +2)
 
