@@ -117,7 +117,15 @@ __attribute__((amdgpu_waves_per_eu(E <= 64 ? 3 : 1))) void emi_wide_kernel(const
     int tid = threadIdx.x;
     __asm__ volatile("" : "+v"(tid));  // keeps per-lane addresses from being hoisted out of the point loop
     const int lane = tid & 63;
-    const int row = tid / R, c = tid % R;
+    // Row held by this lane (R adjacent lanes per row slot).  In blocks of more
+    // than four waves, wave w + 4 shares a SIMD with wave w; those extra waves
+    // take the first rows (reduced first), waves 0..3 the last 4 (64 / R), so a
+    // shared SIMD issues for two active waves during fewer Householder steps.
+    // Slots past the last row stay dead (row >= n), as in the identity order.
+    const int slot = tid / R, c = tid % R;
+    constexpr int S4 = 4 * (64 / R);  // row slots of waves 0..3
+    const int nx = W > 4 ? n - S4 : 0;
+    const int row = slot < S4 ? slot + nx : (slot < S4 + nx ? slot - S4 : slot);
     const bool live = row < n;
     const float2* src = coh + pt * (long long)P;
 
@@ -250,7 +258,7 @@ __attribute__((amdgpu_waves_per_eu(E <= 64 ? 3 : 1))) void emi_wide_kernel(const
     // last matrix row held by this wave: once it is reduced (<= k) the wave's
     // product and update terms are all zero, so it skips both loops and only
     // takes part in the reductions and barriers
-    const int wave_last = __builtin_amdgcn_readfirstlane((64 * wv + 63) / R);
+    const int wave_last = wmax_i(live ? row : -1);
     for (int k = 0; k + 2 < n; ++k) {
       // per-step opaque copy of n: the column-group activity tests below are
       // recomputed each step (scalar compares) instead of being hoisted out of
