@@ -6,7 +6,7 @@
 
 namespace mr {
 
-constexpr int COM_WAVES = 2;  // points per block (one per wave); 2 x ~25 KiB LDS per block
+constexpr int COM_WAVES = 2;  // points per block (one per wave); LDS per wave: com_wave_bytes (~9 KiB)
 
 // LDS per wave: the window list (while accumulating), then one 32x32 complex
 // tile at a time for the output pass, plus scales and 1/sqrt(diag).
