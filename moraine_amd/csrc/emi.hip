@@ -141,7 +141,21 @@ __global__ __launch_bounds__(EMI_FAST_THREADS) void emi_fast_kernel(const float2
     int lane = lane0;
     __asm__ volatile("" : "+v"(lane));
     const float2* src = coh + pt * (long long)P;
-    for (int t = lane; t < P; t += 64) cs[t] = src[t];
+    // stage the row: sixteen loads per lane are in flight before their LDS
+    // stores (a plain copy loop waits for every load before the next one)
+    for (int t0 = 0; t0 < P; t0 += 64 * 16) {
+      float2 v[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const int t = t0 + 64 * u + lane;
+        v[u] = t < P ? src[t] : make_float2(0.f, 0.f);
+      }
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const int t = t0 + 64 * u + lane;
+        if (t < P) cs[t] = v[u];
+      }
+    }
     __builtin_amdgcn_wave_barrier();
     float vr, vi;
     double rq;

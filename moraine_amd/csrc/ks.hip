@@ -160,10 +160,23 @@ template <typename Key>
 __device__ __forceinline__ void stage_stacks(Key* __restrict__ dst, const Key* __restrict__ src, int nstk, int n,
                                              int ns) {
   const int tot = nstk * n;
-  for (int i = threadIdx.x; i < tot; i += KS_THREADS) {
-    const int s = i / n;
-    const int e = i - s * n;
-    dst[s * ns + e] = src[i];
+  // eight loads per thread are in flight before their LDS stores (a plain
+  // copy loop waits for every load before issuing the next)
+  for (int i0 = 0; i0 < tot; i0 += 8 * KS_THREADS) {
+    Key v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = i0 + u * KS_THREADS + threadIdx.x;
+      v[u] = i < tot ? src[i] : (Key)0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int i = i0 + u * KS_THREADS + threadIdx.x;
+      if (i < tot) {
+        const int s = i / n;
+        dst[s * ns + (i - s * n)] = v[u];
+      }
+    }
   }
   for (int s = threadIdx.x; s < nstk; s += KS_THREADS) dst[s * ns + n] = ~(Key)0;  // sentinel
 }
