@@ -445,7 +445,11 @@ __attribute__((amdgpu_waves_per_eu(E <= 64 ? 3 : 1))) void emi_wide_kernel(const
     {
       const int G = 64 * W;
       const double rg = 1.0 / (double)(G + 1);
-      for (int round = 0; round < 6; ++round) {
+      // rounds until the bracket has shrunk by at least 65^5 ~ 1.2e9 (the
+      // one-wave kernels' five rounds of 64 shifts): five for one or two
+      // waves, four from three waves on
+      const int rounds = W >= 3 ? 4 : 5;
+      for (int round = 0; round < rounds; ++round) {
         if (hi - lo <= 2e-15 * scale) break;
         const int g = wv * 64 + lane;
         const double x = lo + (hi - lo) * (double)(g + 1) * rg;
@@ -484,7 +488,7 @@ __attribute__((amdgpu_waves_per_eu(E <= 64 ? 3 : 1))) void emi_wide_kernel(const
       if (fabs(dp) < tiny) dp = tiny;
       dpl[0] = dp;
       for (int i = 1; i < n; ++i) {
-        dp = (td[i] - lam) - te[i - 1] * te[i - 1] / dp;
+        dp = (td[i] - lam) - te[i - 1] * te[i - 1] * rcp_d(dp);
         if (fabs(dp) < tiny) dp = tiny;
         dpl[i] = dp;
       }
@@ -493,7 +497,7 @@ __attribute__((amdgpu_waves_per_eu(E <= 64 ? 3 : 1))) void emi_wide_kernel(const
       if (fabs(dm) < tiny) dm = tiny;
       dmi[n - 1] = dm;
       for (int i = n - 2; i >= 0; --i) {
-        dm = (td[i] - lam) - te[i] * te[i] / dm;
+        dm = (td[i] - lam) - te[i] * te[i] * rcp_d(dm);
         if (fabs(dm) < tiny) dm = tiny;
         dmi[i] = dm;
       }
@@ -523,13 +527,13 @@ __attribute__((amdgpu_waves_per_eu(E <= 64 ? 3 : 1))) void emi_wide_kernel(const
       double z = 1.0;
       zz[rtw] = 1.0;
       for (int i = rtw - 1; i >= 0; --i) {
-        z *= -te[i] / dpl[i];
+        z *= -te[i] * rcp_d(dpl[i]);
         zz[i] = z;
       }
     } else if (threadIdx.x == t2) {
       double z = 1.0;
       for (int i = rtw + 1; i < n; ++i) {
-        z *= -te[i - 1] / dmi[i];
+        z *= -te[i - 1] * rcp_d(dmi[i]);
         zz[i] = z;
       }
     }
