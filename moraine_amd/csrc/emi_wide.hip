@@ -119,9 +119,9 @@ __attribute__((amdgpu_waves_per_eu(E <= 64 ? 3 : 1))) void emi_wide_kernel(const
     const int lane = tid & 63;
     // Row held by this lane (R adjacent lanes per row slot).  In blocks of more
     // than four waves, wave w + 4 shares a SIMD with wave w; those extra waves
-    // take the first rows (reduced first), waves 0..3 the last 4 (64 / R), so a
-    // shared SIMD issues for two active waves during fewer Householder steps.
-    // Slots past the last row stay dead (row >= n), as in the identity order.
+    // take the first n - S4 rows (reduced first) and waves 0..3 the last S4
+    // rows, so a shared SIMD issues for two active waves during fewer
+    // Householder steps.  Slots past the last row stay dead (row >= n).
     const int slot = tid / R, c = tid % R;
     constexpr int S4 = 4 * (64 / R);  // row slots of waves 0..3
     const int nx = W > 4 ? n - S4 : 0;
