@@ -189,6 +189,11 @@ __device__ __forceinline__ void emi_fast_point(const Src& coh_at, int n, int lan
     __builtin_amdgcn_wave_barrier();
     // M22 -= u q^H + q u^H
     {
+      // opaque copies of k and n: the group tests are recomputed here as scalar
+      // compares instead of being carried over from the product loop as lane
+      // masks (a VGPR select and compare per group)
+      int ku = k, nu_ = n;
+      __asm__ volatile("" : "+s"(ku), "+s"(nu_));
       float2 cu[4], cq[4], nu[4], nq[4];
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
@@ -197,7 +202,7 @@ __device__ __forceinline__ void emi_fast_point(const Src& coh_at, int n, int lan
       }
 #pragma unroll
       for (int g = 0; g < NM / 4; ++g) {
-        const bool act = (g * 4 + 3 > k) && (g * 4 < n);
+        const bool act = (g * 4 + 3 > ku) && (g * 4 < nu_);
         if (g + 1 < NM / 4) {
 #pragma unroll
           for (int t = 0; t < 4; ++t) {
@@ -329,10 +334,13 @@ __device__ __forceinline__ void emi_fast_point(const Src& coh_at, int n, int lan
       const float qi = below ? 2.f * (pi - cval * ui) : 0.f;
       if (hh == 0) qb[R] = make_float2(qr, qi);
       __builtin_amdgcn_wave_barrier();
-      // M22 -= u q^H + q u^H over this lane's columns
+      // M22 -= u q^H + q u^H over this lane's columns (group tests on an
+      // opaque k, as in the steps before the split)
+      int ku = k;
+      __asm__ volatile("" : "+s"(ku));
 #pragma unroll
       for (int g = 0; g < 4; ++g) {
-        const bool act = 32 + 2 * (4 * g + 3) + 1 > k;
+        const bool act = 32 + 2 * (4 * g + 3) + 1 > ku;
         if (act) {
 #pragma unroll
           for (int t = 0; t < 4; ++t) {
