@@ -74,10 +74,11 @@ __host__ __device__ inline size_t ew_lds_bytes(int nm) {
 // E <= 64 (n <= 128, blocks of three or four waves) is held to 168 VGPRs so
 // three waves run per SIMD: 12 waves per CU instead of 8 (n = 66..96 1.6x
 // faster; n = 97..128 1.15x, with a few registers spilled outside the
-// Householder loops).
+// Householder loops).  E = 16 (n <= 32, one wave per point) is held to four
+// waves per SIMD.
 template <int R, int E, int GS, int GH>
 __global__ __launch_bounds__(R == 2 ? 512 : 64 * EW_MAX_WAVES)
-__attribute__((amdgpu_waves_per_eu(E <= 64 ? 3 : 1))) void emi_wide_kernel(const float2* __restrict__ coh, long long npts,
+__attribute__((amdgpu_waves_per_eu(E <= 16 ? 4 : E <= 64 ? 3 : 1))) void emi_wide_kernel(const float2* __restrict__ coh, long long npts,
                                                                      int n, int ref, float2* __restrict__ ph,
                                                                      float* __restrict__ quality) {
   constexpr int NM = R * E;
