@@ -154,7 +154,7 @@ __global__ __launch_bounds__(256) void ks_sort_kernel(const T* __restrict__ rmli
 // pairs
 // ---------------------------------------------------------------------------
 constexpr int KS_THREADS = 256;
-constexpr int KS_ILP = 4;
+// walks interleaved per thread: 4 or 6, picked per launch (ks_pick_ilp)
 
 template <typename Key>
 __device__ __forceinline__ void stage_stacks(Key* __restrict__ dst, const Key* __restrict__ src, int nstk, int n,
@@ -181,7 +181,7 @@ __device__ __forceinline__ void stage_stacks(Key* __restrict__ dst, const Key* _
   for (int s = threadIdx.x; s < nstk; s += KS_THREADS) dst[s * ns + n] = ~(Key)0;  // sentinel
 }
 
-template <typename T>
+template <typename T, int KS_ILP>
 __global__ __launch_bounds__(KS_THREADS) void ks_pairs_kernel(const typename KeyOf<T>::type* __restrict__ keys,
                                                               const uint8_t* __restrict__ valid, int nl, int w,
                                                               int n, int az, int r, int tw, KsTables<T> tab,
@@ -340,6 +340,16 @@ __global__ __launch_bounds__(256) void select_shp_kernel(const float* __restrict
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// Walks per thread for a window row of `pairs` (pixel, offset) walks per block:
+// a trip runs KS_THREADS x ilp walks and takes about ilp walk-times, so the
+// cost of a row is ceil(pairs / (KS_THREADS ilp)) * ilp; ties go to 6 (more
+// loads in flight).  E.g. 64 columns x 21 offsets: 4 -> two trips (8), 6 -> one
+// trip (6); 64 x 15: 4 -> one trip (4), 6 -> one trip (6).
+static int ks_pick_ilp(int pairs) {
+  auto cost = [&](int ilp) { return (int)div_up((unsigned long long)pairs, (unsigned long long)KS_THREADS * ilp) * ilp; };
+  return cost(6) <= cost(4) ? 6 : 4;
+}
+
 template <typename T>
 static int ks_test_impl(const T* rmli, int nl, int w, int n, int az, int r, double p_max, T* p_out, T* d_out,
                         uint8_t* m_out, int32_t* shp_num, void* ws, size_t ws_bytes, mr_stream_t stream_) {
@@ -388,8 +398,12 @@ static int ks_test_impl(const T* rmli, int nl, int w, int n, int az, int r, doub
     if (int rc2 = launch_status("ks_test count init")) return rc2;
   }
   dim3 grid(div_up(w, tw), nl);
-  hipLaunchKernelGGL((ks_pairs_kernel<T>), grid, dim3(KS_THREADS), lds, st, keys, valid, nl, w, n, az, r, tw, tab,
-                     p_out, d_out, m_out, shp_num);
+  if (ks_pick_ilp(tw * (2 * r + 1)) == 6)
+    hipLaunchKernelGGL((ks_pairs_kernel<T, 6>), grid, dim3(KS_THREADS), lds, st, keys, valid, nl, w, n, az, r, tw,
+                       tab, p_out, d_out, m_out, shp_num);
+  else
+    hipLaunchKernelGGL((ks_pairs_kernel<T, 4>), grid, dim3(KS_THREADS), lds, st, keys, valid, nl, w, n, az, r, tw,
+                       tab, p_out, d_out, m_out, shp_num);
   return launch_status("ks_pairs");
 }
 
