@@ -382,7 +382,9 @@ static int ks_test_impl(const T* rmli, int nl, int w, int n, int az, int r, doub
   // column tile: as wide as 64 pixels while LDS stays <= 48 KiB (>= 3 blocks/CU)
   const size_t stk = (size_t)(n + 1) * sizeof(Key);
   auto lds_for = [&](int tw) { return (2 * (size_t)tw + 2 * r) * stk + (size_t)tw * 4 + 2 * (size_t)tw + 2 * r + 16; };
-  int tw = 64;
+  // one trip of four walks per thread covers a window row when tw * rw <= 4 * KS_THREADS: a narrower tile
+  // than 64 columns then also leaves LDS for more blocks per CU
+  int tw = std::max(1, std::min(64, 4 * KS_THREADS / (2 * r + 1)));
   while (tw > 1 && lds_for(tw) > 48 * 1024) --tw;
   tw = std::min(tw, w);
   const size_t lds = lds_for(tw);
