@@ -318,8 +318,8 @@ def main():
                         "peak = MI355X FP64 = FP32 non-packed vector rate; flops count EMI only"}
     elif dom == "ks_test+select_shp":
         ach = ks_bytes * npx / t_ks / 1e9
-        roof = {"kernel": "ks_pairs_kernel<float>", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic("ks_pairs_kernel<float>", args)}
+        roof = {"kernel": "ks_pairs_kernel<float, 4>", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic("ks_pairs_kernel<float,4>", args)}
     else:
         ach = co_flops * (npx / nb) / (t_co / nb) / 1e12
         roof = {"kernel": "co_mfma_kernel<false, true>", "bound": "mfma", "achieved": ach, "peak": 2500.0,
