@@ -5,9 +5,10 @@
 // mr_temp_coh replaces moraine.temp_coh (moraine/pqm.py:124-159; numba :46-71;
 // kernel :99-121):  t = |sum_p intf_p conj(r_a) r_b / (|intf_p||r_a||r_b|)| / n_pairs.
 // One wave per point streams the point's pair row (coalesced, the HBM-bound
-// part), gathers the phase terms from an L1-resident N-vector, and reduces with
-// cross-lane shuffles.  Inputs are read only (the numba temp_coh normalises its
-// inputs in place; this one does not).
+// part), gathers the phase terms from the point's N-vector staged in LDS (the
+// gathers are data-dependent, so from global memory each would be a 64-address
+// load), and reduces with cross-lane shuffles.  Inputs are read only (the numba
+// temp_coh normalises its inputs in place; this one does not).
 #include "tcoh.hpp"
 
 namespace mr {
@@ -16,15 +17,20 @@ template <bool DS>
 __global__ __launch_bounds__(256) void tcoh_kernel(const float2* __restrict__ c, const float2* __restrict__ v,
                                                    long long npts, int n, const int32_t* __restrict__ pairs,
                                                    int npairs, float* __restrict__ out) {
+  extern __shared__ float2 tcoh_ph[];  // n phases per wave
   const int lane = threadIdx.x & 63;
+  float2* ph = tcoh_ph + (threadIdx.x >> 6) * n;
   const long long w0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
   for (long long pt = w0; pt < npts; pt += nw) {
     const float2* cp = c + pt * npairs;
     const float2* vp = v + pt * n;
-    const float t = tcoh_point<DS>([&](int p, int, int) { return cp[p]; }, [&](int i) { return vp[i]; }, pairs,
+    for (int i = lane; i < n; i += 64) ph[i] = vp[i];
+    __builtin_amdgcn_wave_barrier();
+    const float t = tcoh_point<DS>([&](int p, int, int) { return cp[p]; }, [&](int i) { return ph[i]; }, pairs,
                                    npairs, lane);
     if (lane == 0) out[pt] = t;
+    __builtin_amdgcn_wave_barrier();  // every lane is past its reads before the next point's phases land
   }
 }
 
@@ -32,10 +38,11 @@ template <bool DS>
 static int tcoh_launch(const float* c, const float* v, long long n_pts, int n, const int32_t* pairs, int n_pairs,
                        float* out, mr_stream_t stream, const char* name) {
   MR_CHECK_ARG(n_pts >= 0 && n >= 1 && n_pairs >= 0, "%s: bad sizes", name);
+  MR_CHECK_ARG(n <= 4096, "%s: n=%d above 4096 (the phases of a point are staged in LDS)", name, n);
   if (n_pts == 0) return MR_OK;
   MR_CHECK_ARG(c && v && pairs && out, "%s: null pointer", name);
   const int nb = (int)std::min<long long>(div_up((unsigned long long)n_pts, 4), 1 << 16);
-  hipLaunchKernelGGL((tcoh_kernel<DS>), dim3(nb), dim3(256), 0, as_stream(stream),
+  hipLaunchKernelGGL((tcoh_kernel<DS>), dim3(nb), dim3(256), (size_t)4 * n * sizeof(float2), as_stream(stream),
                      reinterpret_cast<const float2*>(c), reinterpret_cast<const float2*>(v), n_pts, n, pairs,
                      n_pairs, out);
   return launch_status(name);
