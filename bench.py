@@ -44,8 +44,9 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=160, help="CPU baseline: interior sub-tile edge")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--err-sample", type=int, default=2048)
-    ap.add_argument("--forms", default="plain,batched,serving",
-                    help="comma list: plain (drop-in pipeline, the headline), batched, serving")
+    ap.add_argument("--forms", default="plain,batched,serving,rows",
+                    help="comma list: plain (drop-in pipeline, the headline), batched, serving, rows (each "
+                         "drop-in function timed alone)")
     ap.add_argument("--batched-px", type=int, default=200000, help="pixels of the ragged batch")
     ap.add_argument("--serve-requests", type=int, default=16)
     ap.add_argument("--serve-edge", type=int, default=512, help="largest tile edge of the request stream")
@@ -176,6 +177,91 @@ def batched_leg(args, torch):
         "phase_rms_vs_fp64": {"median": float(np.median(rms_all)), "p99": float(np.quantile(rms_all, 0.99)),
                               "max": float(rms_all.max()), "pixels": int(rms_all.size)},
     }
+
+
+def rows_leg(args, torch):
+    """Device time of each drop-in function on its own (synthetic N=60 inputs, 15x21 windows), with its
+    algorithmic bytes or flops per call and the fraction of the MI355X peak that bounds it (HBM 8 TB/s;
+    FP64 / non-packed FP32 vector 78.6 TFLOP/s for EMI; FP16 dense MFMA 2.5 PFLOP/s for coherence)."""
+    import moraine_amd as mr
+    from moraine_amd import synth
+    from moraine_amd.pl import emi_ds_temp_coh
+
+    def dtime(fn, reps=3):
+        fn()
+        torch.cuda.synchronize()
+        best = float("inf")
+        for _ in range(reps):
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record()
+            fn()
+            b.record()
+            b.synchronize()
+            best = min(best, a.elapsed_time(b) / 1e3)
+        return best
+
+    def hbm(name, t, nbytes, what):
+        rows[name] = {"s": t, "GB/s": nbytes / t / 1e9, "frac": nbytes / t / (HBM_PEAK_GBS * 1e9), "bound": "hbm",
+                      "per_call": what}
+
+    def flops(name, t, nflop, peak, what):
+        rows[name] = {"s": t, "TFLOP/s": nflop / t / 1e12, "frac": nflop / t / (peak * 1e12), "bound": "compute",
+                      "per_call": what}
+
+    rows = {}
+    N, az, r = 60, 7, 10
+    aw, rw = 2 * az + 1, 2 * r + 1
+    W, P = aw * rw, N * (N - 1) // 2
+    S = 1024
+    z = synth.ds_stack_torch(S, S, N, seed=5, ps_frac=0.1, device="cuda")
+    x = torch.view_as_real(z)
+    rmli = (x[..., 0] * x[..., 0] + x[..., 1] * x[..., 1]).contiguous()
+    del x
+    npx = S * S
+    hbm("ks_test", dtime(lambda: mr.ks_test(rmli, az, r)), npx * (4 * N + 4 * W),
+        f"{S}x{S}x{N}, rmli in, p out")
+    p = mr.ks_test(rmli, az, r)
+    hbm("select_shp", dtime(lambda: mr.select_shp(p, 0.05)), npx * (4 * W + W + 4), "p in, mask + count out")
+    del p
+    hbm("ks_test_select_shp", dtime(lambda: mr.ks_test_select_shp(rmli, az, r, 0.05, return_p=False)),
+        npx * (4 * N + W + 4), "rmli in, mask + count out")
+    _, shp, cnt = mr.ks_test_select_shp(rmli, az, r, 0.05, return_p=False)
+    kmean = float(cnt.float().mean())
+    co_fl = 4.0 * kmean * N * (N + 1)
+    # raster coherence on a 256x256 corner (cov and coh: 2 N^2 complex outputs per pixel)
+    zs, ss = z[:256, :256].contiguous(), shp[:256, :256].contiguous()
+    ks = float(ss.reshape(256 * 256, -1).sum(1).float().mean())  # SHP flags per pixel (the kernel skips those outside the corner)
+    flops("emperical_co", dtime(lambda: mr.emperical_co(zs, ss)), 256 * 256 * 4.0 * ks * N * (N + 1), 2500.0,
+          "256x256 raster, cov + coh (N x N each)")
+    ii, jj = torch.meshgrid(torch.arange(512, device="cuda", dtype=torch.int32),
+                            torch.arange(512, device="cuda", dtype=torch.int32), indexing="ij")
+    idx = torch.stack([ii.reshape(-1), jj.reshape(-1)], -1).contiguous()
+    pcs = shp[:512, :512].reshape(-1, aw, rw).contiguous()
+    npt = idx.shape[0]
+    flops("emperical_co_pc", dtime(lambda: mr.emperical_co_pc(z, idx, pcs)), npt * co_fl, 2500.0,
+          f"{npt} points (512x512), full pair network")
+    coh = mr.emperical_co_pc(z, idx, pcs)
+    ch = coh[:65536].contiguous()
+    hbm("uncompress_coh", dtime(lambda: mr.uncompress_coh(ch)), 65536 * (8 * P + 8 * N * N),
+        "65536 points, (P) -> (N, N)")
+    emi_fl = (2.0 + 16.0 / 3.0) * N ** 3
+    flops("emi", dtime(lambda: mr.emi(coh)), npt * emi_fl, FP64_PEAK_TFLOPS, f"{npt} points")
+    ph, q = mr.emi(coh)
+    hbm("ds_temp_coh", dtime(lambda: mr.ds_temp_coh(coh, ph)), npt * (8 * P + 8 * N + 4), f"{npt} points")
+    rs = z[:512, :512].reshape(-1, N).contiguous()
+    hbm("temp_coh", dtime(lambda: mr.temp_coh(coh, rs)), npt * (8 * P + 8 * N + 4), f"{npt} points")
+    flops("emi_ds_temp_coh", dtime(lambda: emi_ds_temp_coh(coh)), npt * emi_fl, FP64_PEAK_TFLOPS,
+          f"{npt} points, EMI flops only")
+    t = dtime(lambda: mr.emperical_co_emi_temp_coh_pc(z, idx, pcs, batch_size=npt), reps=2)
+    rows["emperical_co_emi_temp_coh_pc"] = {"s": t, "points/s": npt / t, "per_call": f"{npt} points, one batch"}
+    for n2, npt2 in ((120, 32768), (200, 16384)):
+        c2 = torch.randn(npt2, n2 * (n2 - 1) // 2, dtype=torch.complex64, device="cuda") * 0.1
+        flops(f"emi_n{n2}", dtime(lambda: mr.emi(c2), reps=2), npt2 * (2.0 + 16.0 / 3.0) * n2 ** 3,
+              FP64_PEAK_TFLOPS, f"{npt2} points, N={n2} (random coherence rows)")
+        del c2
+    return {"workload": f"N={N}, 15x21 windows, synthetic DS stack; each function timed alone (HIP events)",
+            "rows": rows}
 
 
 def serving_leg(args, torch):
@@ -351,6 +437,9 @@ def main():
         torch.cuda.empty_cache()
     if "serving" in wanted:
         forms["serving"] = serving_leg(args, torch)
+        torch.cuda.empty_cache()
+    if "rows" in wanted:
+        forms["rows"] = rows_leg(args, torch)
         torch.cuda.empty_cache()
     value = npx / dt
     out = {

@@ -68,6 +68,16 @@ def uncompress_coh(coh, image_pairs=None):
     else:
         pairs_np = np.asarray(image_pairs.cpu() if _dev.is_torch(image_pairs) else image_pairs, np.int32)
         n = int(pairs_np[-1, -1]) + 1
+        # NumPy's indexing rules: negative indices count from the end, others out of range raise
+        if np.any(pairs_np < -n) or np.any(pairs_np >= n):
+            raise IndexError(f"image pair index out of range for {n} images")
+        pairs_np = np.where(pairs_np < 0, pairs_np + n, pairs_np).astype(np.int32)
+    P = pairs_np.shape[0]
+    if c.shape[-1] != P:
+        if c.shape[-1] != 1:  # NumPy broadcasting: a last axis of 1 serves every pair
+            raise ValueError(f"coh has {c.shape[-1]} pairs, image_pairs {P}")
+        c = c.expand(tuple(c.shape[:-1]) + (P,))
+    c = c.contiguous()
     lead = tuple(c.shape[:-1])
     npts = int(np.prod(lead)) if lead else 1
     out = torch.empty(lead + (n, n), dtype=torch.complex64, device=c.device)

@@ -178,28 +178,103 @@ __global__ __launch_bounds__(CO_THREADS) void co_kernel(const float2* __restrict
   }
 }
 
-__global__ void uncompress_kernel(const float2* __restrict__ coh, long long npts, int n,
-                                  const int32_t* __restrict__ pairs, int npairs, float2* __restrict__ out) {
-  const long long nn = (long long)n * n;
-  const long long tot = npts * nn;
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (long long)gridDim.x * blockDim.x) {
-    const long long pt = t / nn;
-    const int e = (int)(t - pt * nn);
-    const int i = e / n, j = e - (e / n) * n;
-    out[t] = (i == j) ? make_float2(1.f, 0.f) : make_float2(0.f, 0.f);
-  }
-}
+// uncompress_coh, each output entry written once.  A block owns a band of
+// rows [row0, row0 + rows) of the N x N matrix and a run of points.  It first
+// resolves, for every entry of its band, which of the reference's three
+// assignments (moraine/co.py:296-298) lands there last, as a code in an LDS
+// table: 0 zero, 1 the unit diagonal, 2 + p the pair assignment
+// out[a_p, b_p] = coh[p], 2 + npairs + p the conjugate assignment
+// out[b_p, a_p] = conj(coh[p]).  Later assignments carry larger codes (the
+// conjugate pass after the direct one, a later duplicate after an earlier
+// one), so an LDS atomicMax resolves collisions as NumPy's in-order assignment
+// does, and the diagonal is set last.  The block then streams its points'
+// bands, consecutive lanes on consecutive entries, gathering coh through the
+// cache.  One band covers N <= 128.
+constexpr int UNC_LUT_ENTRIES = 16384;  // 64 KiB of codes per block
+constexpr int UNC_STAGE = 4096;         // rows of up to 32 KiB are staged in LDS
+constexpr int UNC_STAGE_GROUP = 2048;   // G points' rows per stage: G = max(1, this / n_pairs)
+constexpr int UNC_THREADS = 256;
 
-__global__ void uncompress_scatter_kernel(const float2* __restrict__ coh, long long npts, int n,
-                                          const int32_t* __restrict__ pairs, int npairs, float2* __restrict__ out) {
-  const long long tot = npts * npairs;
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (long long)gridDim.x * blockDim.x) {
-    const long long pt = t / npairs;
-    const int p = (int)(t - pt * npairs);
+// STAGE: the rows of G points at a time are copied to LDS (one coalesced read)
+// and gathered from there; otherwise the gathers go through the cache, where
+// the lower triangle's reads (a column of the upper triangle) touch one cache
+// line per lane
+template <bool STAGE>
+__global__ __launch_bounds__(UNC_THREADS) void uncompress_kernel(const float2* __restrict__ coh, long long npts, int n,
+                                                                 const int32_t* __restrict__ pairs, int npairs,
+                                                                 int band_rows, long long pts_per_block, int G,
+                                                                 float2* __restrict__ out) {
+  extern __shared__ unsigned unc_lut[];
+  const int row0 = blockIdx.y * band_rows;
+  const int rows = min(band_rows, n - row0);
+  const int ne = rows * n;
+  for (int e = threadIdx.x; e < ne; e += UNC_THREADS) unc_lut[e] = 0u;
+  __syncthreads();
+  for (int p = threadIdx.x; p < npairs; p += UNC_THREADS) {
     const int a = pairs[2 * p], b = pairs[2 * p + 1];
-    const float2 c = coh[t];
-    out[(pt * n + a) * n + b] = c;
-    out[(pt * n + b) * n + a] = make_float2(c.x, -c.y);
+    if ((unsigned)a >= (unsigned)n || (unsigned)b >= (unsigned)n) continue;  // the host rejects these
+    if ((unsigned)(a - row0) < (unsigned)rows) atomicMax(&unc_lut[(a - row0) * n + b], (unsigned)(2 + p));
+    if ((unsigned)(b - row0) < (unsigned)rows) atomicMax(&unc_lut[(b - row0) * n + a], (unsigned)(2 + npairs + p));
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < rows; i += UNC_THREADS) unc_lut[i * n + row0 + i] = 1u;
+  __syncthreads();
+  float2* stage = reinterpret_cast<float2*>(unc_lut + ((ne + 1) & ~1));
+  const long long p0 = (long long)blockIdx.x * pts_per_block;
+  const long long p1 = min(npts, p0 + pts_per_block);
+  const long long nn = (long long)n * n;
+  for (long long g0 = p0; g0 < p1; g0 += STAGE ? G : p1 - p0) {
+    const int gp = (int)min((long long)(STAGE ? G : p1 - p0), p1 - g0);
+    const float2* cb = coh + g0 * npairs;
+    if (STAGE) {
+      const int m = gp * npairs;
+      for (int t0 = threadIdx.x; t0 < m; t0 += 8 * UNC_THREADS) {
+        float2 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int t = t0 + u * UNC_THREADS;
+          v[u] = t < m ? cb[t] : make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int t = t0 + u * UNC_THREADS;
+          if (t < m) stage[t] = v[u];
+        }
+      }
+      __syncthreads();
+    }
+    // the host keeps pts_per_block * ne below 2^31
+    const unsigned tot = (unsigned)gp * (unsigned)ne;
+    float2* ob = out + g0 * nn + (long long)row0 * n;
+    constexpr int U = 4;
+    for (unsigned t0 = threadIdx.x; t0 < tot; t0 += U * UNC_THREADS) {
+      float2 v[U];
+      long long o[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const unsigned t = t0 + u * UNC_THREADS;
+        v[u] = make_float2(0.f, 0.f);
+        o[u] = -1;
+        if (t < tot) {
+          const unsigned q = t / (unsigned)ne;
+          const int e = (int)(t - q * (unsigned)ne);
+          o[u] = (long long)q * nn + e;
+          const unsigned code = unc_lut[e];
+          if (code == 1u) {
+            v[u].x = 1.f;
+          } else if (code >= 2u) {
+            const bool cj = code >= 2u + (unsigned)npairs;
+            const int pi = (int)q * npairs + (int)(code - 2u - (cj ? (unsigned)npairs : 0u));
+            const float2 c = STAGE ? stage[pi] : cb[pi];
+            v[u] = make_float2(c.x, cj ? -c.y : c.y);
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (o[u] >= 0) ob[o[u]] = v[u];
+    }
+    if (STAGE) __syncthreads();
   }
 }
 
@@ -271,18 +346,32 @@ int mr_emperical_co_pc(const float* rslc, int nl, int w, int n, const int32_t* i
 int mr_uncompress_coh(const float* coh, long long n_pts, int n, const int32_t* pairs, int n_pairs, float* out,
                       mr_stream_t stream) {
   MR_CHECK_ARG(n >= 1 && n_pts >= 0 && n_pairs >= 0, "uncompress_coh: bad sizes");
+  MR_CHECK_ARG(n <= mr::UNC_LUT_ENTRIES, "uncompress_coh: n = %d above %d", n, mr::UNC_LUT_ENTRIES);
+  MR_CHECK_ARG(n_pairs <= (1 << 30), "uncompress_coh: %d pairs above 2^30", n_pairs);
   if (n_pts == 0) return MR_OK;
   MR_CHECK_ARG(out != nullptr, "uncompress_coh: null output");
-  hipStream_t st = mr::as_stream(stream);
-  const int nb = (int)std::min<long long>(mr::div_up((unsigned long long)n_pts * n * n, 256), 1 << 16);
-  hipLaunchKernelGGL(mr::uncompress_kernel, dim3(nb), dim3(256), 0, st, reinterpret_cast<const float2*>(coh), n_pts, n,
-                     pairs, n_pairs, reinterpret_cast<float2*>(out));
-  if (int rc = mr::launch_status("uncompress_coh")) return rc;
-  if (n_pairs == 0) return MR_OK;
-  MR_CHECK_ARG(coh && pairs, "uncompress_coh: null input");
-  const int nb2 = (int)std::min<long long>(mr::div_up((unsigned long long)n_pts * n_pairs, 256), 1 << 16);
-  hipLaunchKernelGGL(mr::uncompress_scatter_kernel, dim3(nb2), dim3(256), 0, st, reinterpret_cast<const float2*>(coh),
-                     n_pts, n, pairs, n_pairs, reinterpret_cast<float2*>(out));
+  MR_CHECK_ARG(n_pairs == 0 || (coh && pairs), "uncompress_coh: null input");
+  const int band = std::min(n, mr::UNC_LUT_ENTRIES / n);
+  const int nbands = (int)mr::div_up(n, band);
+  // one band and rows that fit: stage G points' rows at a time in LDS
+  const bool stage = nbands == 1 && n_pairs >= 1 && n_pairs <= mr::UNC_STAGE;
+  const int G = stage ? std::max(1, mr::UNC_STAGE_GROUP / n_pairs) : 1;
+  // four blocks per CU, each over a run of whole points (more blocks build
+  // more tables for no gain: 12% slower at 4096 blocks, n = 60)
+  const long long nb = std::min<long long>(n_pts, std::max(1, 1024 / nbands));
+  const long long per = std::min<long long>(mr::div_up((unsigned long long)n_pts, (unsigned long long)nb),
+                                            ((1ll << 31) - 1 - 4 * mr::UNC_THREADS) / ((long long)band * n));
+  const long long blocks = (n_pts + per - 1) / per;
+  MR_CHECK_ARG(blocks < (1ll << 31), "uncompress_coh: %lld points too many", n_pts);
+  const size_t lut = ((size_t)band * n + 1) / 2 * 8;
+  if (stage)
+    hipLaunchKernelGGL(mr::uncompress_kernel<true>, dim3((unsigned)blocks, (unsigned)nbands), dim3(mr::UNC_THREADS),
+                       lut + (size_t)G * n_pairs * 8, mr::as_stream(stream), reinterpret_cast<const float2*>(coh),
+                       n_pts, n, pairs, n_pairs, band, per, G, reinterpret_cast<float2*>(out));
+  else
+    hipLaunchKernelGGL(mr::uncompress_kernel<false>, dim3((unsigned)blocks, (unsigned)nbands), dim3(mr::UNC_THREADS),
+                       lut, mr::as_stream(stream), reinterpret_cast<const float2*>(coh), n_pts, n, pairs, n_pairs,
+                       band, per, 1, reinterpret_cast<float2*>(out));
   return mr::launch_status("uncompress_coh");
 }
 

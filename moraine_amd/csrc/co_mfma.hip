@@ -80,19 +80,25 @@ __global__ __launch_bounds__(64 * COM_WAVES) void co_mfma_kernel(
       __builtin_amdgcn_wave_barrier();
       auto in_tile = [&](int lo, int hi) { return !TWO || ((lo >= 32) ? 2 : (hi >= 32 ? 1 : 0)) == t; };
       if (RASTER) {
+        // the tile's 32x32 block of the N x N output and, off the diagonal, its
+        // mirrored block (conjugated); a half-wave writes 32 consecutive entries
         const float fk = (float)K;
-        for (int e = lane; e < n * n; e += 64) {
-          const int m = e / n, j = e - (e / n) * n;
-          const int lo = min(m, j), hi = max(m, j);
-          if (!in_tile(lo, hi)) continue;
-          float2 v = tiles[(lo & 31) * 32 + (hi & 31)];
-          if (m > j) v.y = -v.y;
-          const float f = rs[m] * rs[j];
-          const size_t o = (size_t)pt * n * n + e;
-          coh[o] = make_float2(v.x * f, v.y * f);
-          if (cov) {
-            const float us = 1.f / (sc[m] * sc[j]);  // exact: powers of two
-            cov[o] = make_float2(v.x * us / fk, v.y * us / fk);
+        const int ta = t == 2 ? 1 : 0, tb = t == 0 ? 0 : 1;
+        for (int mir = 0; mir < (ta != tb ? 2 : 1); ++mir) {
+          const int r0 = 32 * (mir ? tb : ta), c0 = 32 * (mir ? ta : tb);
+          for (int it = 0; it < 16; ++it) {
+            const int m = r0 + 2 * it + (lane >> 5), j = c0 + (lane & 31);
+            if (m >= n || j >= n) continue;
+            const int lo = min(m, j), hi = max(m, j);
+            float2 v = tiles[(lo & 31) * 32 + (hi & 31)];
+            if (m > j) v.y = -v.y;
+            const float f = rs[m] * rs[j];
+            const size_t o = (size_t)pt * n * n + (size_t)m * n + j;
+            coh[o] = make_float2(v.x * f, v.y * f);
+            if (cov) {
+              const float us = 1.f / (sc[m] * sc[j]);  // exact: powers of two
+              cov[o] = make_float2(v.x * us / fk, v.y * us / fk);
+            }
           }
         }
       } else {

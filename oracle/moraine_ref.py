@@ -81,8 +81,11 @@ def uncompress_coh(coh: np.ndarray, image_pairs: np.ndarray | None = None) -> np
     else:
         n = int(image_pairs[-1, -1]) + 1
     out = np.zeros(coh.shape[:-1] + (n, n), dtype=coh.dtype)
+    # three passes in the reference's order (co.py:296-298): every pair, then
+    # every conjugate, then the diagonal; a later write to an entry wins
     for p, (a, b) in enumerate(image_pairs):
         out[..., a, b] = coh[..., p]
+    for p, (a, b) in enumerate(image_pairs):
         out[..., b, a] = np.conj(coh[..., p])
     for i in range(n):
         out[..., i, i] = 1

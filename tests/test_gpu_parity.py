@@ -188,6 +188,34 @@ def test_uncompress_gpu():
     np.testing.assert_array_equal(mr.uncompress_coh(c2, pairs), R.uncompress_coh(c2, pairs))
 
 
+@pytest.mark.parametrize("n", [2, 31, 60, 128, 129, 200, 230])
+def test_uncompress_sizes_gpu(n):
+    # one row band of the LDS code table up to n = 128, several above
+    mr = _mr()
+    P = n * (n - 1) // 2
+    rng = np.random.default_rng(n)
+    c = (rng.standard_normal((3, P)) + 1j * rng.standard_normal((3, P))).astype(np.complex64)
+    _case(f"uncompress_coh n={n}")
+    np.testing.assert_array_equal(mr.uncompress_coh(c), R.uncompress_coh(c))
+
+
+@pytest.mark.parametrize("n", [5, 210])
+def test_uncompress_colliding_pairs_gpu(n):
+    # reversed, repeated, diagonal and negative-index pairs: later writes win,
+    # conjugates after pairs, the unit diagonal last (moraine/co.py:296-298)
+    mr = _mr()
+    rng = np.random.default_rng(7)
+    base = np.array([[0, 1], [1, 0], [0, 0], [0, 1], [2, -1], [n - 1, 2], [3, 3], [1, n - 1]], np.int32)
+    extra = rng.integers(0, n, (40, 2)).astype(np.int32)
+    pairs = np.concatenate([base, extra, [[0, n - 1]]]).astype(np.int32)
+    c = (rng.standard_normal((4, pairs.shape[0])) + 1j * rng.standard_normal((4, pairs.shape[0]))).astype(np.complex64)
+    _case(f"uncompress_coh colliding pairs n={n}")
+    want = R.uncompress_coh(c, np.where(pairs < 0, pairs + n, pairs))
+    np.testing.assert_array_equal(mr.uncompress_coh(c, pairs), want)
+    with pytest.raises(IndexError):
+        mr.uncompress_coh(c, np.concatenate([pairs[:-1], [[n, 0]], pairs[-1:]]))
+
+
 # ---------------------------------------------------------------------------- EMI
 @pytest.mark.parametrize("n", [2, 3, 17, 30, 32, 33, 60, 64, 65, 90, 96, 97, 120, 128, 129, 150, 160, 161, 200, 201])
 def test_emi_closure_free(n):

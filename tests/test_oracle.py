@@ -218,6 +218,19 @@ def test_pairs_and_uncompress_printed():
     np.testing.assert_array_equal(u2, u2.conj().T)
 
 
+def test_uncompress_colliding_pairs_hand_derived():
+    # pairs (0,1), (1,0), (0,0), (0,1) with coh c0..c3, n = 2 from the last
+    # pair; the reference's statements (moraine/co.py:296-298) give, in order:
+    #   pass 1: [0,1]=c0, [1,0]=c1, [0,0]=c2, [0,1]=c3
+    #   pass 2: [1,0]=c0*, [0,1]=c1*, [0,0]=c2*, [1,0]=c3*
+    #   diag  : [0,0]=[1,1]=1
+    c = np.array([1 + 2j, 3 + 4j, 5 + 6j, 7 + 8j], np.complex64)
+    pairs = np.array([[0, 1], [1, 0], [0, 0], [0, 1]], np.int32)
+    u = R.uncompress_coh(c, pairs)
+    want = np.array([[1, np.conj(c[1])], [np.conj(c[3]), 1]], np.complex64)
+    np.testing.assert_array_equal(u, want)
+
+
 # ---------------------------------------------------------------- EMI
 @pytest.mark.parametrize("n", [17, 30, 60])
 def test_emi64_closure_free_closed_form(n):
