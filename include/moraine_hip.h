@@ -107,7 +107,10 @@ int mr_emi(const float* coh, long long n_pts, int n, int ref, float* ph, float* 
            void* workspace, size_t workspace_bytes, mr_stream_t stream);
 
 /* uncompress_coh (moraine/co.py:281-299): (n_pts, n_pairs) -> (n_pts, n, n),
- * [a,b] = c, [b,a] = conj(c), diagonal 1, other entries 0. */
+ * [a,b] = c, [b,a] = conj(c), diagonal 1, other entries 0, resolved in the
+ * reference's order when entries collide: all pairs, then all conjugates, a
+ * later pair over an earlier one, the diagonal last.  Pair indices must lie in
+ * [0, n) (others are skipped); n <= 16384, n_pairs <= 2^30. */
 int mr_uncompress_coh(const float* coh, long long n_pts, int n, const int32_t* pairs, int n_pairs,
                       float* out, mr_stream_t stream);
 
