@@ -122,12 +122,16 @@ constexpr int EMI_FAST_THREADS = 64;
 
 // TC: also the DS temporal coherence of each point from the staged row
 // (moraine/pl.py:126-145), as mr_ds_temp_coh computes it from (coh, ph, pairs).
-template <int NM, bool TC>
+// CHK: points whose conditioning test fails (tau) are appended to
+// flag_list[*flag_count] for the FP64 re-solve.
+template <int NM, bool TC, bool CHK = false>
 __global__ __launch_bounds__(EMI_FAST_THREADS) void emi_fast_kernel(const float2* __restrict__ coh, long long npts,
                                                                     int n, int ref, float2* __restrict__ ph,
                                                                     float* __restrict__ quality,
                                                                     const int32_t* __restrict__ pairs,
-                                                                    float* __restrict__ t_coh) {
+                                                                    float* __restrict__ t_coh, double tau = 0.0,
+                                                                    unsigned* __restrict__ flag_count = nullptr,
+                                                                    int* __restrict__ flag_list = nullptr) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int P = n * (n - 1) / 2;
   const int lane0 = threadIdx.x;
@@ -159,11 +163,13 @@ __global__ __launch_bounds__(EMI_FAST_THREADS) void emi_fast_kernel(const float2
     __builtin_amdgcn_wave_barrier();
     float vr, vi;
     double rq;
-    emi_fast_point<NM>(CohPacked{cs, n}, n, lane, work, vr, vi, rq);
+    bool flag = false;
+    emi_fast_point<NM, CHK>(CohPacked{cs, n}, n, lane, work, vr, vi, rq, tau, &flag);
     // ---- output: ph = v conj(v_ref) / |.|, quality --------------------------
     const float2 pv = emi_link_phase(vr, vi, ref);
     if (lane < n) ph[pt * n + lane] = pv;
     if (lane == 0) quality[pt] = (float)rq;
+    if (CHK && flag && lane == 0) flag_list[atomicAdd(flag_count, 1u)] = (int)pt;
     if (TC) {
       float2* phb = reinterpret_cast<float2*>(work);  // the EMI scratch is free again
       __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
@@ -201,17 +207,26 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   return s;
 }
 
+// LIST: the FP64 re-solve of flagged points: the points are list[0 .. *count),
+// the block's slice lives in LDS (n <= 64), and with t_coh set each re-solved
+// point's DS temporal coherence is recomputed from its new phases.
+template <bool LIST>
 __global__ __launch_bounds__(EMI_GEN_THREADS) void emi_general_kernel(const float2* __restrict__ coh,
                                                                       long long npts, int n, int ref,
                                                                       float2* __restrict__ ph,
                                                                       float* __restrict__ quality,
-                                                                      unsigned char* __restrict__ ws) {
+                                                                      unsigned char* __restrict__ ws,
+                                                                      const int* __restrict__ list = nullptr,
+                                                                      const unsigned* __restrict__ count = nullptr,
+                                                                      const int32_t* __restrict__ pairs = nullptr,
+                                                                      float* __restrict__ t_coh = nullptr) {
   __shared__ double red[EMI_GEN_THREADS / 64];
   __shared__ double sh_key[EMI_GEN_THREADS];
   __shared__ int sh_idx[EMI_GEN_THREADS];
+  extern __shared__ __align__(16) unsigned char gen_lds[];
   const int P = n * (n - 1) / 2;
   const int tid = threadIdx.x;
-  unsigned char* base = ws + (size_t)blockIdx.x * emi_gen_slice_bytes(n);
+  unsigned char* base = LIST ? gen_lds : ws + (size_t)blockIdx.x * emi_gen_slice_bytes(n);
   double* A = reinterpret_cast<double*>(base);
   c128* M = reinterpret_cast<c128*>(A + (size_t)n * n);
   c128* col = M + (size_t)n * n;  // n
@@ -228,7 +243,9 @@ __global__ __launch_bounds__(EMI_GEN_THREADS) void emi_general_kernel(const floa
   double* wk = dphi + n;  // 5n
   int* swept = reinterpret_cast<int*>(wk + 5 * n);
 
-  for (long long pt = blockIdx.x; pt < npts; pt += gridDim.x) {
+  const long long nit = LIST ? (long long)*count : npts;
+  for (long long it = blockIdx.x; it < nit; it += gridDim.x) {
+    const long long pt = LIST ? (long long)list[it] : it;
     const float2* src = coh + pt * (long long)P;
     for (int t = tid; t < n * n; t += EMI_GEN_THREADS) {
       const int i = t / n, j = t - (t / n) * n;
@@ -437,13 +454,24 @@ __global__ __launch_bounds__(EMI_GEN_THREADS) void emi_general_kernel(const floa
       const double pr = pv[i].re * vr.re + pv[i].im * vr.im;
       const double pim = pv[i].im * vr.re - pv[i].re * vr.im;
       const double am = sqrt(pr * pr + pim * pim);
-      ph[pt * n + i] = make_float2((float)(pr / am), (float)(pim / am));
+      const float2 v = make_float2((float)(pr / am), (float)(pim / am));
+      ph[pt * n + i] = v;
+      if (LIST) reinterpret_cast<float2*>(wk)[i] = v;  // LU work space is free again
+    }
+    __syncthreads();
+    if (LIST && t_coh != nullptr && tid < 64) {
+      const float2* phl = reinterpret_cast<const float2*>(wk);
+      const float tc = tcoh_point<true>([&](int p, int, int) { return src[p]; }, [&](int i) { return phl[i]; }, pairs,
+                                        P, tid);
+      if (tid == 0) t_coh[pt] = tc;
     }
     __syncthreads();
   }
 }
 
 int emi_wide_launch(const float2* coh, long long npts, int n, int ref, float2* ph, float* quality, hipStream_t st);
+
+__global__ void emi_zero_count_kernel(unsigned* c) { *c = 0u; }
 
 static size_t emi_fast_lds(int n, int nm) {
   const int P = n * (n - 1) / 2;
@@ -482,7 +510,7 @@ int mr_emi(const float* coh, long long n_pts, int n, int ref, float* ph, float* 
   MR_CHECK_ARG(workspace != nullptr && workspace_bytes >= need, "emi: workspace %zu < %zu bytes", workspace_bytes,
                need);
   const int nb = (int)std::min<long long>(n_pts, mr::EMI_GEN_BLOCKS);
-  hipLaunchKernelGGL(mr::emi_general_kernel, dim3(nb), dim3(mr::EMI_GEN_THREADS), 0, st, c, n_pts, n, ref, p,
+  hipLaunchKernelGGL(mr::emi_general_kernel<false>, dim3(nb), dim3(mr::EMI_GEN_THREADS), 0, st, c, n_pts, n, ref, p,
                      quality, reinterpret_cast<unsigned char*>(workspace));
   return mr::launch_status("emi_general");
 }
@@ -509,6 +537,48 @@ int mr_emi_ds_temp_coh(const float* coh, long long n_pts, int n, int ref, const 
   const int rc = mr_emi(coh, n_pts, n, ref, ph, quality, workspace, workspace_bytes, stream);
   if (rc != MR_OK) return rc;
   return mr_ds_temp_coh(coh, ph, n_pts, n, pairs, n_pairs, t_coh, stream);
+}
+
+size_t mr_emi_refined_workspace_bytes(long long n_pts, int n) {
+  (void)n;
+  return 256 + (((size_t)(n_pts > 0 ? n_pts : 0) * 4 + 255) & ~(size_t)255);
+}
+
+int mr_emi_refined(const float* coh, long long n_pts, int n, int ref, double tol, const int32_t* pairs, int n_pairs,
+                   float* ph, float* quality, float* t_coh, void* workspace, size_t workspace_bytes,
+                   mr_stream_t stream) {
+  MR_CHECK_ARG(n > 32 && n <= 64, "emi_refined: covers 32 < n <= 64 (n=%d)", n);
+  MR_CHECK_ARG(ref >= 0 && ref < n, "emi_refined: ref=%d outside [0,%d)", ref, n);
+  MR_CHECK_ARG(n_pts >= 0 && n_pts < (1ll << 31), "emi_refined: point count %lld outside [0, 2^31)", n_pts);
+  MR_CHECK_ARG(tol > 0.0, "emi_refined: tol must be positive");
+  MR_CHECK_ARG(t_coh == nullptr || n_pairs == n * (n - 1) / 2,
+               "emi_refined: coherence rows hold %d pairs, n=%d needs %d", n_pairs, n, n * (n - 1) / 2);
+  if (n_pts == 0) return MR_OK;
+  MR_CHECK_ARG(coh && ph && quality && (t_coh == nullptr || pairs), "emi_refined: null pointer");
+  const size_t need = mr_emi_refined_workspace_bytes(n_pts, n);
+  MR_CHECK_ARG(workspace != nullptr && workspace_bytes >= need, "emi_refined: workspace %zu < %zu bytes",
+               workspace_bytes, need);
+  hipStream_t st = mr::as_stream(stream);
+  unsigned* count = reinterpret_cast<unsigned*>(workspace);
+  int* list = reinterpret_cast<int*>(reinterpret_cast<unsigned char*>(workspace) + 256);
+  hipLaunchKernelGGL(mr::emi_zero_count_kernel, dim3(1), dim3(1), 0, st, count);
+  if (int rc = mr::launch_status("emi_refined")) return rc;
+  const float2* c = reinterpret_cast<const float2*>(coh);
+  float2* p = reinterpret_cast<float2*>(ph);
+  const size_t lds = mr::emi_fast_lds(n, 64);
+  const int nb = (int)std::min<long long>(n_pts, 1 << 20);
+  if (t_coh)
+    hipLaunchKernelGGL((mr::emi_fast_kernel<64, true, true>), dim3(nb), dim3(mr::EMI_FAST_THREADS), lds, st, c, n_pts,
+                       n, ref, p, quality, pairs, t_coh, tol, count, list);
+  else
+    hipLaunchKernelGGL((mr::emi_fast_kernel<64, false, true>), dim3(nb), dim3(mr::EMI_FAST_THREADS), lds, st, c,
+                       n_pts, n, ref, p, quality, nullptr, nullptr, tol, count, list);
+  if (int rc = mr::launch_status("emi_refined")) return rc;
+  // FP64 re-solve of the flagged points, one block per point with its slice in LDS
+  const int nb2 = (int)std::min<long long>(n_pts, mr::EMI_GEN_BLOCKS);
+  hipLaunchKernelGGL(mr::emi_general_kernel<true>, dim3(nb2), dim3(mr::EMI_GEN_THREADS), mr::emi_gen_slice_bytes(n),
+                     st, c, n_pts, n, ref, p, quality, nullptr, list, count, pairs, t_coh);
+  return mr::launch_status("emi_refined");
 }
 
 }  // extern "C"

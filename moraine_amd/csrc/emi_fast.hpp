@@ -12,9 +12,11 @@ namespace mr {
 // EMI_FAST_LDS_BYTES of 16-byte aligned LDS owned by this wave.
 constexpr int EMI_FAST_LDS_BYTES = 1536;
 
-template <int NM, class Src>
+// CHECK: also decide whether the point needs the FP64 re-solve (*flag, from
+// tau; see the end of the function).
+template <int NM, bool CHECK = false, class Src>
 __device__ __forceinline__ void emi_fast_point(const Src& coh_at, int n, int lane, unsigned char* lds, float& vr,
-                                               float& vi, double& rq) {
+                                               float& vi, double& rq, double tau = 0.0, bool* flag = nullptr) {
   double* colb = reinterpret_cast<double*>(lds);        // 64: sweep column broadcast
   float2* ub = reinterpret_cast<float2*>(lds + 512);    // 64: reflector u
   float2* qb = ub + 64;                                 // 64: q
@@ -433,6 +435,29 @@ __device__ __forceinline__ void emi_fast_point(const Src& coh_at, int n, int lan
     const float si = wsum_f(ar * vi - ai * vr);
     vr -= 2.f * (ar * sr - ai * si);
     vi -= 2.f * (ar * si + ai * sr);
+  }
+  if constexpr (CHECK) {
+    // The FP32 stages perturb M by about eps32 ||T||, which moves the unit
+    // eigenvector by eps32 ||T|| / gap and the phase of entry i by that over
+    // |v_i|.  Flag the point when eps32 ||T|| / (gap min_i |v_i| sqrt(n)) > tau,
+    // i.e. when T (still in LDS) has a second eigenvalue below lambda + delta
+    // with delta = eps32 ||T|| / (tau min_i |v_i| sqrt(n)): one Sturm count.
+    const double av = lane < n ? (double)sqrtf(vr * vr + vi * vi) : 1e300;
+    const double vmin = fmax(wmin_d(av), 1e-300);
+    const double tn = wmax_d(lane < n ? fabs(my_d) : 0.0) + 2.0 * wmax_d(lane < n ? fabs(my_e) : 0.0);
+    const double x = rq + 5.9604644775390625e-8 * tn / (tau * vmin * sqrt((double)n));
+    const double pivmin = 1e-300 + tn * tn * 1e-290;
+    double q = tde[0].x - x;
+    int cnt = q < 0.0;
+    for (int i = 1; i < n; ++i) {
+      if (fabs(q) < pivmin) q = -pivmin;
+      double r = __builtin_amdgcn_rcp(q);
+      r = fma(r, fma(-q, r, 1.0), r);
+      const double2 t = tde[i];
+      q = (t.x - x) - t.y * r;
+      cnt += q < 0.0;
+    }
+    *flag = cnt >= 2 && isfinite(x);
   }
 }
 

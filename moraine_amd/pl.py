@@ -14,10 +14,43 @@ from .co import _pairs, emperical_co_pc
 from .tnet import nimage_from_npair
 
 
-def _emi_dev(c: torch.Tensor, n: int, ref: int):
+#: after a call with ``refine`` set: a 0-d int32 device tensor, the number of
+#: points that were solved again in FP64 (reading it synchronises the stream)
+last_refined = None
+
+
+def _check_refine(refine, n: int):
+    if refine is None:
+        return None
+    tol = float(refine)
+    if not tol > 0.0:
+        raise ValueError(f"refine must be a positive phase tolerance in radians, got {refine}")
+    if not 32 < n <= 64:
+        raise ValueError(f"refine covers 32 < N <= 64 images, got N={n}")
+    return tol
+
+
+def _emi_refined_dev(c: torch.Tensor, n: int, ref: int, tol: float, ph: torch.Tensor, q: torch.Tensor, t=None):
+    """mr_emi_refined: the one-wave solve, then an FP64 re-solve of every point
+    whose estimated phase error exceeds tol (with t: the fused temporal coherence)."""
+    global last_refined
+    npts = c.shape[0]
+    pairs = _full_pairs_dev(n, c.device) if t is not None else None
+    wsb = _lib.query("mr_emi_refined_workspace_bytes", npts, n)
+    ws = _dev.workspace(wsb, c.device)
+    _lib.call("mr_emi_refined", _dev.ptr(c), npts, n, int(ref), tol, _dev.ptr(pairs) if pairs is not None else None,
+              pairs.shape[0] if pairs is not None else 0, _dev.ptr(ph), _dev.ptr(q),
+              _dev.ptr(t) if t is not None else None, _dev.ptr(ws), ws.numel(), _dev.stream())
+    last_refined = ws[:4].view(torch.int32)[0]
+
+
+def _emi_dev(c: torch.Tensor, n: int, ref: int, tol=None):
     npts = c.shape[0]
     ph = torch.empty((npts, n), dtype=torch.complex64, device=c.device)
     q = torch.empty(npts, dtype=torch.float32, device=c.device)
+    if tol is not None:
+        _emi_refined_dev(c, n, ref, tol, ph, q)
+        return ph, q
     wsb = _lib.query("mr_emi_workspace_bytes", npts, n)
     ws = _dev.workspace(wsb, c.device)
     _lib.call("mr_emi", _dev.ptr(c), npts, n, int(ref), _dev.ptr(ph), _dev.ptr(q), _dev.ptr(ws), ws.numel(),
@@ -25,14 +58,19 @@ def _emi_dev(c: torch.Tensor, n: int, ref: int):
     return ph, q
 
 
-def emi(coh, ref: int = 0):
+def emi(coh, ref: int = 0, *, refine=None):
     """EMI phase history (n_pts, N) complex64 and quality (n_pts,) float32 from
-    compressed coherence (n_pts, N(N-1)/2) in triu order."""
+    compressed coherence (n_pts, N(N-1)/2) in triu order.
+
+    refine (not in moraine; 32 < N <= 64): a phase tolerance in radians, e.g.
+    1e-4.  Points whose estimated phase error from the FP32 stages exceeds it
+    are solved again in FP64 throughout; the others are unchanged."""
     as_torch = _dev.is_torch(coh)
     c = _dev.to_dev(coh, torch.complex64)
     n = nimage_from_npair(c.shape[-1])
+    tol = _check_refine(refine, n)
     lead = tuple(c.shape[:-1])
-    ph, q = _emi_dev(c.reshape(-1, c.shape[-1]), n, ref)
+    ph, q = _emi_dev(c.reshape(-1, c.shape[-1]), n, ref, tol)
     return _dev.back(ph.reshape(lead + (n,)), as_torch), _dev.back(q.reshape(lead), as_torch)
 
 
@@ -71,9 +109,12 @@ def _full_pairs_dev(n: int, device) -> torch.Tensor:
     return _FULL_PAIRS[key]
 
 
-def _emi_tcoh_dev(c: torch.Tensor, n: int, ref: int, ph: torch.Tensor, q: torch.Tensor, t: torch.Tensor):
+def _emi_tcoh_dev(c: torch.Tensor, n: int, ref: int, ph: torch.Tensor, q: torch.Tensor, t: torch.Tensor, tol=None):
     """emi then ds_temp_coh on the same full-network rows, into ph, q, t (one
     kernel for 32 < n <= 64; results equal the two calls bit for bit)."""
+    if tol is not None:
+        _emi_refined_dev(c, n, ref, tol, ph, q, t)
+        return
     npts = c.shape[0]
     pairs = _full_pairs_dev(n, c.device)
     wsb = _lib.query("mr_emi_workspace_bytes", npts, n)
@@ -82,34 +123,37 @@ def _emi_tcoh_dev(c: torch.Tensor, n: int, ref: int, ph: torch.Tensor, q: torch.
               _dev.ptr(q), _dev.ptr(t), _dev.ptr(ws), ws.numel(), _dev.stream())
 
 
-def emi_ds_temp_coh(coh, ref: int = 0):
-    """(ph, quality, t_coh) = emi(coh, ref) followed by ds_temp_coh(coh, ph):
+def emi_ds_temp_coh(coh, ref: int = 0, *, refine=None):
+    """(ph, quality, t_coh) = emi(coh, ref, refine) followed by ds_temp_coh(coh, ph):
     the inner stages of moraine.emperical_co_emi_temp_coh_pc (moraine/pl.py:165-167)
     for full-network coherence rows (n_pts, N(N-1)/2) in triu order."""
     as_torch = _dev.is_torch(coh)
     c = _dev.to_dev(coh, torch.complex64).reshape(-1, coh.shape[-1])
     n = nimage_from_npair(c.shape[-1])
+    tol = _check_refine(refine, n)
     ph = torch.empty((c.shape[0], n), dtype=torch.complex64, device=c.device)
     q = torch.empty(c.shape[0], dtype=torch.float32, device=c.device)
     t = torch.empty(c.shape[0], dtype=torch.float32, device=c.device)
-    _emi_tcoh_dev(c, n, ref, ph, q, t)
+    _emi_tcoh_dev(c, n, ref, ph, q, t, tol)
     lead = tuple(coh.shape[:-1])
     return (_dev.back(ph.reshape(lead + (n,)), as_torch), _dev.back(q.reshape(lead), as_torch),
             _dev.back(t.reshape(lead), as_torch))
 
 
-def emperical_co_emi_temp_coh_pc(rslc, idx, pc_is_shp, batch_size: int = 1000):
+def emperical_co_emi_temp_coh_pc(rslc, idx, pc_is_shp, batch_size: int = 1000, *, refine=None):
     """Coherence -> EMI -> temporal coherence for a point cloud, in batches.
 
     Every point is independent and every kernel deterministic, so the result
     does not depend on the batch size; batches below 65536 points are merged
     (they only bound memory: 8*N(N-1)/2 bytes of coherence per point).
+    refine: as for emi (not in moraine).
     """
     as_torch = _dev.is_torch(rslc)
     z = _dev.to_dev(rslc, torch.complex64)
     ix = _dev.to_dev(idx, torch.int32)
     m = _dev.to_dev(pc_is_shp, torch.bool)
     n = z.shape[2]
+    tol = _check_refine(refine, n)
     n_pc = ix.shape[0]
     ph = torch.empty((n_pc, n), dtype=torch.complex64, device=z.device)
     q = torch.empty(n_pc, dtype=torch.float32, device=z.device)
@@ -118,5 +162,5 @@ def emperical_co_emi_temp_coh_pc(rslc, idx, pc_is_shp, batch_size: int = 1000):
     for s in range(0, n_pc, bs):
         e = min(n_pc, s + bs)
         c = emperical_co_pc(z, ix[s:e], m[s:e])
-        _emi_tcoh_dev(c, n, 0, ph[s:e], q[s:e], t[s:e])
+        _emi_tcoh_dev(c, n, 0, ph[s:e], q[s:e], t[s:e], tol)
     return _dev.back(ph, as_torch), _dev.back(q, as_torch), _dev.back(t, as_torch)

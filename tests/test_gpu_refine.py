@@ -1,0 +1,102 @@
+"""GPU: the FP64 re-solve of poorly conditioned points (emi(..., refine=tol)).
+
+Against the FP64 oracle (oracle.moraine_ref.emi64) on a synthetic DS stack
+whose SHP sets include fewer samples than images (K < N: |Gamma| close to
+singular, where the FP32 stages lose the most).  Unflagged points must equal
+the plain call bit for bit; every point must be within the stated phase
+tolerance of FP64 after refinement.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from moraine_amd import synth
+from oracle import moraine_ref as R
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(msg):
+    print(f"[gpu case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def _rms_phase(a, b):
+    e = np.angle(a.astype(np.complex128) * np.conj(b.astype(np.complex128)))
+    return np.sqrt(np.mean(e ** 2, axis=-1))
+
+
+@pytest.fixture(scope="module")
+def coh60():
+    import torch
+    import moraine_amd as mr
+
+    S, n = 160, 60
+    z = synth.ds_stack_torch(S, S, n, seed=0, ps_frac=0.1, device="cuda")
+    x = torch.view_as_real(z)
+    rmli = (x[..., 0] ** 2 + x[..., 1] ** 2).contiguous()
+    _, shp, cnt = mr.ks_test_select_shp(rmli, 7, 10, 0.05, return_p=False)
+    ii, jj = torch.meshgrid(torch.arange(S, device="cuda", dtype=torch.int32),
+                            torch.arange(S, device="cuda", dtype=torch.int32), indexing="ij")
+    idx = torch.stack([ii.reshape(-1), jj.reshape(-1)], -1).contiguous()
+    coh = mr.emperical_co_pc(z, idx, shp.view(-1, 15, 21)).contiguous()
+    assert int((cnt.view(-1) < n).sum()) > 1000  # the case the refinement is for
+    return coh
+
+
+def test_refine_flags_fixes_and_leaves_the_rest(coh60):
+    import moraine_amd as mr
+
+    tol = 1e-4
+    _case("emi refine=1e-4, n=60, 25600 points")
+    ph0, q0 = (a.cpu().numpy() for a in mr.emi(coh60))
+    ph1, q1 = mr.emi(coh60, refine=tol)
+    ph1, q1 = ph1.cpu().numpy(), q1.cpu().numpy()
+    n_ref = int(mr.pl.last_refined)
+    changed = np.any(ph0 != ph1, axis=1) | (q0 != q1)
+    # flagged points are re-solved; everything else is the plain call's output, bit for bit
+    assert 0 < n_ref < 0.05 * ph0.shape[0]
+    assert changed.sum() <= n_ref
+    same = ~changed
+    np.testing.assert_array_equal(ph0[same].view(np.uint32), ph1[same].view(np.uint32))
+    # FP64 oracle on the re-solved points and on the plain call's worst points
+    c = coh60.cpu().numpy()
+    fix = np.flatnonzero(changed)
+    p64_fix, q64_fix = R.emi64(c[fix])
+    err_fix = _rms_phase(ph1[fix], p64_fix)
+    assert err_fix.max() < 1e-5, err_fix.max()  # FP64 throughout, phases rounded to complex64
+    np.testing.assert_allclose(q1[fix], q64_fix, rtol=1e-6, atol=1e-6 * np.abs(q64_fix).max())
+    sample = np.unique(np.concatenate([fix, np.arange(0, ph0.shape[0], 37)]))
+    p64, _ = R.emi64(c[sample])
+    e0 = _rms_phase(ph0[sample], p64)
+    e1 = _rms_phase(ph1[sample], p64)
+    # tolerance: per-point RMS phase error vs FP64 stays near tol after refinement
+    assert e1.max() <= 2 * tol, e1.max()
+    assert e1.max() <= e0.max()
+
+
+def test_refine_fused_temporal_coherence(coh60):
+    import moraine_amd as mr
+
+    _case("emi_ds_temp_coh refine=1e-4, n=60")
+    ph0, q0, t0 = (a.cpu().numpy() for a in mr.pl.emi_ds_temp_coh(coh60))
+    ph1, q1, t1 = mr.pl.emi_ds_temp_coh(coh60, refine=1e-4)
+    # the fused temporal coherence equals ds_temp_coh on the refined phases, bit for bit
+    t_sep = mr.ds_temp_coh(coh60, ph1).cpu().numpy()
+    ph1, q1, t1 = ph1.cpu().numpy(), q1.cpu().numpy(), t1.cpu().numpy()
+    np.testing.assert_array_equal(t1.view(np.uint32), t_sep.view(np.uint32))
+    ph2, q2 = mr.emi(coh60, refine=1e-4)
+    np.testing.assert_array_equal(ph1.view(np.uint32), ph2.cpu().numpy().view(np.uint32))
+    same = ~(np.any(ph0 != ph1, axis=1) | (q0 != q1))
+    np.testing.assert_array_equal(t0[same].view(np.uint32), t1[same].view(np.uint32))
+
+
+def test_refine_argument_checks(coh60):
+    import moraine_amd as mr
+
+    with pytest.raises(ValueError):
+        mr.emi(coh60, refine=0.0)
+    c30 = coh60[:8, : 30 * 29 // 2].contiguous()
+    with pytest.raises(ValueError):
+        mr.emi(c30, refine=1e-4)

@@ -98,4 +98,9 @@ def test_api_signatures_match_reference_names():
         "uncompress_coh": ["coh", "image_pairs"],
     }
     for name, params in sig.items():
-        assert list(inspect.signature(getattr(mr, name)).parameters) == params, name
+        ps = inspect.signature(getattr(mr, name)).parameters
+        # the reference's parameters in order; extras only keyword-only with a default
+        assert [k for k, v in ps.items() if v.kind != v.KEYWORD_ONLY] == params, name
+        for k, v in ps.items():
+            if v.kind == v.KEYWORD_ONLY:
+                assert v.default is not inspect.Parameter.empty, (name, k)
