@@ -425,7 +425,27 @@ def main():
     err = torch.angle(ph[s0:s0 + m].to(torch.complex128) * ph64.conj())
     rms = torch.sqrt(torch.nanmean(err ** 2, dim=1))
     phase = {"rms_median_rad": float(torch.nanmedian(rms)), "rms_p99_rad": float(torch.nanquantile(rms, 0.99)),
-             "rms_max_rad": float(rms[~torch.isnan(rms)].max()), "points": m}
+             "rms_max_rad": float(rms[~torch.isnan(rms)].max()),
+             "rms_all_rad": float(torch.sqrt(torch.nanmean(rms ** 2))), "points": m}
+    # precision form on the same batch: FP64 re-solve of points whose estimated
+    # phase error exceeds tol (emi(..., refine=tol)); error on the same sample
+    if 32 < args.n <= 64:
+        from moraine_amd.pl import emi_ds_temp_coh
+        tol = 1e-3
+        emi_ds_temp_coh(coh, refine=tol)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        ph_r, _, _ = emi_ds_temp_coh(coh, refine=tol)
+        e1.record()
+        torch.cuda.synchronize()
+        k = int(mr.pl.last_refined)
+        err_r = torch.angle(ph_r[:m].to(torch.complex128) * ph64.conj())
+        rms_r = torch.sqrt(torch.nanmean(err_r ** 2, dim=1))
+        phase["refined"] = {"tol_rad": tol, "ms_per_batch": e0.elapsed_time(e1), "points_per_batch": coh.shape[0],
+                            "re_solved": k, "rms_max_rad": float(rms_r[~torch.isnan(rms_r)].max()),
+                            "rms_all_rad": float(torch.sqrt(torch.nanmean(rms_r ** 2))),
+                            "note": "EMI + temporal coherence of the last batch with refine=tol, HIP events"}
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(args)
     forms = {}

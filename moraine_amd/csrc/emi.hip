@@ -223,6 +223,8 @@ __global__ __launch_bounds__(EMI_GEN_THREADS) void emi_general_kernel(const floa
   __shared__ double red[EMI_GEN_THREADS / 64];
   __shared__ double sh_key[EMI_GEN_THREADS];
   __shared__ int sh_idx[EMI_GEN_THREADS];
+  __shared__ double2 tri_de[LIST ? 64 : 1];  // LIST: T for the one-wave eigen-solve
+  __shared__ double tri_e[LIST ? 64 : 1];
   extern __shared__ __align__(16) unsigned char gen_lds[];
   const int P = n * (n - 1) / 2;
   const int tid = threadIdx.x;
@@ -269,26 +271,36 @@ __global__ __launch_bounds__(EMI_GEN_THREADS) void emi_general_kernel(const floa
     __syncthreads();
     // sweep inverse
     for (int s = 0; s < n; ++s) {
-      double key = -1.0;
-      int idx = 0x7fffffff;
-      for (int i = tid; i < n; i += EMI_GEN_THREADS)
-        if (!swept[i] && (fabs(dg[i]) > key || (fabs(dg[i]) == key && i < idx))) {
-          key = fabs(dg[i]);
-          idx = i;
-        }
-      sh_key[tid] = key;
-      sh_idx[tid] = idx;
-      __syncthreads();
-      for (int st = EMI_GEN_THREADS / 2; st > 0; st >>= 1) {
-        if (tid < st) {
-          const double ok = sh_key[tid + st];
-          const int oi = sh_idx[tid + st];
-          if (ok > sh_key[tid] || (ok == sh_key[tid] && oi < sh_idx[tid])) {
-            sh_key[tid] = ok;
-            sh_idx[tid] = oi;
-          }
+      if (LIST) {
+        // n <= 64: the pivot (largest |d| not yet swept, ties to the smallest
+        // row) from one wave
+        if (tid < 64) {
+          const int k1 = wargmax_d(tid < n && !swept[tid] ? fabs(dg[tid]) : -1.0, tid);
+          if (tid == 0) sh_idx[0] = k1;
         }
         __syncthreads();
+      } else {
+        double key = -1.0;
+        int idx = 0x7fffffff;
+        for (int i = tid; i < n; i += EMI_GEN_THREADS)
+          if (!swept[i] && (fabs(dg[i]) > key || (fabs(dg[i]) == key && i < idx))) {
+            key = fabs(dg[i]);
+            idx = i;
+          }
+        sh_key[tid] = key;
+        sh_idx[tid] = idx;
+        __syncthreads();
+        for (int st = EMI_GEN_THREADS / 2; st > 0; st >>= 1) {
+          if (tid < st) {
+            const double ok = sh_key[tid + st];
+            const int oi = sh_idx[tid + st];
+            if (ok > sh_key[tid] || (ok == sh_key[tid] && oi < sh_idx[tid])) {
+              sh_key[tid] = ok;
+              sh_idx[tid] = oi;
+            }
+          }
+          __syncthreads();
+        }
       }
       const int k = sh_idx[0];
       for (int i = tid; i < n; i += EMI_GEN_THREADS) col[i].re = A[(size_t)i * n + k];
@@ -398,38 +410,52 @@ __global__ __launch_bounds__(EMI_GEN_THREADS) void emi_general_kernel(const floa
       const double sr = ae > 0 ? e.re / ae : 1.0, si = ae > 0 ? e.im / ae : 0.0;
       dphr[n - 1] = dcr * sr - dci * si;
       dphi[n - 1] = dcr * si + dci * sr;
-      // bisection (serial, FP64) + inverse iteration
-      double lo = 1e308, hi = -1e308, sc = 0.0;
-      for (int i = 0; i < n; ++i) {
-        const double off = (i > 0 ? tae[i - 1] : 0.0) + tae[i];
-        lo = fmin(lo, td[i] - off);
-        hi = fmax(hi, td[i] + off);
-        sc = fmax(sc, fabs(td[i]) + off);
+      if (!LIST) {
+        // bisection (serial, FP64) + inverse iteration
+        double lo = 1e308, hi = -1e308, sc = 0.0;
+        for (int i = 0; i < n; ++i) {
+          const double off = (i > 0 ? tae[i - 1] : 0.0) + tae[i];
+          lo = fmin(lo, td[i] - off);
+          hi = fmax(hi, td[i] + off);
+          sc = fmax(sc, fabs(td[i]) + off);
+        }
+        const double scale = sc > 0 ? sc : 1.0;
+        lo -= 1e-14 * scale;
+        hi += 1e-14 * scale;
+        const double pivmin = 1e-300 + scale * scale * 1e-290;
+        for (int it = 0; it < 200 && hi - lo > 2e-15 * scale; ++it) {
+          const double mid = 0.5 * (lo + hi);
+          if (mid <= lo || mid >= hi) break;
+          if (sturm_count(td, te2, n, mid, pivmin) >= 1)
+            hi = mid;
+          else
+            lo = mid;
+        }
+        const double lam = 0.5 * (lo + hi);
+        tridiag_min_eigpair(td, tae, te2, ty, wk, n, lam, scale);
+        double rq = 0.0;
+        for (int i = 0; i < n; ++i) {
+          double t = td[i] * ty[i];
+          if (i > 0) t += tae[i - 1] * ty[i - 1];
+          if (i + 1 < n) t += tae[i] * ty[i + 1];
+          rq += ty[i] * t;
+        }
+        quality[pt] = (float)rq;
       }
-      const double scale = sc > 0 ? sc : 1.0;
-      lo -= 1e-14 * scale;
-      hi += 1e-14 * scale;
-      const double pivmin = 1e-300 + scale * scale * 1e-290;
-      for (int it = 0; it < 200 && hi - lo > 2e-15 * scale; ++it) {
-        const double mid = 0.5 * (lo + hi);
-        if (mid <= lo || mid >= hi) break;
-        if (sturm_count(td, te2, n, mid, pivmin) >= 1)
-          hi = mid;
-        else
-          lo = mid;
-      }
-      const double lam = 0.5 * (lo + hi);
-      tridiag_min_eigpair(td, tae, te2, ty, wk, n, lam, scale);
-      double rq = 0.0;
-      for (int i = 0; i < n; ++i) {
-        double t = td[i] * ty[i];
-        if (i > 0) t += tae[i - 1] * ty[i - 1];
-        if (i + 1 < n) t += tae[i] * ty[i + 1];
-        rq += ty[i] * t;
-      }
-      quality[pt] = (float)rq;
     }
     __syncthreads();
+    if (LIST) {
+      // n <= 64: one wave finds the smallest eigenpair of T by multisection
+      // (run to FP64 resolution) and a twisted factorisation
+      if (tid < 64) {
+        double rq;
+        const double z = emi_tri_smallest(n, tid, tid < n ? td[tid] : 0.0, tid < n ? tae[tid] : 0.0, tri_de, tri_e,
+                                          rq, 12);
+        if (tid < n) ty[tid] = z;
+        if (tid == 0) quality[pt] = (float)rq;
+      }
+      __syncthreads();
+    }
     // back-transformation v = H_0 ... H_{n-3} D y
     for (int i = tid; i < n; i += EMI_GEN_THREADS) pv[i] = {dphr[i] * ty[i], dphi[i] * ty[i]};
     __syncthreads();

@@ -10,8 +10,10 @@ namespace mr {
 // hold zeros).  Returns z_i, lane i's entry of the unit eigenvector of the
 // smallest eigenvalue, and rq = z^T T z.  tde (64 x double2) and tee (64 x
 // double) are LDS the wave owns; T is staged there for the serial recurrences.
+// rounds: multisection rounds at most (each narrows the bracket 65-fold; they
+// stop once it is at FP64 resolution).
 __device__ __forceinline__ double emi_tri_smallest(int n, int lane, double my_d, double my_e, double2* tde,
-                                                   double* tee, double& rq) {
+                                                   double* tee, double& rq, int rounds = 5) {
   // ---- step 4a: smallest eigenvalue of T by multisection ------------------
   const double e_prev = dpp_d<DPP_WAVE_SHR1>(my_e);  // e_{i-1} (0 for lane 0)
   // T to LDS: the serial recurrences below read d_i, e_{i-1}^2 and e_i as one
@@ -34,7 +36,7 @@ __device__ __forceinline__ double emi_tri_smallest(int n, int lane, double my_d,
   // five rounds narrow the Gershgorin bracket by 65^5 ~ 1.2e9, finer than the
   // FP32 reduction resolves the eigenvalue; the reported quality is the
   // Rayleigh quotient of the eigenvector, accurate to second order
-  for (int round = 0; round < 5; ++round) {
+  for (int round = 0; round < rounds; ++round) {
     if (hi - lo <= 2e-15 * scale) break;
     const double x = lo + (hi - lo) * (double)(lane + 1) * (1.0 / 65.0);
     // Sturm count of eigenvalues below x
