@@ -397,7 +397,7 @@ def main():
     dom = max(["ks_test+select_shp", "emperical_co_pc", "emi+ds_temp_coh"], key=lambda k: stages[k]["s"])
     if dom == "emi+ds_temp_coh":
         ach = emi_flops * (npx / nb) / (t_emi / nb) / 1e12
-        roof = {"kernel": "emi_fast_kernel<64, true>", "bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS,
+        roof = {"kernel": "emi_fast_kernel<64, true, false>", "bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
                 "traffic": measured_traffic("emi_fast_kernel<64,tc>", args),
                 "note": "EMI with the DS temporal coherence fused; VALU-bound (FP64 sweep + FP32 Householder); "
