@@ -104,3 +104,18 @@ def test_api_signatures_match_reference_names():
         for k, v in ps.items():
             if v.kind == v.KEYWORD_ONLY:
                 assert v.default is not inspect.Parameter.empty, (name, k)
+
+
+def test_refine_argument_rules():
+    # the precision form's argument checks run on the host before any GPU call
+    from moraine_amd.pl import _check_refine
+
+    assert _check_refine(None, 60) is None
+    assert _check_refine(1e-3, 60) == 1e-3
+    assert _check_refine(2e-4, 33) == 2e-4 and _check_refine(2e-4, 64) == 2e-4
+    for bad in (0.0, -1e-3, float("nan")):
+        with pytest.raises(ValueError):
+            _check_refine(bad, 60)
+    for n in (2, 32, 65, 120):
+        with pytest.raises(ValueError):
+            _check_refine(1e-3, n)
