@@ -139,14 +139,15 @@ int mr_emi_ds_temp_coh(const float* coh, long long n_pts, int n, int ref, const 
                        float* ph, float* quality, float* t_coh, void* workspace, size_t workspace_bytes,
                        mr_stream_t stream);
 
-/* Precision form of emi / emi + ds_temp_coh for 32 < n <= 64, no reference
- * counterpart (moraine/pl.py:18-77 computes in the array library's precision):
- * the same one-wave solve as mr_emi, then every point whose estimated phase
- * error eps32 ||T|| / (gap min_i |v_i| sqrt(n)) exceeds tol (radians; T the
- * tridiagonal form, gap its two smallest eigenvalues' distance, v the unit
- * eigenvector) is solved again in FP64 throughout.  With t_coh non-null the
- * DS temporal coherence is written as mr_emi_ds_temp_coh does (pairs: the
- * full triu network).  Points not flagged get exactly mr_emi's outputs.
+/* Precision form of emi / emi + ds_temp_coh for 2 <= n <= 200 (above, mr_emi
+ * is FP64 throughout), no reference counterpart (moraine/pl.py:18-77 computes
+ * in the array library's precision): the same solve as mr_emi, then every
+ * point whose estimated phase error eps32 ||T|| n / (gap min_i |v_i| 60 sqrt 60)
+ * exceeds tol (radians; T the tridiagonal form, gap its two smallest
+ * eigenvalues' distance, v the unit eigenvector) is solved again in FP64
+ * throughout.  With t_coh non-null the DS temporal coherence is written as
+ * mr_emi_ds_temp_coh does (pairs: the full triu network).  Points not flagged
+ * get exactly mr_emi's outputs.
  * workspace >= mr_emi_refined_workspace_bytes(n_pts, n); once the stream has
  * passed the call, its first 4 bytes hold the number of re-solved points. */
 size_t mr_emi_refined_workspace_bytes(long long n_pts, int n);

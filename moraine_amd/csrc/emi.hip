@@ -189,6 +189,7 @@ __global__ __launch_bounds__(EMI_FAST_THREADS) void emi_fast_kernel(const float2
 // ----------------------------------------------------------------------------
 constexpr int EMI_GEN_THREADS = 256;
 constexpr int EMI_GEN_BLOCKS = 1024;
+constexpr int EMI_REF_BLOCKS = 256;  // FP64 re-solve blocks for n > 64 (each with a global slice)
 
 __host__ __device__ inline size_t emi_gen_slice_bytes(int n) {
   // A (n*n f64) + M (n*n c128) + vectors (col, u, q, p: n c128 each) + 12 n-double arrays
@@ -207,9 +208,9 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   return s;
 }
 
-// LIST: the FP64 re-solve of flagged points: the points are list[0 .. *count),
-// the block's slice lives in LDS (n <= 64), and with t_coh set each re-solved
-// point's DS temporal coherence is recomputed from its new phases.
+// LIST: the FP64 re-solve of flagged points: the points are list[0 .. *count);
+// for n <= 64 the block's slice lives in LDS, and with t_coh set each
+// re-solved point's DS temporal coherence is recomputed from its new phases.
 template <bool LIST>
 __global__ __launch_bounds__(EMI_GEN_THREADS) void emi_general_kernel(const float2* __restrict__ coh,
                                                                       long long npts, int n, int ref,
@@ -228,7 +229,9 @@ __global__ __launch_bounds__(EMI_GEN_THREADS) void emi_general_kernel(const floa
   extern __shared__ __align__(16) unsigned char gen_lds[];
   const int P = n * (n - 1) / 2;
   const int tid = threadIdx.x;
-  unsigned char* base = LIST ? gen_lds : ws + (size_t)blockIdx.x * emi_gen_slice_bytes(n);
+  // LIST with n <= 64: the slice in LDS, pivot and eigen-solve by one wave
+  const bool small = LIST && n <= 64;
+  unsigned char* base = small ? gen_lds : ws + (size_t)blockIdx.x * emi_gen_slice_bytes(n);
   double* A = reinterpret_cast<double*>(base);
   c128* M = reinterpret_cast<c128*>(A + (size_t)n * n);
   c128* col = M + (size_t)n * n;  // n
@@ -271,7 +274,7 @@ __global__ __launch_bounds__(EMI_GEN_THREADS) void emi_general_kernel(const floa
     __syncthreads();
     // sweep inverse
     for (int s = 0; s < n; ++s) {
-      if (LIST) {
+      if (small) {
         // n <= 64: the pivot (largest |d| not yet swept, ties to the smallest
         // row) from one wave
         if (tid < 64) {
@@ -410,7 +413,7 @@ __global__ __launch_bounds__(EMI_GEN_THREADS) void emi_general_kernel(const floa
       const double sr = ae > 0 ? e.re / ae : 1.0, si = ae > 0 ? e.im / ae : 0.0;
       dphr[n - 1] = dcr * sr - dci * si;
       dphi[n - 1] = dcr * si + dci * sr;
-      if (!LIST) {
+      if (!small) {
         // bisection (serial, FP64) + inverse iteration
         double lo = 1e308, hi = -1e308, sc = 0.0;
         for (int i = 0; i < n; ++i) {
@@ -444,7 +447,7 @@ __global__ __launch_bounds__(EMI_GEN_THREADS) void emi_general_kernel(const floa
       }
     }
     __syncthreads();
-    if (LIST) {
+    if (small) {
       // n <= 64: one wave finds the smallest eigenpair of T by multisection
       // (run to FP64 resolution) and a twisted factorisation
       if (tid < 64) {
@@ -495,7 +498,8 @@ __global__ __launch_bounds__(EMI_GEN_THREADS) void emi_general_kernel(const floa
   }
 }
 
-int emi_wide_launch(const float2* coh, long long npts, int n, int ref, float2* ph, float* quality, hipStream_t st);
+int emi_wide_launch(const float2* coh, long long npts, int n, int ref, float2* ph, float* quality, hipStream_t st,
+                    double tau = 0.0, unsigned* flag_count = nullptr, int* flag_list = nullptr);
 
 __global__ void emi_zero_count_kernel(unsigned* c) { *c = 0u; }
 
@@ -566,14 +570,18 @@ int mr_emi_ds_temp_coh(const float* coh, long long n_pts, int n, int ref, const 
 }
 
 size_t mr_emi_refined_workspace_bytes(long long n_pts, int n) {
-  (void)n;
-  return 256 + (((size_t)(n_pts > 0 ? n_pts : 0) * 4 + 255) & ~(size_t)255);
+  const size_t list = 256 + (((size_t)(n_pts > 0 ? n_pts : 0) * 4 + 255) & ~(size_t)255);
+  if (n <= 64) return list;
+  // n > 64: the FP64 re-solve's per-block slices in global memory
+  const long long nb = n_pts < mr::EMI_REF_BLOCKS ? n_pts : mr::EMI_REF_BLOCKS;
+  return list + (size_t)(nb > 0 ? nb : 1) * mr::emi_gen_slice_bytes(n);
 }
 
 int mr_emi_refined(const float* coh, long long n_pts, int n, int ref, double tol, const int32_t* pairs, int n_pairs,
                    float* ph, float* quality, float* t_coh, void* workspace, size_t workspace_bytes,
                    mr_stream_t stream) {
-  MR_CHECK_ARG(n > 32 && n <= 64, "emi_refined: covers 32 < n <= 64 (n=%d)", n);
+  MR_CHECK_ARG(n >= 2 && n <= mr::EMI_WIDE_MAX_N,
+               "emi_refined: covers 2 <= n <= %d (n=%d; above, mr_emi is FP64 throughout)", mr::EMI_WIDE_MAX_N, n);
   MR_CHECK_ARG(ref >= 0 && ref < n, "emi_refined: ref=%d outside [0,%d)", ref, n);
   MR_CHECK_ARG(n_pts >= 0 && n_pts < (1ll << 31), "emi_refined: point count %lld outside [0, 2^31)", n_pts);
   MR_CHECK_ARG(tol > 0.0, "emi_refined: tol must be positive");
@@ -585,26 +593,43 @@ int mr_emi_refined(const float* coh, long long n_pts, int n, int ref, double tol
   MR_CHECK_ARG(workspace != nullptr && workspace_bytes >= need, "emi_refined: workspace %zu < %zu bytes",
                workspace_bytes, need);
   hipStream_t st = mr::as_stream(stream);
-  unsigned* count = reinterpret_cast<unsigned*>(workspace);
-  int* list = reinterpret_cast<int*>(reinterpret_cast<unsigned char*>(workspace) + 256);
+  unsigned char* w = reinterpret_cast<unsigned char*>(workspace);
+  unsigned* count = reinterpret_cast<unsigned*>(w);
+  int* list = reinterpret_cast<int*>(w + 256);
   hipLaunchKernelGGL(mr::emi_zero_count_kernel, dim3(1), dim3(1), 0, st, count);
   if (int rc = mr::launch_status("emi_refined")) return rc;
   const float2* c = reinterpret_cast<const float2*>(coh);
   float2* p = reinterpret_cast<float2*>(ph);
-  const size_t lds = mr::emi_fast_lds(n, 64);
-  const int nb = (int)std::min<long long>(n_pts, 1 << 20);
-  if (t_coh)
-    hipLaunchKernelGGL((mr::emi_fast_kernel<64, true, true>), dim3(nb), dim3(mr::EMI_FAST_THREADS), lds, st, c, n_pts,
-                       n, ref, p, quality, pairs, t_coh, tol, count, list);
-  else
-    hipLaunchKernelGGL((mr::emi_fast_kernel<64, false, true>), dim3(nb), dim3(mr::EMI_FAST_THREADS), lds, st, c,
-                       n_pts, n, ref, p, quality, nullptr, nullptr, tol, count, list);
+  const bool fused = t_coh != nullptr && n > 32 && n <= 64;
+  if (n > 32 && n <= 64) {
+    const size_t lds = mr::emi_fast_lds(n, 64);
+    const int nb = (int)std::min<long long>(n_pts, 1 << 20);
+    if (fused)
+      hipLaunchKernelGGL((mr::emi_fast_kernel<64, true, true>), dim3(nb), dim3(mr::EMI_FAST_THREADS), lds, st, c,
+                         n_pts, n, ref, p, quality, pairs, t_coh, tol, count, list);
+    else
+      hipLaunchKernelGGL((mr::emi_fast_kernel<64, false, true>), dim3(nb), dim3(mr::EMI_FAST_THREADS), lds, st, c,
+                         n_pts, n, ref, p, quality, nullptr, nullptr, tol, count, list);
+    if (int rc = mr::launch_status("emi_refined")) return rc;
+  } else {
+    if (int rc = mr::emi_wide_launch(c, n_pts, n, ref, p, quality, st, tol, count, list)) return rc;
+  }
+  // FP64 re-solve of the flagged points, one block per point (slice in LDS for
+  // n <= 64, in the workspace after the list above)
+  if (n <= 64) {
+    const int nb2 = (int)std::min<long long>(n_pts, mr::EMI_GEN_BLOCKS);
+    hipLaunchKernelGGL(mr::emi_general_kernel<true>, dim3(nb2), dim3(mr::EMI_GEN_THREADS),
+                       mr::emi_gen_slice_bytes(n), st, c, n_pts, n, ref, p, quality, nullptr, list, count, pairs,
+                       fused ? t_coh : nullptr);
+  } else {
+    const int nb2 = (int)std::min<long long>(n_pts, mr::EMI_REF_BLOCKS);
+    unsigned char* slices = w + 256 + (((size_t)n_pts * 4 + 255) & ~(size_t)255);
+    hipLaunchKernelGGL(mr::emi_general_kernel<true>, dim3(nb2), dim3(mr::EMI_GEN_THREADS), 0, st, c, n_pts, n, ref,
+                       p, quality, slices, list, count, nullptr, nullptr);
+  }
   if (int rc = mr::launch_status("emi_refined")) return rc;
-  // FP64 re-solve of the flagged points, one block per point with its slice in LDS
-  const int nb2 = (int)std::min<long long>(n_pts, mr::EMI_GEN_BLOCKS);
-  hipLaunchKernelGGL(mr::emi_general_kernel<true>, dim3(nb2), dim3(mr::EMI_GEN_THREADS), mr::emi_gen_slice_bytes(n),
-                     st, c, n_pts, n, ref, p, quality, nullptr, list, count, pairs, t_coh);
-  return mr::launch_status("emi_refined");
+  if (t_coh && !fused) return mr_ds_temp_coh(coh, ph, n_pts, n, pairs, n_pairs, t_coh, stream);
+  return MR_OK;
 }
 
 }  // extern "C"

@@ -5,6 +5,9 @@
 
 namespace mr {
 
+// c(n) of the FP64 re-solve test of the EMI kernels: n / (60 sqrt 60)
+__host__ __device__ inline double emi_flag_scale(int n) { return (double)n / (60.0 * 7.745966692414834); }
+
 constexpr int EMI_WIDE_MAX_N = 200;  // largest n served by the register-resident kernels
 
 // Per-lane register array read at a wave-uniform runtime index through

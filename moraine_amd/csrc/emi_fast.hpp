@@ -439,13 +439,15 @@ __device__ __forceinline__ void emi_fast_point(const Src& coh_at, int n, int lan
   if constexpr (CHECK) {
     // The FP32 stages perturb M by about eps32 ||T||, which moves the unit
     // eigenvector by eps32 ||T|| / gap and the phase of entry i by that over
-    // |v_i|.  Flag the point when eps32 ||T|| / (gap min_i |v_i| sqrt(n)) > tau,
-    // i.e. when T (still in LDS) has a second eigenvalue below lambda + delta
-    // with delta = eps32 ||T|| / (tau min_i |v_i| sqrt(n)): one Sturm count.
+    // |v_i|.  Flag the point when eps32 ||T|| c(n) / (gap min_i |v_i|) > tau,
+    // c(n) = n / (60 sqrt 60) (measured against FP64: the worst errors sit
+    // near 1/sqrt(60) of the bare estimate at n = 60 and grow with n), i.e.
+    // when T (still in LDS) has a second eigenvalue below lambda + delta,
+    // delta = eps32 ||T|| c(n) / (tau min_i |v_i|): one Sturm count.
     const double av = lane < n ? (double)sqrtf(vr * vr + vi * vi) : 1e300;
     const double vmin = fmax(wmin_d(av), 1e-300);
     const double tn = wmax_d(lane < n ? fabs(my_d) : 0.0) + 2.0 * wmax_d(lane < n ? fabs(my_e) : 0.0);
-    const double x = rq + 5.9604644775390625e-8 * tn / (tau * vmin * sqrt((double)n));
+    const double x = rq + 5.9604644775390625e-8 * tn * emi_flag_scale(n) / (tau * vmin);
     const double pivmin = 1e-300 + tn * tn * 1e-290;
     double q = tde[0].x - x;
     int cnt = q < 0.0;

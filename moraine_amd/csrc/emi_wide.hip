@@ -76,11 +76,16 @@ __host__ __device__ inline size_t ew_lds_bytes(int nm) {
 // faster; n = 97..128 1.15x, with a few registers spilled outside the
 // Householder loops).  E = 16 (n <= 32, one wave per point) is held to four
 // waves per SIMD.
-template <int R, int E, int GS, int GH>
+// CHK: points whose conditioning test fails (tau; the test of the one-wave
+// kernel, with the Gershgorin bound of T for ||T||) are appended to
+// flag_list[*flag_count] for the FP64 re-solve.
+template <int R, int E, int GS, int GH, bool CHK = false>
 __global__ __launch_bounds__(R == 2 ? 512 : 64 * EW_MAX_WAVES)
 __attribute__((amdgpu_waves_per_eu(E <= 16 ? 4 : E <= 64 ? 3 : 1))) void emi_wide_kernel(const float2* __restrict__ coh, long long npts,
                                                                      int n, int ref, float2* __restrict__ ph,
-                                                                     float* __restrict__ quality) {
+                                                                     float* __restrict__ quality, double tau = 0.0,
+                                                                     unsigned* __restrict__ flag_count = nullptr,
+                                                                     int* __restrict__ flag_list = nullptr) {
   constexpr int NM = R * E;
   static_assert(E % GS == 0 && E % GH == 0, "groups must tile the row");
   extern __shared__ __align__(16) unsigned char smem[];
@@ -605,6 +610,29 @@ __attribute__((amdgpu_waves_per_eu(E <= 16 ? 4 : E <= 64 ? 3 : 1))) void emi_wid
       vr -= 2.f * (ar * sr - ai * si);
       vi -= 2.f * (ar * si + ai * sr);
     }
+    if constexpr (CHK) {
+      // phase error estimate eps32 ||T|| c(n) / (gap min_i |v_i|) > tau  <=>
+      // T has a second eigenvalue below lambda + eps32 ||T|| c(n) / (tau min_i |v_i|);
+      // T is still in td / te, dpl is free since the twisted factorisation
+      const double wm = wmin_d(live && c == 0 ? (double)sqrtf(vr * vr + vi * vi) : 1e300);
+      if (lane == 0) dpl[wv] = wm;
+      __syncthreads();
+      double vmin = dpl[0];
+      for (int w2 = 1; w2 < W; ++w2) vmin = fmin(vmin, dpl[w2]);
+      const double x = rq + 5.9604644775390625e-8 * scale * emi_flag_scale(n) / (tau * fmax(vmin, 1e-300));
+      if (wv == 0) {
+        double q = td[0] - x;
+        int cnt = q < 0.0;
+        for (int i = 1; i < n; ++i) {
+          if (fabs(q) < pivmin) q = -pivmin;
+          double r = __builtin_amdgcn_rcp(q);
+          r = fma(r, fma(-q, r, 1.0), r);
+          q = (td[i] - x) - (te[i - 1] * te[i - 1]) * r;
+          cnt += q < 0.0;
+        }
+        if (lane == 0 && cnt >= 2 && isfinite(x)) flag_list[atomicAdd(flag_count, 1u)] = (int)pt;
+      }
+    }
     // ---- output ---------------------------------------------------------------
     if (row == ref && c == 0) {
       misc[4] = vr;
@@ -623,21 +651,27 @@ __attribute__((amdgpu_waves_per_eu(E <= 16 ? 4 : E <= 64 ? 3 : 1))) void emi_wid
   }
 }
 
-int emi_wide_launch(const float2* coh, long long npts, int n, int ref, float2* ph, float* quality, hipStream_t st) {
+int emi_wide_launch(const float2* coh, long long npts, int n, int ref, float2* ph, float* quality, hipStream_t st,
+                    double tau, unsigned* flag_count, int* flag_list) {
   const int nb = (int)std::min<long long>(npts, 1 << 20);
-#define MR_EW(R, E, GS, GH)                                                                                     \
-  hipLaunchKernelGGL((emi_wide_kernel<R, E, GS, GH>), dim3(nb), dim3(((R) * n + 63) / 64 * 64),                \
-                     ew_lds_bytes((R) * (E)), st, coh, npts, n, ref, ph, quality)
-  if (n <= 32)
+#define MR_EW(R, E, GS, GH)                                                                                       \
+  if (flag_list)                                                                                                  \
+    hipLaunchKernelGGL((emi_wide_kernel<R, E, GS, GH, true>), dim3(nb), dim3(((R) * n + 63) / 64 * 64),          \
+                       ew_lds_bytes((R) * (E)), st, coh, npts, n, ref, ph, quality, tau, flag_count, flag_list);  \
+  else                                                                                                            \
+    hipLaunchKernelGGL((emi_wide_kernel<R, E, GS, GH>), dim3(nb), dim3(((R) * n + 63) / 64 * 64),                \
+                       ew_lds_bytes((R) * (E)), st, coh, npts, n, ref, ph, quality)
+  if (n <= 32) {
     MR_EW(2, 16, 8, 4);
-  else if (n <= 96)
+  } else if (n <= 96) {
     MR_EW(2, 48, 4, 2);
-  else if (n <= 128)
+  } else if (n <= 128) {
     MR_EW(2, 64, 4, 2);
-  else if (n <= 160)
+  } else if (n <= 160) {
     MR_EW(2, 80, 8, 4);
-  else
+  } else {
     MR_EW(2, 100, 4, 2);
+  }
 #undef MR_EW
   return launch_status("emi_wide");
 }

@@ -25,8 +25,8 @@ def _check_refine(refine, n: int):
     tol = float(refine)
     if not tol > 0.0:
         raise ValueError(f"refine must be a positive phase tolerance in radians, got {refine}")
-    if not 32 < n <= 64:
-        raise ValueError(f"refine covers 32 < N <= 64 images, got N={n}")
+    if n > 200:
+        return None  # EMI runs in FP64 throughout above 200 images
     return tol
 
 
@@ -62,9 +62,10 @@ def emi(coh, ref: int = 0, *, refine=None):
     """EMI phase history (n_pts, N) complex64 and quality (n_pts,) float32 from
     compressed coherence (n_pts, N(N-1)/2) in triu order.
 
-    refine (not in moraine; 32 < N <= 64): a phase tolerance in radians, e.g.
-    1e-4.  Points whose estimated phase error from the FP32 stages exceeds it
-    are solved again in FP64 throughout; the others are unchanged."""
+    refine (not in moraine): a phase tolerance in radians, e.g. 1e-3.  Points
+    whose estimated phase error from the FP32 stages exceeds it are solved
+    again in FP64 throughout; the others are unchanged.  (Above 200 images
+    EMI is FP64 throughout and refine has nothing to do.)"""
     as_torch = _dev.is_torch(coh)
     c = _dev.to_dev(coh, torch.complex64)
     n = nimage_from_npair(c.shape[-1])

@@ -97,6 +97,48 @@ def test_refine_argument_checks(coh60):
 
     with pytest.raises(ValueError):
         mr.emi(coh60, refine=0.0)
-    c30 = coh60[:8, : 30 * 29 // 2].contiguous()
     with pytest.raises(ValueError):
-        mr.emi(c30, refine=1e-4)
+        mr.emi(coh60, refine=-1e-4)
+
+
+@pytest.mark.parametrize("n,S", [(17, 96), (30, 96), (90, 48), (120, 48), (200, 24)])
+def test_refine_other_sizes(n, S):
+    """The one-wave (n <= 32) and workgroup (64 < n <= 200) kernels flag the
+    same way; flagged points are re-solved in FP64 (LDS slices for n <= 64,
+    workspace slices above).  Unflagged points keep the plain call's bits."""
+    import torch
+    import moraine_amd as mr
+
+    z = synth.ds_stack_torch(S, S, n, seed=n, ps_frac=0.1, device="cuda")
+    x = torch.view_as_real(z)
+    rmli = (x[..., 0] ** 2 + x[..., 1] ** 2).contiguous()
+    _, shp, cnt = mr.ks_test_select_shp(rmli, 7, 10, 0.05, return_p=False)
+    ii, jj = torch.meshgrid(torch.arange(S, device="cuda", dtype=torch.int32),
+                            torch.arange(S, device="cuda", dtype=torch.int32), indexing="ij")
+    idx = torch.stack([ii.reshape(-1), jj.reshape(-1)], -1).contiguous()
+    coh = mr.emperical_co_pc(z, idx, shp.view(-1, 15, 21)).contiguous()
+    _case(f"emi refine=1e-4 n={n}, {coh.shape[0]} points")
+    ph0, q0 = (a.cpu().numpy() for a in mr.emi(coh))
+    ph1, q1 = mr.emi(coh, refine=1e-4)
+    ph1, q1 = ph1.cpu().numpy(), q1.cpu().numpy()
+    n_ref = int(mr.pl.last_refined)
+    changed = np.any(ph0 != ph1, axis=1) | (q0 != q1)
+    print(f"n={n}: {n_ref} re-solved, {changed.sum()} changed of {ph0.shape[0]}")
+    assert changed.sum() <= n_ref <= ph0.shape[0]
+    c = coh.cpu().numpy()
+    fix = np.flatnonzero(changed)
+    if fix.size:
+        p64, _ = R.emi64(c[fix])
+        err = _rms_phase(ph1[fix], p64)
+        assert err.max() < 1e-5, err.max()  # FP64 throughout, phases rounded to complex64
+    # calibration of the flag: unflagged points of a sample stay within 3 tol of FP64
+    rest = np.setdiff1d(np.arange(0, ph0.shape[0], max(1, ph0.shape[0] // 400)), fix)
+    p64r, _ = R.emi64(c[rest])
+    err_r = _rms_phase(ph1[rest], p64r)
+    print(f"n={n}: unflagged sample of {rest.size}: max {err_r.max():.2e}")
+    assert err_r.max() < 3e-4, err_r.max()
+    # with the temporal coherence: equal to ds_temp_coh of the refined phases
+    ph2, q2, t2 = mr.pl.emi_ds_temp_coh(coh, refine=1e-4)
+    np.testing.assert_array_equal(ph2.cpu().numpy().view(np.uint32), ph1.view(np.uint32))
+    t_sep = mr.ds_temp_coh(coh, ph2).cpu().numpy()
+    np.testing.assert_array_equal(t2.cpu().numpy().view(np.uint32), t_sep.view(np.uint32))
