@@ -40,12 +40,12 @@ def _bits_equal(a, b):
 
 
 # ---------------------------------------------------------------------------- KS
-def test_ks_config1_bit_exact(port):
+def test_ks_200x200x17_11x15_bit_exact(port):
     """200x200x17 DS stack with 4 brightness regions, 11x15 window, p=0.05."""
     mr = _mr()
     z = synth.ds_stack(200, 200, 17, seed=1)
     rmli = (np.abs(z) ** 2).astype(np.float32)
-    _case("ks config1 200x200x17 az=5 r=7")
+    _case("ks 200x200x17 az=5 r=7")
     d, p = mr.ks_test(rmli, 5, 7, return_dist=True)
     d_ref, p_ref = port.ks_test(rmli, 5, 7, return_dist=True)
     _bits_equal(p, p_ref)
@@ -62,7 +62,7 @@ def test_ks_config1_bit_exact(port):
     np.testing.assert_array_equal(c2, c_ref)
 
 
-def test_ks_config2_model_subtile_bit_exact(port):
+def test_ks_n60_15x21_model_subtile_bit_exact(port):
     """N=60 with 10% PS-like pixels (constant amplitude -> tie-heavy), 15x21 window."""
     mr = _mr()
     z = synth.ds_stack(48, 96, 60, seed=2, ps_frac=0.1)
