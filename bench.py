@@ -226,6 +226,9 @@ def rows_leg(args, torch):
     del p
     hbm("ks_test_select_shp", dtime(lambda: mr.ks_test_select_shp(rmli, az, r, 0.05, return_p=False)),
         npx * (4 * N + W + 4), "rmli in, mask + count out")
+    hbm("rslc2amp", dtime(lambda: mr.rslc2amp(z)), npx * N * (8 + 4), f"{S}x{S}x{N} complex64 in, float32 out")
+    hbm("ks_test_select_shp_rslc", dtime(lambda: mr.ks_test_select_shp(z, az, r, 0.05, return_p=False)),
+        npx * (8 * N + W + 4), "complex64 stack in (amplitude while staging), mask + count out")
     _, shp, cnt = mr.ks_test_select_shp(rmli, az, r, 0.05, return_p=False)
     kmean = float(cnt.float().mean())
     co_fl = 4.0 * kmean * N * (N + 1)

@@ -61,6 +61,21 @@ int mr_ks_test_f64(const double* rmli, int nl, int w, int n, int az, int r, doub
                    double* p_out, double* dist_out, uint8_t* mask_out, int32_t* shp_num,
                    void* workspace, size_t workspace_bytes, mr_stream_t stream);
 
+/* rslc2amp (moraine/calamp.py:33-40): amp[i] = |rslc[i]|, complex64 (n
+ * interleaved re/im float pairs) -> float32, rounded as the C library's hypotf
+ * (glibc: the FP64 square sum's correctly rounded sqrt, rounded to FP32),
+ * which the reference's CPU path calls (moraine/calamp.py:17-25). */
+int mr_rslc2amp(const float* rslc, long long n, float* amp, mr_stream_t stream);
+
+/* ks_test on the amplitudes of a complex64 stack (nl, w, n): the CLI's
+ * rslc2amp then ks_test (moraine/cli/shp.py:98-101; moraine/shp.py:269-305)
+ * with the amplitude computed while the stacks are staged, so the amplitude
+ * array is never written.  Equals mr_ks_test_f32 on mr_rslc2amp's output bit
+ * for bit; outputs, workspace and limits as mr_ks_test_f32. */
+int mr_ks_test_rslc_f32(const float* rslc, int nl, int w, int n, int az, int r, double p_max,
+                        float* p_out, float* dist_out, uint8_t* mask_out, int32_t* shp_num,
+                        void* workspace, size_t workspace_bytes, mr_stream_t stream);
+
 /* Host-only (no GPU): the k -> (p, dist, is_shp) tables ks_test uses, for
  * k = 0..n (n+1 entries each; shp as 0/1 bytes).  Exposed so the tables can be
  * checked against the reference series without a device. */

@@ -6,6 +6,7 @@ emperical_co_emi_temp_coh_pc, TempNet, nimage_from_npair.  Every array
 function runs in hand-written HIP kernels (libmoraine_hip.so, C ABI in
 include/moraine_hip.h); there is no CPU fallback.
 """
+from .calamp import rslc2amp
 from .co import emperical_co, emperical_co_pc, uncompress_coh
 from .pl import ds_temp_coh, emi, emperical_co_emi_temp_coh_pc
 from .pqm import temp_coh
@@ -16,5 +17,5 @@ __version__ = "0.1.0"
 
 __all__ = [
     "ks_test", "ks_test_select_shp", "select_shp", "emperical_co", "emperical_co_pc", "uncompress_coh", "emi",
-    "ds_temp_coh", "temp_coh", "emperical_co_emi_temp_coh_pc", "TempNet", "nimage_from_npair",
+    "ds_temp_coh", "temp_coh", "emperical_co_emi_temp_coh_pc", "TempNet", "nimage_from_npair", "rslc2amp",
 ]

@@ -70,6 +70,33 @@ static void build_tables(int n, double p_max, KsTables<T>& t) {
 }
 #pragma clang fp contract(on)
 
+// |z| of a complex64 sample as moraine's CPU path computes it (rslc2amp,
+// moraine/calamp.py:17-25: numba's abs of a complex64 calls the C library's
+// hypotf, which in glibc is this): both squares and their sum in FP64 (the
+// squares are exact, the sum rounds once), a correctly rounded FP64 sqrt, one
+// rounding to FP32; an infinite part gives +inf even beside a NaN.
+#pragma clang fp contract(off)
+__device__ __forceinline__ float rslc_amp(float2 z) {
+  if (__builtin_isinf(z.x) || __builtin_isinf(z.y)) return __builtin_inff();
+  const double x = z.x, y = z.y;
+  const double s = x * x + y * y;
+  return (float)__builtin_sqrt(s);
+}
+#pragma clang fp contract(on)
+
+// rslc2amp: one float per complex sample, two per thread per trip
+__global__ __launch_bounds__(256) void rslc2amp_kernel(const float2* __restrict__ z, long long n,
+                                                       float* __restrict__ amp) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += 2 * stride) {
+    const float2 a = z[i];
+    const bool two = i + stride < n;
+    const float2 b = two ? z[i + stride] : make_float2(0.f, 0.f);
+    amp[i] = rslc_amp(a);
+    if (two) amp[i + stride] = rslc_amp(b);
+  }
+}
+
 template <typename T>
 struct KeyOf;
 template <>
@@ -94,10 +121,12 @@ struct KeyOf<double> {
 // ---------------------------------------------------------------------------
 // sort: one wave per pixel, NP = 64*E >= n keys in registers (index e*64+lane)
 // ---------------------------------------------------------------------------
-template <typename T, int E>
+// CPLX (T = float): the stacks are complex64 samples, sorted by rslc_amp.
+template <typename T, int E, bool CPLX = false>
 __global__ __launch_bounds__(256) void ks_sort_kernel(const T* __restrict__ rmli, long long npx, int n,
                                                       typename KeyOf<T>::type* __restrict__ keys,
-                                                      uint8_t* __restrict__ valid) {
+                                                      uint8_t* __restrict__ valid,
+                                                      const float2* __restrict__ rslc = nullptr) {
   using Key = typename KeyOf<T>::type;
   constexpr int NP = 64 * E;
   const int lane = threadIdx.x & 63;
@@ -110,7 +139,11 @@ __global__ __launch_bounds__(256) void ks_sort_kernel(const T* __restrict__ rmli
     for (int e = 0; e < E; ++e) {
       const int i = e * 64 + lane;
       if (i < n) {
-        const T x = rmli[px * n + i];
+        T x;
+        if constexpr (CPLX)
+          x = rslc_amp(rslc[px * n + i]);
+        else
+          x = rmli[px * n + i];
         has_nan |= (x != x);
         v[e] = KeyOf<T>::key(x);
       } else {
@@ -350,12 +383,14 @@ static int ks_pick_ilp(int pairs) {
   return cost(6) <= cost(4) ? 6 : 4;
 }
 
+// rslc (T = float only): KS on rslc_amp of a complex64 stack instead of rmli
 template <typename T>
 static int ks_test_impl(const T* rmli, int nl, int w, int n, int az, int r, double p_max, T* p_out, T* d_out,
-                        uint8_t* m_out, int32_t* shp_num, void* ws, size_t ws_bytes, mr_stream_t stream_) {
+                        uint8_t* m_out, int32_t* shp_num, void* ws, size_t ws_bytes, mr_stream_t stream_,
+                        const float2* rslc = nullptr) {
   using Key = typename KeyOf<T>::type;
   const int maxn = sizeof(T) == 4 ? MR_KS_MAX_N : KS_MAX_N_F64;
-  MR_CHECK_ARG(rmli != nullptr && ws != nullptr, "ks_test: null rmli or workspace");
+  MR_CHECK_ARG((rmli != nullptr || rslc != nullptr) && ws != nullptr, "ks_test: null rmli or workspace");
   MR_CHECK_ARG(nl > 0 && w > 0, "ks_test: empty raster (%d x %d)", nl, w);
   if (n < 1 || n > maxn) {
     set_error("ks_test: n=%d outside 1..%d", n, maxn);
@@ -370,12 +405,27 @@ static int ks_test_impl(const T* rmli, int nl, int w, int n, int az, int r, doub
   uint8_t* valid = reinterpret_cast<uint8_t*>(ws) + align256(npx * n * sizeof(Key));
 
   const int nb = (int)std::min<size_t>(div_up(npx, 4), 65536);
-  if (n <= 64)
-    hipLaunchKernelGGL((ks_sort_kernel<T, 1>), dim3(nb), dim3(256), 0, st, rmli, (long long)npx, n, keys, valid);
-  else if (n <= 128)
-    hipLaunchKernelGGL((ks_sort_kernel<T, 2>), dim3(nb), dim3(256), 0, st, rmli, (long long)npx, n, keys, valid);
-  else
-    hipLaunchKernelGGL((ks_sort_kernel<T, 4>), dim3(nb), dim3(256), 0, st, rmli, (long long)npx, n, keys, valid);
+  if constexpr (sizeof(T) == 4) {
+    if (rslc) {
+      if (n <= 64)
+        hipLaunchKernelGGL((ks_sort_kernel<T, 1, true>), dim3(nb), dim3(256), 0, st, nullptr, (long long)npx, n, keys,
+                           valid, rslc);
+      else if (n <= 128)
+        hipLaunchKernelGGL((ks_sort_kernel<T, 2, true>), dim3(nb), dim3(256), 0, st, nullptr, (long long)npx, n, keys,
+                           valid, rslc);
+      else
+        hipLaunchKernelGGL((ks_sort_kernel<T, 4, true>), dim3(nb), dim3(256), 0, st, nullptr, (long long)npx, n, keys,
+                           valid, rslc);
+    }
+  }
+  if (!rslc) {
+    if (n <= 64)
+      hipLaunchKernelGGL((ks_sort_kernel<T, 1>), dim3(nb), dim3(256), 0, st, rmli, (long long)npx, n, keys, valid);
+    else if (n <= 128)
+      hipLaunchKernelGGL((ks_sort_kernel<T, 2>), dim3(nb), dim3(256), 0, st, rmli, (long long)npx, n, keys, valid);
+    else
+      hipLaunchKernelGGL((ks_sort_kernel<T, 4>), dim3(nb), dim3(256), 0, st, rmli, (long long)npx, n, keys, valid);
+  }
   int rc = launch_status("ks_sort");
   if (rc) return rc;
 
@@ -430,6 +480,24 @@ int mr_ks_test_f64(const double* rmli, int nl, int w, int n, int az, int r, doub
                    mr_stream_t stream) {
   return mr::ks_test_impl<double>(rmli, nl, w, n, az, r, p_max, p_out, dist_out, mask_out, shp_num, workspace,
                                   workspace_bytes, stream);
+}
+
+int mr_ks_test_rslc_f32(const float* rslc, int nl, int w, int n, int az, int r, double p_max, float* p_out,
+                        float* dist_out, uint8_t* mask_out, int32_t* shp_num, void* workspace, size_t workspace_bytes,
+                        mr_stream_t stream) {
+  MR_CHECK_ARG(rslc != nullptr, "ks_test_rslc: null rslc");
+  return mr::ks_test_impl<float>(nullptr, nl, w, n, az, r, p_max, p_out, dist_out, mask_out, shp_num, workspace,
+                                 workspace_bytes, stream, reinterpret_cast<const float2*>(rslc));
+}
+
+int mr_rslc2amp(const float* rslc, long long n, float* amp, mr_stream_t stream) {
+  MR_CHECK_ARG(n >= 0, "rslc2amp: negative size");
+  if (n == 0) return MR_OK;
+  MR_CHECK_ARG(rslc && amp, "rslc2amp: null pointer");
+  const int nb = (int)std::min<long long>(mr::div_up((unsigned long long)n, 512), 1 << 16);
+  hipLaunchKernelGGL(mr::rslc2amp_kernel, dim3(nb), dim3(256), 0, mr::as_stream(stream),
+                     reinterpret_cast<const float2*>(rslc), n, amp);
+  return mr::launch_status("rslc2amp");
 }
 
 int mr_ks_tables_f32(int n, double p_max, float* p, float* dist, uint8_t* shp) {

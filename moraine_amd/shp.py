@@ -12,21 +12,23 @@ import torch
 
 from . import _dev, _lib
 
-_KS = {torch.float32: "mr_ks_test_f32", torch.float64: "mr_ks_test_f64"}
+_KS = {torch.float32: "mr_ks_test_f32", torch.float64: "mr_ks_test_f64", torch.complex64: "mr_ks_test_rslc_f32"}
 
 
 def _ks(rmli, az_half_win: int, r_half_win: int, want_p: bool, want_dist: bool, p_max, want_mask: bool):
     as_torch = _dev.is_torch(rmli)
     x = _dev.to_dev(rmli)
     if x.dtype not in _KS:
-        raise TypeError(f"ks_test: rmli must be float32 or float64, got {x.dtype}")
+        raise TypeError(f"ks_test: rmli must be float32, float64 or complex64, got {x.dtype}")
     if x.ndim != 3:
         raise ValueError("ks_test: rmli must be (nlines, width, nimages)")
+    x = x.contiguous()
     nl, w, n = x.shape
     aw, rw = 2 * az_half_win + 1, 2 * r_half_win + 1
     shape = (nl, w, aw, rw)
-    p = torch.empty(shape, dtype=x.dtype, device=x.device) if want_p else None
-    d = torch.empty(shape, dtype=x.dtype, device=x.device) if want_dist else None
+    rdt = torch.float32 if x.dtype == torch.complex64 else x.dtype
+    p = torch.empty(shape, dtype=rdt, device=x.device) if want_p else None
+    d = torch.empty(shape, dtype=rdt, device=x.device) if want_dist else None
     m = torch.empty(shape, dtype=torch.bool, device=x.device) if want_mask else None
     cnt = torch.empty((nl, w), dtype=torch.int32, device=x.device) if want_mask else None
     wsb = _lib.query("mr_ks_workspace_bytes", nl, w, n)
@@ -45,6 +47,9 @@ def ks_test(rmli, az_half_win: int, r_half_win: int, block_size: int = 128, retu
     or (dist, p) with ``return_dist``; shape (nlines, width, 2*az+1, 2*r+1), dtype
     of rmli.  NaN where the secondary is outside the image or either stack has a
     NaN.  ``block_size`` is accepted for signature compatibility and unused.
+    A complex64 stack (not accepted by moraine) is tested on its amplitudes:
+    the result equals ``ks_test(rslc2amp(rslc), ...)`` (float32) bit for bit,
+    without writing the amplitude array.
     """
     p, d, _, _ = _ks(rmli, az_half_win, r_half_win, True, return_dist, None, False)
     return (d, p) if return_dist else p

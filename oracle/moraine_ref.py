@@ -93,6 +93,26 @@ def uncompress_coh(coh: np.ndarray, image_pairs: np.ndarray | None = None) -> np
 
 
 # ---------------------------------------------------------------------------
+# SLC amplitude (moraine/calamp.py)
+# ---------------------------------------------------------------------------
+
+
+def rslc2amp(rslc: np.ndarray) -> np.ndarray:
+    """Restates ``_rslc2amp_numba`` (moraine/calamp.py:17-25): ``np.abs`` of each
+    complex64 sample under numba, i.e. the C library's ``hypotf``; glibc computes
+    it as the FP64 sum of the exact squares, a correctly rounded sqrt and one
+    rounding to float32, with +inf for an infinite part.  (NumPy's own complex64
+    ``abs`` rounds differently in the last bit.)"""
+    z = np.asarray(rslc, dtype=np.complex64)
+    x = z.real.astype(np.float64)
+    y = z.imag.astype(np.float64)
+    with np.errstate(invalid="ignore", over="ignore"):
+        a = np.sqrt(x * x + y * y).astype(np.float32)
+    a[np.isinf(z.real) | np.isinf(z.imag)] = np.inf
+    return a
+
+
+# ---------------------------------------------------------------------------
 # KS test (moraine/shp.py)
 # ---------------------------------------------------------------------------
 

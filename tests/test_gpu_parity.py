@@ -178,6 +178,49 @@ def test_co_zero_shp_is_nan():
     assert np.all(np.isnan(c))
 
 
+def test_rslc2amp_gpu():
+    """Amplitude of random bit patterns, specials and denormals, bit for bit with
+    the oracle (the C library's hypotf)."""
+    mr = _mr()
+    rng = np.random.default_rng(11)
+    xy = rng.integers(0, 2 ** 32, size=(200000, 2), dtype=np.uint64).astype(np.uint32).view(np.float32)
+    sp = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 1e-45, 1.17e-38, 3.4e38, 1.0, 3.0, 4.0], np.float32)
+    X, Y = np.meshgrid(sp, sp)
+    xy = np.concatenate([xy, np.stack([X.ravel(), Y.ravel()], -1), rng.standard_normal((9999, 2)).astype(np.float32)])
+    z = np.empty(xy.shape[0], np.complex64)
+    z.real[:] = xy[:, 0]
+    z.imag[:] = xy[:, 1]
+    _case("rslc2amp 210120 samples")
+    got = mr.rslc2amp(z.reshape(-1, 5, 4))
+    want = R.rslc2amp(z).reshape(-1, 5, 4)
+    assert got.shape == want.shape and got.dtype == np.float32
+    same = (got.view(np.uint32) == want.view(np.uint32)) | (np.isnan(got) & np.isnan(want))
+    assert same.all()
+
+
+@pytest.mark.parametrize("n", [17, 60, 130])
+def test_ks_from_complex_stack_equals_composition_gpu(n):
+    """ks_test / ks_test_select_shp on a complex64 stack equal the same calls on
+    rslc2amp of it, bit for bit (constant-amplitude PS pixels give ties; NaN samples)."""
+    import torch
+
+    mr = _mr()
+    z = synth.ds_stack_torch(40, 56, n, seed=n, ps_frac=0.1, device="cuda")
+    z[3, 5, 0] = complex(float("nan"), 0.0)
+    z[7, 9, n - 1] = complex(float("inf"), float("nan"))
+    _case(f"ks from complex stack n={n}")
+    amp = mr.rslc2amp(z)
+    d1, p1 = mr.ks_test(z, 5, 7, return_dist=True)
+    d2, p2 = mr.ks_test(amp, 5, 7, return_dist=True)
+    _bits_equal(p1.cpu().numpy(), p2.cpu().numpy())
+    _bits_equal(d1.cpu().numpy(), d2.cpu().numpy())
+    q1, m1, c1 = mr.ks_test_select_shp(z, 5, 7, 0.05)
+    q2, m2, c2 = mr.ks_test_select_shp(amp, 5, 7, 0.05)
+    _bits_equal(q1.cpu().numpy(), q2.cpu().numpy())
+    assert torch.equal(m1, m2) and torch.equal(c1, c2)
+    assert torch.isnan(p1[3, 5]).all()
+
+
 def test_uncompress_gpu():
     mr = _mr()
     c = (np.random.default_rng(1).random((3, 4, 45)) + 1j).astype(np.complex64)

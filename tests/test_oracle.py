@@ -294,3 +294,30 @@ def test_tcoh_port_and_golden(port):
     np.testing.assert_allclose(port.ds_temp_coh(g["coh"], g["ph"], pairs), g["t_ds"], atol=2e-6)
     np.testing.assert_allclose(port.temp_coh(g["coh"], g["ph"], pairs), g["t_pq"], atol=2e-6)
     np.testing.assert_allclose(R.ds_temp_coh64(g["coh"], g["ph"]), g["t_ds"], atol=2e-6)
+
+
+# ---------------------------------------------------------------- amplitude
+def test_rslc2amp_matches_libm_hypotf():
+    """The oracle's amplitude equals the C library's hypotf (what numba's abs of a
+    complex64 calls) on random bit patterns, specials and denormals."""
+    import ctypes
+
+    libm = ctypes.CDLL("libm.so.6")
+    hypotf = libm.hypotf
+    hypotf.restype = ctypes.c_float
+    hypotf.argtypes = [ctypes.c_float, ctypes.c_float]
+    rng = np.random.default_rng(5)
+    bits = rng.integers(0, 2 ** 32, size=(20000, 2), dtype=np.uint64).astype(np.uint32)
+    xy = bits.view(np.float32).reshape(-1, 2)
+    sp = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 1e-45, -1e-45, 1.17e-38, 3.4e38, -3.4e38, 1.0, 3.0, 4.0],
+                  np.float32)
+    X, Y = np.meshgrid(sp, sp)
+    xy = np.concatenate([xy, np.stack([X.ravel(), Y.ravel()], -1)])
+    z = (xy[:, 0].astype(np.complex64) + 1j * xy[:, 1].astype(np.complex64)).astype(np.complex64)
+    z.real[:] = xy[:, 0]
+    z.imag[:] = xy[:, 1]
+    got = R.rslc2amp(z)
+    want = np.array([hypotf(float(a), float(b)) for a, b in xy], np.float32)
+    same = (got.view(np.uint32) == want.view(np.uint32)) | (np.isnan(got) & np.isnan(want))
+    assert same.all(), xy[~same][:5]
+    assert R.rslc2amp(np.array([3 + 4j], np.complex64))[0] == 5.0
