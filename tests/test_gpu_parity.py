@@ -168,6 +168,29 @@ def test_co_pc_large_k_vs_fp64(n, az, r):
     np.testing.assert_allclose(cb, c64b, rtol=0, atol=1e-5)
 
 
+def test_ds_candidates_bandwidth_network_gpu():
+    """DS-candidate point cloud (shp_num >= 50) with a bandwidth-limited pair
+    network: coherence and DS temporal coherence on that network against the
+    oracle (phases from EMI on the full network)."""
+    mr = _mr()
+    n, az, r = 30, 7, 10
+    z = synth.ds_stack(36, 44, n, seed=21, ps_frac=0.1)
+    rmli = (np.abs(z) ** 2).astype(np.float32)
+    _, shp, cnt = mr.ks_test_select_shp(rmli, az, r, 0.05, return_p=False)
+    idx = np.argwhere(cnt >= 50).astype(np.int32)
+    assert 50 < idx.shape[0] < cnt.size
+    pc = shp[idx[:, 0], idx[:, 1]]
+    _case(f"DS candidates {idx.shape[0]} points, bandwidth-5 network, n={n}")
+    ph, _ = mr.emi(mr.emperical_co_pc(z, idx, pc))
+    bw = R.image_pairs_from_bandwidth(n, 5)
+    cb = mr.emperical_co_pc(z, idx, pc, image_pairs=bw)
+    np.testing.assert_allclose(cb, R.emperical_co_pc64(z, idx, pc, bw), rtol=0, atol=1e-5)
+    t = mr.ds_temp_coh(cb, ph, image_pairs=bw)
+    t_ref = R.ds_temp_coh(cb, ph, bw)
+    # 1e-5 relative (float32 accumulation order differs)
+    np.testing.assert_allclose(t, t_ref, rtol=1e-5, atol=1e-6)
+
+
 def test_co_zero_shp_is_nan():
     mr = _mr()
     z = synth.ds_stack(4, 4, 17, seed=3)
