@@ -227,6 +227,7 @@ def rows_leg(args, torch):
     hbm("ks_test_select_shp", dtime(lambda: mr.ks_test_select_shp(rmli, az, r, 0.05, return_p=False)),
         npx * (4 * N + W + 4), "rmli in, mask + count out")
     hbm("rslc2amp", dtime(lambda: mr.rslc2amp(z)), npx * N * (8 + 4), f"{S}x{S}x{N} complex64 in, float32 out")
+    hbm("amp_disp", dtime(lambda: mr.amp_disp(z)), npx * (8 * N + 4), f"{S}x{S}x{N} complex64 in, float32 out")
     hbm("ks_test_select_shp_rslc", dtime(lambda: mr.ks_test_select_shp(z, az, r, 0.05, return_p=False)),
         npx * (8 * N + W + 4), "complex64 stack in (amplitude while staging), mask + count out")
     _, shp, cnt = mr.ks_test_select_shp(rmli, az, r, 0.05, return_p=False)

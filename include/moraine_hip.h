@@ -67,6 +67,11 @@ int mr_ks_test_f64(const double* rmli, int nl, int w, int n, int az, int r, doub
  * which the reference's CPU path calls (moraine/calamp.py:17-25). */
 int mr_rslc2amp(const float* rslc, long long n, float* amp, mr_stream_t stream);
 
+/* amp_disp (moraine/ps.py:17-72): amplitude dispersion index per pixel of a
+ * complex64 stack (nl, w, n): std(|z|) / mean(|z|) over the n images
+ * (population std), float32 (nl, w).  |z| as mr_rslc2amp. */
+int mr_amp_disp(const float* rslc, int nl, int w, int n, float* out, mr_stream_t stream);
+
 /* ks_test on the amplitudes of a complex64 stack (nl, w, n): the CLI's
  * rslc2amp then ks_test (moraine/cli/shp.py:98-101; moraine/shp.py:269-305)
  * with the amplitude computed while the stacks are staged, so the amplitude

@@ -10,6 +10,7 @@ from .calamp import rslc2amp
 from .co import emperical_co, emperical_co_pc, uncompress_coh
 from .pl import ds_temp_coh, emi, emperical_co_emi_temp_coh_pc
 from .pqm import temp_coh
+from .ps import amp_disp
 from .shp import ks_test, ks_test_select_shp, select_shp
 from .tnet import TempNet, nimage_from_npair
 
@@ -18,4 +19,5 @@ __version__ = "0.1.0"
 __all__ = [
     "ks_test", "ks_test_select_shp", "select_shp", "emperical_co", "emperical_co_pc", "uncompress_coh", "emi",
     "ds_temp_coh", "temp_coh", "emperical_co_emi_temp_coh_pc", "TempNet", "nimage_from_npair", "rslc2amp",
+    "amp_disp",
 ]

@@ -97,6 +97,53 @@ __global__ __launch_bounds__(256) void rslc2amp_kernel(const float2* __restrict_
   }
 }
 
+// amp_disp (moraine/ps.py:17-29): per pixel, the amplitudes a_k = |z_k| of its
+// n images, D = std(a) / mean(a) with the population std.  One wave per pixel,
+// the sums in FP64 (the reference sums in FP32; its tests hold to 1e-6).  For
+// n <= 64 a wave takes AD_U pixels at a time, one amplitude per lane held in a
+// register, so the AD_U pixels' wave sums overlap.
+constexpr int AD_U = 4;
+__global__ __launch_bounds__(256) void amp_disp_kernel(const float2* __restrict__ z, long long npx, int n,
+                                                       float* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const long long wave0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
+  if (n <= 64) {
+    for (long long p0 = wave0 * AD_U; p0 < npx; p0 += nw * AD_U) {
+      float a[AD_U];
+      double s[AD_U];
+#pragma unroll
+      for (int u = 0; u < AD_U; ++u) {
+        const long long px = p0 + u;
+        a[u] = (px < npx && lane < n) ? rslc_amp(z[px * n + lane]) : 0.f;
+      }
+#pragma unroll
+      for (int u = 0; u < AD_U; ++u) s[u] = wave_sum((double)a[u]);
+#pragma unroll
+      for (int u = 0; u < AD_U; ++u) {
+        const double mean = s[u] / (double)n;
+        const double d = lane < n ? (double)a[u] - mean : 0.0;
+        const double sd = sqrt(wave_sum(d * d) / (double)n);
+        if (lane == 0 && p0 + u < npx) out[p0 + u] = (float)(sd / mean);
+      }
+    }
+    return;
+  }
+  for (long long px = wave0; px < npx; px += nw) {
+    const float2* zp = z + px * n;
+    double s = 0.0;
+    for (int i = lane; i < n; i += 64) s += (double)rslc_amp(zp[i]);
+    const double mean = wave_sum(s) / (double)n;
+    double q = 0.0;
+    for (int i = lane; i < n; i += 64) {
+      const double d = (double)rslc_amp(zp[i]) - mean;
+      q += d * d;
+    }
+    const double sd = sqrt(wave_sum(q) / (double)n);
+    if (lane == 0) out[px] = (float)(sd / mean);
+  }
+}
+
 template <typename T>
 struct KeyOf;
 template <>
@@ -498,6 +545,17 @@ int mr_rslc2amp(const float* rslc, long long n, float* amp, mr_stream_t stream) 
   hipLaunchKernelGGL(mr::rslc2amp_kernel, dim3(nb), dim3(256), 0, mr::as_stream(stream),
                      reinterpret_cast<const float2*>(rslc), n, amp);
   return mr::launch_status("rslc2amp");
+}
+
+int mr_amp_disp(const float* rslc, int nl, int w, int n, float* out, mr_stream_t stream) {
+  MR_CHECK_ARG(nl >= 0 && w >= 0 && n >= 1, "amp_disp: bad sizes (%d x %d x %d)", nl, w, n);
+  const long long npx = (long long)nl * w;
+  if (npx == 0) return MR_OK;
+  MR_CHECK_ARG(rslc && out, "amp_disp: null pointer");
+  const int nb = (int)std::min<long long>(mr::div_up((unsigned long long)npx, 4 * mr::AD_U), 1 << 16);
+  hipLaunchKernelGGL(mr::amp_disp_kernel, dim3(nb), dim3(256), 0, mr::as_stream(stream),
+                     reinterpret_cast<const float2*>(rslc), npx, n, out);
+  return mr::launch_status("amp_disp");
 }
 
 int mr_ks_tables_f32(int n, double p_max, float* p, float* dist, uint8_t* shp) {

@@ -112,6 +112,13 @@ def rslc2amp(rslc: np.ndarray) -> np.ndarray:
     return a
 
 
+def amp_disp(rslc: np.ndarray) -> np.ndarray:
+    """Restates ``_amp_disp_numba`` (moraine/ps.py:17-29): per pixel amp = |z|
+    (as rslc2amp), std(amp) / mean(amp) with the population std, in FP64."""
+    a = rslc2amp(rslc).astype(np.float64)
+    return (a.std(axis=-1) / a.mean(axis=-1)).astype(np.float32)
+
+
 # ---------------------------------------------------------------------------
 # KS test (moraine/shp.py)
 # ---------------------------------------------------------------------------

@@ -244,6 +244,19 @@ def test_ks_from_complex_stack_equals_composition_gpu(n):
     assert torch.isnan(p1[3, 5]).all()
 
 
+def test_amp_disp_gpu():
+    """Amplitude dispersion vs the oracle on the DS/PS model with NaN samples
+    (1e-6 relative: the reference sums in FP32, both sides here in FP64)."""
+    mr = _mr()
+    z = synth.ds_stack(33, 47, 60, seed=4, ps_frac=0.1)
+    z[2, 3, 7] = np.nan
+    _case("amp_disp 33x47x60")
+    d = mr.amp_disp(z)
+    want = R.amp_disp(z)
+    assert np.isnan(d[2, 3]) and np.isnan(want[2, 3])
+    np.testing.assert_allclose(d, want, rtol=1e-6, atol=1e-7)
+
+
 def test_uncompress_gpu():
     mr = _mr()
     c = (np.random.default_rng(1).random((3, 4, 45)) + 1j).astype(np.complex64)

@@ -97,6 +97,7 @@ def test_api_signatures_match_reference_names():
         "emperical_co_emi_temp_coh_pc": ["rslc", "idx", "pc_is_shp", "batch_size"],
         "uncompress_coh": ["coh", "image_pairs"],
         "rslc2amp": ["rslc"],
+        "amp_disp": ["rslc"],
     }
     for name, params in sig.items():
         ps = inspect.signature(getattr(mr, name)).parameters

@@ -321,3 +321,14 @@ def test_rslc2amp_matches_libm_hypotf():
     same = (got.view(np.uint32) == want.view(np.uint32)) | (np.isnan(got) & np.isnan(want))
     assert same.all(), xy[~same][:5]
     assert R.rslc2amp(np.array([3 + 4j], np.complex64))[0] == 5.0
+
+
+def test_amp_disp_closed_forms():
+    """Amplitude dispersion: constant amplitude -> 0; amplitudes {1, 3} -> std 1 /
+    mean 2 = 0.5; phases do not matter; scale invariant."""
+    ph = np.exp(1j * np.linspace(0, 6, 8)).astype(np.complex64)
+    z = np.stack([2 * ph, np.where(np.arange(8) % 2 == 0, 1, 3) * ph]).reshape(1, 2, 8).astype(np.complex64)
+    d = R.amp_disp(z)
+    assert d.shape == (1, 2)
+    assert abs(d[0, 0]) < 1e-7 and abs(d[0, 1] - 0.5) < 1e-6
+    np.testing.assert_allclose(R.amp_disp(z * np.float32(7.5)), d, rtol=1e-6, atol=1e-7)
