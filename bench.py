@@ -245,6 +245,10 @@ def rows_leg(args, torch):
     npt = idx.shape[0]
     flops("emperical_co_pc", dtime(lambda: mr.emperical_co_pc(z, idx, pcs)), npt * co_fl, 2500.0,
           f"{npt} points (512x512), full pair network")
+    z0, z1 = z[..., 0].contiguous(), z[..., 1].contiguous()
+    hbm("ad_intf_pc", dtime(lambda: mr.ad_intf_pc(z0, z1, idx, pcs)), npt * (W + 8 + 8 + 16),
+        f"{npt} points, one pair (SHP mask + index in, intf out, each pixel's two samples once)")
+    del z0, z1
     coh = mr.emperical_co_pc(z, idx, pcs)
     ch = coh[:65536].contiguous()
     hbm("uncompress_coh", dtime(lambda: mr.uncompress_coh(ch)), 65536 * (8 * P + 8 * N * N),

@@ -126,6 +126,14 @@ size_t mr_emi_workspace_bytes(long long n_pts, int n);
 int mr_emi(const float* coh, long long n_pts, int n, int ref, float* ph, float* quality,
            void* workspace, size_t workspace_bytes, mr_stream_t stream);
 
+/* ad_intf_pc (moraine/co.py:302-406): adaptively multilooked interferogram of
+ * one image pair per point: over the SHPs of the point's aw x rw window that
+ * lie inside the nl x w raster, sum ref conj(sec) / sqrt(sum|ref|^2 sum|sec|^2)
+ * (FP64 sums), complex64 (n_pc,); NaN without SHPs.  ref_rslc, sec_rslc:
+ * complex64 (nl, w); idx (n_pc, 2) int32; pc_is_shp (n_pc, aw, rw) bytes. */
+int mr_ad_intf_pc(const float* ref_rslc, const float* sec_rslc, int nl, int w, const int32_t* idx,
+                  const uint8_t* pc_is_shp, long long n_pc, int aw, int rw, float* intf, mr_stream_t stream);
+
 /* uncompress_coh (moraine/co.py:281-299): (n_pts, n_pairs) -> (n_pts, n, n),
  * [a,b] = c, [b,a] = conj(c), diagonal 1, other entries 0, resolved in the
  * reference's order when entries collide: all pairs, then all conjugates, a

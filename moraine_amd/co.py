@@ -58,6 +58,25 @@ def emperical_co_pc(rslc, idx, pc_is_shp, block_size: int = 128, image_pairs=Non
     return _dev.back(coh, as_torch)
 
 
+def ad_intf_pc(ref_rslc, sec_rslc, idx, pc_is_shp, block_size: int = 128):
+    """Adaptively multilooked interferogram of one image pair per point,
+    (n_pc,) complex64 (moraine/co.py:376-406); ``block_size`` is unused."""
+    as_torch = _dev.is_torch(ref_rslc)
+    a = _dev.to_dev(ref_rslc, torch.complex64).contiguous()
+    b = _dev.to_dev(sec_rslc, torch.complex64).contiguous()
+    if a.ndim != 2 or a.shape != b.shape:
+        raise ValueError("ad_intf_pc: ref_rslc and sec_rslc must be (nlines, width) of one shape")
+    nl, w = a.shape
+    ix = _dev.to_dev(idx, torch.int32).contiguous()
+    m = _dev.to_dev(pc_is_shp, torch.bool).contiguous()
+    aw, rw = m.shape[-2:]
+    n_pc = ix.shape[0]
+    out = torch.empty(n_pc, dtype=torch.complex64, device=a.device)
+    _lib.call("mr_ad_intf_pc", _dev.ptr(a), _dev.ptr(b), nl, w, _dev.ptr(ix), _dev.ptr(m), n_pc, aw, rw,
+              _dev.ptr(out), _dev.stream())
+    return _dev.back(out, as_torch)
+
+
 def uncompress_coh(coh, image_pairs=None):
     """(..., n_pairs) compressed -> (..., n, n) Hermitian with unit diagonal."""
     as_torch = _dev.is_torch(coh)

@@ -300,6 +300,45 @@ static int co_check(int n, int aw, int rw) {
   return MR_OK;
 }
 
+// ad_intf_pc (moraine/co.py:302-406): adaptively multilooked interferogram of
+// one image pair at each point: the SHPs of its window (inside the raster)
+// give sum ref conj(sec) / sqrt(sum |ref|^2 sum |sec|^2).  One wave per point,
+// lanes over the window, FP64 sums as the reference's CPU path (complex128 /
+// float64 accumulators); no SHP gives 0/0 = NaN as there.
+__global__ __launch_bounds__(256) void ad_intf_pc_kernel(const float2* __restrict__ ref, const float2* __restrict__ sec,
+                                                         int nl, int w, const int32_t* __restrict__ idx,
+                                                         const bool* __restrict__ shp, long long n_pc, int aw, int rw,
+                                                         float2* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const long long wave0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
+  const int win = aw * rw, ah = aw / 2, rh = rw / 2;
+  for (long long i = wave0; i < n_pc; i += nw) {
+    const int a0 = idx[2 * i] - ah, r0 = idx[2 * i + 1] - rh;
+    double nr = 0.0, ni = 0.0, ra = 0.0, sa = 0.0;
+    for (int e = lane; e < win; e += 64) {
+      const int k = e / rw, l = e - (e / rw) * rw;
+      const int az = a0 + k, rg = r0 + l;
+      if (az >= 0 && az < nl && rg >= 0 && rg < w && shp[i * win + e]) {
+        const float2 x = ref[(long long)az * w + rg], y = sec[(long long)az * w + rg];
+        const double xr = x.x, xi = x.y, yr = y.x, yi = y.y;
+        ra += xr * xr + xi * xi;
+        sa += yr * yr + yi * yi;
+        nr += xr * yr + xi * yi;  // x conj(y)
+        ni += xi * yr - xr * yi;
+      }
+    }
+    nr = wave_sum(nr);
+    ni = wave_sum(ni);
+    ra = wave_sum(ra);
+    sa = wave_sum(sa);
+    if (lane == 0) {
+      const double d = sqrt(ra * sa);
+      out[i] = make_float2((float)(nr / d), (float)(ni / d));
+    }
+  }
+}
+
 }  // namespace mr
 
 extern "C" {
@@ -373,6 +412,19 @@ int mr_uncompress_coh(const float* coh, long long n_pts, int n, const int32_t* p
                        lut, mr::as_stream(stream), reinterpret_cast<const float2*>(coh), n_pts, n, pairs, n_pairs,
                        band, per, 1, reinterpret_cast<float2*>(out));
   return mr::launch_status("uncompress_coh");
+}
+
+int mr_ad_intf_pc(const float* ref_rslc, const float* sec_rslc, int nl, int w, const int32_t* idx,
+                  const uint8_t* pc_is_shp, long long n_pc, int aw, int rw, float* intf, mr_stream_t stream) {
+  MR_CHECK_ARG(nl > 0 && w > 0 && n_pc >= 0, "ad_intf_pc: bad sizes");
+  MR_CHECK_ARG(aw >= 1 && rw >= 1 && (aw & 1) && (rw & 1), "ad_intf_pc: window %d x %d must be odd", aw, rw);
+  if (n_pc == 0) return MR_OK;
+  MR_CHECK_ARG(ref_rslc && sec_rslc && idx && pc_is_shp && intf, "ad_intf_pc: null pointer");
+  const int nb = (int)std::min<long long>(mr::div_up((unsigned long long)n_pc, 4), 1 << 16);
+  hipLaunchKernelGGL(mr::ad_intf_pc_kernel, dim3(nb), dim3(256), 0, mr::as_stream(stream),
+                     reinterpret_cast<const float2*>(ref_rslc), reinterpret_cast<const float2*>(sec_rslc), nl, w, idx,
+                     reinterpret_cast<const bool*>(pc_is_shp), n_pc, aw, rw, reinterpret_cast<float2*>(intf));
+  return mr::launch_status("ad_intf_pc");
 }
 
 }  // extern "C"

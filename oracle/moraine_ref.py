@@ -70,6 +70,29 @@ def nimage_from_npair(npair: int) -> int:
     raise ValueError("input is not correct value of number of all pairs")
 
 
+def ad_intf_pc(ref_rslc, sec_rslc, idx, pc_is_shp) -> np.ndarray:
+    """Restates ``_ad_intf_pc_numba`` (moraine/co.py:302-333) with FP64 sums:
+    over window entries inside the raster and marked SHP, sum ref*conj(sec) /
+    sqrt(sum|ref|^2 * sum|sec|^2)."""
+    ref = np.asarray(ref_rslc, np.complex64).astype(np.complex128)
+    sec = np.asarray(sec_rslc, np.complex64).astype(np.complex128)
+    nl, w = ref.shape
+    aw, rw = pc_is_shp.shape[1:]
+    out = np.empty(idx.shape[0], np.complex64)
+    for i, (a, r) in enumerate(idx):
+        num, ra, sa = 0j, 0.0, 0.0
+        for k in range(aw):
+            for l in range(rw):
+                y, x = a - aw // 2 + k, r - rw // 2 + l
+                if 0 <= y < nl and 0 <= x < w and pc_is_shp[i, k, l]:
+                    num += ref[y, x] * np.conj(sec[y, x])
+                    ra += abs(ref[y, x]) ** 2
+                    sa += abs(sec[y, x]) ** 2
+        with np.errstate(invalid="ignore", divide="ignore"):
+            out[i] = num / np.sqrt(ra * sa) if ra * sa > 0 else complex(np.nan, np.nan)
+    return out
+
+
 def uncompress_coh(coh: np.ndarray, image_pairs: np.ndarray | None = None) -> np.ndarray:
     """Compressed (..., P) -> Hermitian (..., N, N), diag = 1.
 

@@ -257,6 +257,24 @@ def test_amp_disp_gpu():
     np.testing.assert_allclose(d, want, rtol=1e-6, atol=1e-7)
 
 
+def test_ad_intf_pc_gpu():
+    """Adaptive multilooked interferogram vs the oracle: random SHP masks, points
+    at the raster edges, one point without SHPs (NaN)."""
+    mr = _mr()
+    z = synth.ds_stack(30, 40, 12, seed=8, ps_frac=0.1)
+    rng = np.random.default_rng(8)
+    idx = np.concatenate([np.stack([rng.integers(0, 30, 200), rng.integers(0, 40, 200)], -1),
+                          [[0, 0], [29, 39], [0, 39], [15, 20]]]).astype(np.int32)
+    shp = rng.random((idx.shape[0], 15, 21)) < 0.5
+    shp[-1] = False
+    _case("ad_intf_pc 204 points 15x21")
+    got = mr.ad_intf_pc(z[..., 2], z[..., 7], idx, shp)
+    want = R.ad_intf_pc(z[..., 2], z[..., 7], idx, shp)
+    assert np.isnan(got[-1].real) and np.isnan(want[-1].real)
+    # FP64 sums on both sides, complex64 out
+    np.testing.assert_allclose(got[:-1], want[:-1], rtol=0, atol=1e-6)
+
+
 def test_uncompress_gpu():
     mr = _mr()
     c = (np.random.default_rng(1).random((3, 4, 45)) + 1j).astype(np.complex64)

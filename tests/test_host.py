@@ -98,6 +98,7 @@ def test_api_signatures_match_reference_names():
         "uncompress_coh": ["coh", "image_pairs"],
         "rslc2amp": ["rslc"],
         "amp_disp": ["rslc"],
+        "ad_intf_pc": ["ref_rslc", "sec_rslc", "idx", "pc_is_shp", "block_size"],
     }
     for name, params in sig.items():
         ps = inspect.signature(getattr(mr, name)).parameters

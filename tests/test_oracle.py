@@ -332,3 +332,20 @@ def test_amp_disp_closed_forms():
     assert d.shape == (1, 2)
     assert abs(d[0, 0]) < 1e-7 and abs(d[0, 1] - 0.5) < 1e-6
     np.testing.assert_allclose(R.amp_disp(z * np.float32(7.5)), d, rtol=1e-6, atol=1e-7)
+
+
+def test_ad_intf_pc_closed_forms():
+    """sec = ref * exp(i phi) gives exp(-i phi) at every point with an SHP
+    inside the raster; no SHP gives NaN; windows clipped at the raster edge."""
+    rng = np.random.default_rng(3)
+    ref = (rng.standard_normal((9, 11)) + 1j * rng.standard_normal((9, 11))).astype(np.complex64)
+    sec = (ref * np.exp(1j * 0.7)).astype(np.complex64)
+    idx = np.array([[0, 0], [4, 5], [8, 10], [2, 9]], np.int32)
+    shp = np.ones((4, 3, 5), bool)
+    shp[3] = False
+    out = R.ad_intf_pc(ref, sec, idx, shp)
+    np.testing.assert_allclose(out[:3], np.exp(-1j * 0.7), atol=1e-6)
+    assert np.isnan(out[3].real)
+    # a different secondary: |intf| <= 1 (Cauchy-Schwarz), equals 1 only for proportional stacks
+    sec2 = (rng.standard_normal((9, 11)) + 1j * rng.standard_normal((9, 11))).astype(np.complex64)
+    assert np.all(np.abs(R.ad_intf_pc(ref, sec2, idx[:3], shp[:3])) < 1)
