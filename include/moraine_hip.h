@@ -134,6 +134,13 @@ int mr_emi(const float* coh, long long n_pts, int n, int ref, float* ph, float* 
 int mr_ad_intf_pc(const float* ref_rslc, const float* sec_rslc, int nl, int w, const int32_t* idx,
                   const uint8_t* pc_is_shp, long long n_pc, int aw, int rw, float* intf, mr_stream_t stream);
 
+/* regularize_spectral (moraine/co.py:466-477): out = (1 - beta) coh + beta I
+ * for n_mat complex64 n x n matrices; beta (device, FP64) holds one value, or
+ * one per matrix when beta_per_matrix != 0; FP64 arithmetic, complex64 out
+ * (out may equal coh). */
+int mr_regularize_spectral(const float* coh, long long n_mat, int n, const double* beta, int beta_per_matrix,
+                           float* out, mr_stream_t stream);
+
 /* uncompress_coh (moraine/co.py:281-299): (n_pts, n_pairs) -> (n_pts, n, n),
  * [a,b] = c, [b,a] = conj(c), diagonal 1, other entries 0, resolved in the
  * reference's order when entries collide: all pairs, then all conjugates, a

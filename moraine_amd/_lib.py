@@ -33,6 +33,7 @@ SIGNATURES = {
     "mr_emi": (_i, [_vp, _ll, _i, _i, _vp, _vp, _vp, _sz, _vp]),
     "mr_uncompress_coh": (_i, [_vp, _ll, _i, _vp, _i, _vp, _vp]),
     "mr_ad_intf_pc": (_i, [_vp, _vp, _i, _i, _vp, _vp, _ll, _i, _i, _vp, _vp]),
+    "mr_regularize_spectral": (_i, [_vp, _ll, _i, _vp, _i, _vp, _vp]),
     "mr_ds_temp_coh": (_i, [_vp, _vp, _ll, _i, _vp, _i, _vp, _vp]),
     "mr_emi_ds_temp_coh": (_i, [_vp, _ll, _i, _i, _vp, _i, _vp, _vp, _vp, _vp, _sz, _vp]),
     "mr_temp_coh": (_i, [_vp, _vp, _ll, _i, _vp, _i, _vp, _vp]),

@@ -77,6 +77,28 @@ def ad_intf_pc(ref_rslc, sec_rslc, idx, pc_is_shp, block_size: int = 128):
     return _dev.back(out, as_torch)
 
 
+def regularize_spectral(coh, beta):
+    """(1 - beta) coh + beta I for a stack (..., N, N); beta a number or an array
+    of the stack's leading shape.  complex64 out (the reference's dtype follows
+    its array library's promotion rules)."""
+    as_torch = _dev.is_torch(coh)
+    c = _dev.to_dev(coh, torch.complex64).contiguous()
+    n = c.shape[-1]
+    if c.ndim < 2 or c.shape[-2] != n:
+        raise ValueError("regularize_spectral: coh must be (..., N, N)")
+    lead = tuple(c.shape[:-2])
+    nmat = int(np.prod(lead)) if lead else 1
+    b = beta.detach().to(torch.float64) if torch.is_tensor(beta) else torch.as_tensor(np.asarray(beta, np.float64))
+    if b.numel() == 1:
+        b, per = b.reshape(1), 0
+    else:
+        b, per = torch.broadcast_to(b.reshape(b.shape), lead).reshape(-1), 1
+    b = b.to(c.device).contiguous()
+    out = torch.empty_like(c)
+    _lib.call("mr_regularize_spectral", _dev.ptr(c), nmat, n, _dev.ptr(b), per, _dev.ptr(out), _dev.stream())
+    return _dev.back(out, as_torch)
+
+
 def uncompress_coh(coh, image_pairs=None):
     """(..., n_pairs) compressed -> (..., n, n) Hermitian with unit diagonal."""
     as_torch = _dev.is_torch(coh)

@@ -339,6 +339,27 @@ __global__ __launch_bounds__(256) void ad_intf_pc_kernel(const float2* __restric
   }
 }
 
+// regularize_spectral (moraine/co.py:466-477): (1 - beta) coh + beta I for a
+// stack of n x n complex matrices, beta one value or one per matrix; computed
+// in FP64 (product, then sum: no contraction, as NumPy rounds), rounded to
+// complex64.
+#pragma clang fp contract(off)
+__global__ __launch_bounds__(256) void regularize_spectral_kernel(const float2* __restrict__ c, long long nmat, int n,
+                                                                  const double* __restrict__ beta, int per_matrix,
+                                                                  float2* __restrict__ out) {
+  const long long nn = (long long)n * n, tot = nmat * nn;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (long long)gridDim.x * blockDim.x) {
+    const long long m = t / nn;
+    const int e = (int)(t - m * nn);
+    const double b = beta[per_matrix ? m : 0];
+    const float2 v = c[t];
+    double re = (1.0 - b) * (double)v.x, im = (1.0 - b) * (double)v.y;
+    if (e / n == e % n) re += b;
+    out[t] = make_float2((float)re, (float)im);
+  }
+}
+#pragma clang fp contract(on)
+
 }  // namespace mr
 
 extern "C" {
@@ -425,6 +446,18 @@ int mr_ad_intf_pc(const float* ref_rslc, const float* sec_rslc, int nl, int w, c
                      reinterpret_cast<const float2*>(ref_rslc), reinterpret_cast<const float2*>(sec_rslc), nl, w, idx,
                      reinterpret_cast<const bool*>(pc_is_shp), n_pc, aw, rw, reinterpret_cast<float2*>(intf));
   return mr::launch_status("ad_intf_pc");
+}
+
+int mr_regularize_spectral(const float* coh, long long n_mat, int n, const double* beta, int beta_per_matrix,
+                           float* out, mr_stream_t stream) {
+  MR_CHECK_ARG(n_mat >= 0 && n >= 1, "regularize_spectral: bad sizes");
+  if (n_mat == 0) return MR_OK;
+  MR_CHECK_ARG(coh && beta && out, "regularize_spectral: null pointer");
+  const int nb = (int)std::min<long long>(mr::div_up((unsigned long long)n_mat * n * n, 256), 1 << 16);
+  hipLaunchKernelGGL(mr::regularize_spectral_kernel, dim3(nb), dim3(256), 0, mr::as_stream(stream),
+                     reinterpret_cast<const float2*>(coh), n_mat, n, beta, beta_per_matrix,
+                     reinterpret_cast<float2*>(out));
+  return mr::launch_status("regularize_spectral");
 }
 
 }  // extern "C"

@@ -93,6 +93,14 @@ def ad_intf_pc(ref_rslc, sec_rslc, idx, pc_is_shp) -> np.ndarray:
     return out
 
 
+def regularize_spectral(coh, beta) -> np.ndarray:
+    """Restates ``regularize_spectral`` (moraine/co.py:466-477) in complex128:
+    (1 - beta) coh + beta I, beta broadcast over the stack's leading shape."""
+    c = np.asarray(coh).astype(np.complex128)
+    b = np.asarray(beta, np.float64)[..., None, None]
+    return (1 - b) * c + b * np.eye(c.shape[-1])
+
+
 def uncompress_coh(coh: np.ndarray, image_pairs: np.ndarray | None = None) -> np.ndarray:
     """Compressed (..., P) -> Hermitian (..., N, N), diag = 1.
 

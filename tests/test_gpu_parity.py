@@ -275,6 +275,21 @@ def test_ad_intf_pc_gpu():
     np.testing.assert_allclose(got[:-1], want[:-1], rtol=0, atol=1e-6)
 
 
+def test_regularize_spectral_gpu():
+    """Scalar and per-matrix beta vs the oracle (FP64 arithmetic, complex64 out:
+    one float32 rounding)."""
+    mr = _mr()
+    rng = np.random.default_rng(6)
+    c = (rng.standard_normal((7, 3, 60, 60)) + 1j * rng.standard_normal((7, 3, 60, 60))).astype(np.complex64)
+    b = rng.random((7, 3))
+    _case("regularize_spectral 21 x 60 x 60")
+    for beta in (0.25, b):
+        got = mr.regularize_spectral(c, beta)
+        want = R.regularize_spectral(c, beta).astype(np.complex64)
+        assert got.dtype == np.complex64 and got.shape == c.shape
+        np.testing.assert_array_equal(got, want)
+
+
 def test_uncompress_gpu():
     mr = _mr()
     c = (np.random.default_rng(1).random((3, 4, 45)) + 1j).astype(np.complex64)

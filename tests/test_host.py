@@ -99,6 +99,7 @@ def test_api_signatures_match_reference_names():
         "rslc2amp": ["rslc"],
         "amp_disp": ["rslc"],
         "ad_intf_pc": ["ref_rslc", "sec_rslc", "idx", "pc_is_shp", "block_size"],
+        "regularize_spectral": ["coh", "beta"],
     }
     for name, params in sig.items():
         ps = inspect.signature(getattr(mr, name)).parameters

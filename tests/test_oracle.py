@@ -349,3 +349,18 @@ def test_ad_intf_pc_closed_forms():
     # a different secondary: |intf| <= 1 (Cauchy-Schwarz), equals 1 only for proportional stacks
     sec2 = (rng.standard_normal((9, 11)) + 1j * rng.standard_normal((9, 11))).astype(np.complex64)
     assert np.all(np.abs(R.ad_intf_pc(ref, sec2, idx[:3], shp[:3])) < 1)
+
+
+def test_regularize_spectral_closed_forms():
+    """beta = 0 keeps coh, beta = 1 gives I, the diagonal of a unit-diagonal
+    coherence stays 1, per-matrix beta broadcasts over the stack."""
+    rng = np.random.default_rng(2)
+    c = (rng.standard_normal((3, 4, 5, 5)) + 1j * rng.standard_normal((3, 4, 5, 5))).astype(np.complex64)
+    np.testing.assert_array_equal(R.regularize_spectral(c, 0.0), c.astype(np.complex128))
+    np.testing.assert_array_equal(R.regularize_spectral(c, 1.0), np.broadcast_to(np.eye(5), c.shape))
+    u = c.copy()
+    u[..., range(5), range(5)] = 1
+    np.testing.assert_allclose(np.diagonal(R.regularize_spectral(u, 0.3), axis1=-2, axis2=-1), 1, atol=1e-15)
+    b = rng.random((3, 4))
+    r = R.regularize_spectral(c, b)
+    np.testing.assert_allclose(r[1, 2], (1 - b[1, 2]) * c[1, 2] + b[1, 2] * np.eye(5), atol=1e-12)
