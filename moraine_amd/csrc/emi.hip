@@ -189,7 +189,7 @@ __global__ __launch_bounds__(EMI_FAST_THREADS) void emi_fast_kernel(const float2
 // ----------------------------------------------------------------------------
 constexpr int EMI_GEN_THREADS = 256;
 constexpr int EMI_GEN_BLOCKS = 1024;
-constexpr int EMI_REF_BLOCKS = 256;  // FP64 re-solve blocks for n > 64 (each with a global slice)
+constexpr int EMI_REF_BLOCKS = 1024;  // FP64 re-solve blocks for n > 64, four per CU (each with a global slice)
 
 __host__ __device__ inline size_t emi_gen_slice_bytes(int n) {
   // A (n*n f64) + M (n*n c128) + vectors (col, u, q, p: n c128 each) + 12 n-double arrays
