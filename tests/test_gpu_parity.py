@@ -290,6 +290,21 @@ def test_regularize_spectral_gpu():
         np.testing.assert_array_equal(got, want)
 
 
+def test_neighbours_empty_inputs_gpu():
+    """Empty stacks, point lists and matrix stacks give empty outputs of the right shape and dtype."""
+    mr = _mr()
+    _case("neighbour functions on empty inputs")
+    a = mr.rslc2amp(np.zeros((0, 4, 6), np.complex64))
+    assert a.shape == (0, 4, 6) and a.dtype == np.float32
+    d = mr.amp_disp(np.zeros((0, 5, 6), np.complex64))
+    assert d.shape == (0, 5) and d.dtype == np.float32
+    z = synth.ds_stack(6, 7, 5, seed=1)
+    i = mr.ad_intf_pc(z[..., 0], z[..., 1], np.zeros((0, 2), np.int32), np.zeros((0, 3, 3), bool))
+    assert i.shape == (0,) and i.dtype == np.complex64
+    r = mr.regularize_spectral(np.zeros((0, 4, 4), np.complex64), 0.5)
+    assert r.shape == (0, 4, 4)
+
+
 def test_uncompress_gpu():
     mr = _mr()
     c = (np.random.default_rng(1).random((3, 4, 45)) + 1j).astype(np.complex64)
