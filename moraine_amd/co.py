@@ -41,6 +41,19 @@ def _pairs(image_pairs, n):
     return np.ascontiguousarray(np.asarray(image_pairs, dtype=np.int32))
 
 
+_FULL_PAIRS: dict = {}
+
+
+def _full_pairs_dev(n: int, device) -> torch.Tensor:
+    """The full pair network of n images in triu order, uploaded once per (n, device):
+    an upload from pageable memory per call would wait for the stream and stop the
+    host from queueing ahead of the GPU."""
+    key = (n, str(device))
+    if key not in _FULL_PAIRS:
+        _FULL_PAIRS[key] = _dev.to_dev(_pairs(None, n)).to(device)
+    return _FULL_PAIRS[key]
+
+
 def emperical_co_pc(rslc, idx, pc_is_shp, block_size: int = 128, image_pairs=None):
     """Coherence of the selected image pairs for a point cloud: (n_pc, n_pairs) complex64."""
     as_torch = _dev.is_torch(rslc)
@@ -49,12 +62,11 @@ def emperical_co_pc(rslc, idx, pc_is_shp, block_size: int = 128, image_pairs=Non
     ix = _dev.to_dev(idx, torch.int32)
     m = _dev.to_dev(pc_is_shp, torch.bool)
     aw, rw = m.shape[-2:]
-    pairs_np = _pairs(image_pairs, n)
-    pairs = _dev.to_dev(pairs_np)
+    pairs = _full_pairs_dev(n, z.device) if image_pairs is None else _dev.to_dev(_pairs(image_pairs, n))
     n_pc = ix.shape[0]
-    coh = torch.empty((n_pc, pairs_np.shape[0]), dtype=torch.complex64, device=z.device)
+    coh = torch.empty((n_pc, pairs.shape[0]), dtype=torch.complex64, device=z.device)
     _lib.call("mr_emperical_co_pc", _dev.ptr(z), nl, w, n, _dev.ptr(ix), _dev.ptr(m), n_pc, aw, rw, _dev.ptr(pairs),
-              pairs_np.shape[0], _dev.ptr(coh), _dev.stream())
+              pairs.shape[0], _dev.ptr(coh), _dev.stream())
     return _dev.back(coh, as_torch)
 
 

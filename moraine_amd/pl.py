@@ -10,7 +10,7 @@ import numpy as np
 import torch
 
 from . import _dev, _lib
-from .co import _pairs, emperical_co_pc
+from .co import _full_pairs_dev, _pairs, emperical_co_pc
 from .tnet import nimage_from_npair
 
 
@@ -96,18 +96,6 @@ def _tcoh(kind: str, a, b, image_pairs):
 def ds_temp_coh(coh, ph, image_pairs=None, block_size: int = 128):
     """|sum_p coh_p/|coh_p| conj(ph_a) ph_b| / n_pairs per point (float32)."""
     return _tcoh("mr_ds_temp_coh", coh, ph, image_pairs)
-
-
-_FULL_PAIRS: dict = {}
-
-
-def _full_pairs_dev(n: int, device) -> torch.Tensor:
-    """The full pair network of n images in triu order, uploaded once per (n, device)
-    (a pageable upload per batch would block the host on the stream)."""
-    key = (n, str(device))
-    if key not in _FULL_PAIRS:
-        _FULL_PAIRS[key] = _dev.to_dev(_pairs(None, n)).to(device)
-    return _FULL_PAIRS[key]
 
 
 def _emi_tcoh_dev(c: torch.Tensor, n: int, ref: int, ph: torch.Tensor, q: torch.Tensor, t: torch.Tensor, tol=None):

@@ -100,7 +100,6 @@ def ds_pipeline_streamed(rslc, az_half_win: int, r_half_win: int, p_max: float =
     for k in range(len(bands)):
         s = k & 1
         comp.wait_event(ready)
-        nxt = stage(k + 1) if k + 1 < len(bands) else None  # overlaps the compute below
         h0, h1 = rows_of(k)
         b0, b1 = bands[k]
         zb = dev_in[s][: h1 - h0]
@@ -119,6 +118,9 @@ def ds_pipeline_streamed(rslc, az_half_win: int, r_half_win: int, p_max: float =
             _emi_tcoh_dev(coh, n, 0, ph_b[p0:p1], qtc[0, p0:p1], qtc[1, p0:p1])
         qtc[2].copy_(c[i0:i0 + m].reshape(-1).view(torch.float32))
         done_in[s].record(comp)  # this slot's input may be overwritten from here on
+        # the next band's host copy and upload, and the host copy of band k-2's results,
+        # run while the GPU computes band k (all of band k is enqueued above)
+        nxt = stage(k + 1) if k + 1 < len(bands) else None
         drain(s)  # the pinned output slot s held band k-2
         pin_ph[s][:m].view(-1, n).copy_(ph_b, non_blocking=True)
         pin_qtc[s][:, :m].view(3, -1).copy_(qtc, non_blocking=True)
