@@ -191,7 +191,7 @@ __global__ __launch_bounds__(CO_THREADS) void co_kernel(const float2* __restrict
 // bands, consecutive lanes on consecutive entries, gathering coh through the
 // cache.  One band covers N <= 128.
 constexpr int UNC_LUT_ENTRIES = 16384;  // 64 KiB of codes per block
-constexpr int UNC_STAGE = 4096;         // rows of up to 32 KiB are staged in LDS
+constexpr int UNC_STAGE = 8192;         // rows of up to 64 KiB are staged in LDS
 constexpr int UNC_STAGE_GROUP = 2048;   // G points' rows per stage: G = max(1, this / n_pairs)
 constexpr int UNC_THREADS = 256;
 
