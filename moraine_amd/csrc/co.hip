@@ -25,6 +25,18 @@ constexpr int CO_QB = 8;             // pairs per thread per batch
 constexpr int CO_MAX_WIN = 4096;     // window entries
 constexpr int CO_STAGE_BYTES = 32768;
 
+// Entry p of the row-major upper triangle (diagonal included) of an n x n
+// matrix: row m holds n - m entries starting at m n - m (m - 1) / 2.
+__device__ __forceinline__ void tri_index(int p, int n, int& m, int& j) {
+  const float b = 2.f * n + 1.f;
+  int r = (int)((b - sqrtf(b * b - 8.f * (float)p)) * 0.5f);
+  r = max(0, min(r, n - 1));
+  while (r > 0 && r * n - r * (r - 1) / 2 > p) --r;
+  while (r + 1 < n && (r + 1) * n - (r + 1) * r / 2 <= p) ++r;
+  m = r;
+  j = r + (p - (r * n - r * (r - 1) / 2));
+}
+
 // SAMP: batched form, point pt is pixel pix[pix0 + pt] whose K = n_samp[id]
 // stacks are the rows of its sample matrix at rslc + row_off[id] (no window).
 template <bool RASTER, bool SAMP = false>
@@ -47,7 +59,9 @@ __global__ __launch_bounds__(CO_THREADS) void co_kernel(const float2* __restrict
   const int win = aw * rw;
   const int az = (aw - 1) / 2, r = (rw - 1) / 2;
   const int kc = CO_STAGE_BYTES / (8 * n);  // stacks per LDS chunk
-  const int np = RASTER ? n * n : npairs;
+  // raster: the upper triangle m <= j (n(n+1)/2 entries) is accumulated and
+  // written with its conjugate mirror, so the output is exactly Hermitian
+  const int np = RASTER ? n * (n + 1) / 2 : npairs;
 
   for (long long pt = blockIdx.x; pt < npts; pt += gridDim.x) {
     int ia = 0, ir = 0;
@@ -108,8 +122,7 @@ __global__ __launch_bounds__(CO_THREADS) void co_kernel(const float2* __restrict
         const int p = pb + q * CO_THREADS + tid;
         if (p < np) {
           if (RASTER) {
-            pa[q] = p / n;
-            pbb[q] = p - (p / n) * n;
+            tri_index(p, n, pa[q], pbb[q]);
           } else {
             pa[q] = pairs[2 * p];
             pbb[q] = pairs[2 * p + 1];
@@ -166,9 +179,15 @@ __global__ __launch_bounds__(CO_THREADS) void co_kernel(const float2* __restrict
         const float den = sqrtf(pw[pa[q]] * pw[pbb[q]]);
         const float2 c = make_float2(accr[q] / den, acci[q] / den);
         if (RASTER) {
-          const size_t o = (size_t)pt * n * n + p;
+          const size_t o = (size_t)pt * n * n + (size_t)pa[q] * n + pbb[q];
+          const size_t o2 = (size_t)pt * n * n + (size_t)pbb[q] * n + pa[q];
+          const float2 cv = make_float2(accr[q] / (float)K, acci[q] / (float)K);
           coh[o] = c;
-          if (cov) cov[o] = make_float2(accr[q] / (float)K, acci[q] / (float)K);
+          if (cov) cov[o] = cv;
+          if (o2 != o) {
+            coh[o2] = make_float2(c.x, -c.y);
+            if (cov) cov[o2] = make_float2(cv.x, -cv.y);
+          }
         } else {
           coh[(size_t)pt * npairs + p] = c;
         }
