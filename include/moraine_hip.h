@@ -61,6 +61,23 @@ int mr_ks_test_f64(const double* rmli, int nl, int w, int n, int az, int r, doub
                    double* p_out, double* dist_out, uint8_t* mask_out, int32_t* shp_num,
                    void* workspace, size_t workspace_bytes, mr_stream_t stream);
 
+/* ks_test + select_shp with no device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, outputs and bit-exact results as mr_ks_test_f32, but the
+ * sorted-key workspace is the library's static device scratch of
+ * MR_KS_NOALLOC_SCRATCH_BYTES per GPU (allocated when the library's code is
+ * loaded on the GPU, held until the process ends): a stack needs
+ * align256(4 nl w n) + align256(nl w) bytes of it, and a larger one is
+ * rejected with MR_ERR_UNSUPPORTED before any device work.  Nothing is
+ * allocated and nothing needs clearing: every call writes the scratch before
+ * reading it.  All calls on one GPU share its scratch, so calls that overlap on
+ * one GPU (two streams, two host threads) race; serialise them. */
+#define MR_KS_NOALLOC_SCRATCH_BYTES ((size_t)128 << 20)
+int mr_ks_test_f32_noalloc(const float* rmli, int nl, int w, int n, int az, int r, double p_max,
+                           float* p_out, float* dist_out, uint8_t* mask_out, int32_t* shp_num,
+                           mr_stream_t stream);
+
 /* rslc2amp (moraine/calamp.py:33-40): amp[i] = |rslc[i]|, complex64 (n
  * interleaved re/im float pairs) -> float32, rounded as the C library's hypotf
  * (glibc: the FP64 square sum's correctly rounded sqrt, rounded to FP32),

@@ -22,6 +22,7 @@ SIGNATURES = {
     "mr_ks_workspace_bytes": (_sz, [_i, _i, _i]),
     "mr_ks_test_f32": (_i, [_vp, _i, _i, _i, _i, _i, _d, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
     "mr_ks_test_f64": (_i, [_vp, _i, _i, _i, _i, _i, _d, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "mr_ks_test_f32_noalloc": (_i, [_vp, _i, _i, _i, _i, _i, _d, _vp, _vp, _vp, _vp, _vp]),
     "mr_select_shp_f32": (_i, [_vp, _ll, _i, _d, _vp, _vp, _vp]),
     "mr_rslc2amp": (_i, [_vp, _ll, _vp, _vp]),
     "mr_amp_disp": (_i, [_vp, _i, _i, _i, _vp, _vp]),

@@ -20,6 +20,8 @@
 //     pixel itself); each result is also written, mirrored, to the partner
 //     pixel's opposite offset, and SHP counts of mirrored entries accumulate
 //     with global atomics.
+#include <mutex>
+
 #include "common.hpp"
 
 namespace mr {
@@ -420,6 +422,11 @@ __global__ __launch_bounds__(256) void select_shp_kernel(const float* __restrict
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// Workspace of ks_test_impl: sorted keys (sizeof(Key) per sample) + one NaN flag per pixel.
+static size_t ks_workspace_need(size_t npx, int n, size_t key_bytes) {
+  return align256(npx * (size_t)n * key_bytes) + align256(npx);
+}
+
 // Walks per thread for a window row of `pairs` (pixel, offset) walks per block:
 // a trip runs KS_THREADS x ilp walks and takes about ilp walk-times, so the
 // cost of a row is ceil(pairs / (KS_THREADS ilp)) * ilp; ties go to 6 (more
@@ -445,7 +452,7 @@ static int ks_test_impl(const T* rmli, int nl, int w, int n, int az, int r, doub
   }
   MR_CHECK_ARG(az >= 0 && r >= 0, "ks_test: negative half window");
   const size_t npx = (size_t)nl * w;
-  const size_t need = align256(npx * n * sizeof(Key)) + align256(npx);
+  const size_t need = ks_workspace_need(npx, n, sizeof(Key));
   MR_CHECK_ARG(ws_bytes >= need, "ks_test: workspace %zu < %zu bytes", ws_bytes, need);
   hipStream_t st = as_stream(stream_);
   Key* keys = reinterpret_cast<Key*>(ws);
@@ -506,9 +513,61 @@ static int ks_test_impl(const T* rmli, int nl, int w, int n, int az, int r, doub
   return launch_status("ks_pairs");
 }
 
+// ---------------------------------------------------------------------------
+// Library-owned scratch for mr_ks_test_f32_noalloc: the sorted-key workspace
+// of ks_test_impl in a file-scope device array, for callers that cannot
+// allocate device memory.  In host code `ks_scratch` names a host-side stand-in,
+// not the device array: its device address is looked up once per device with
+// hipGetSymbolAddress (with that device current) and only that address is ever
+// passed to kernels.  Every call on one GPU shares that GPU's copy.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char ks_scratch[MR_KS_NOALLOC_SCRATCH_BYTES];
+
+constexpr int KS_SCRATCH_MAX_DEVICES = 64;
+static std::mutex ks_scratch_mutex;
+static void* ks_scratch_dev[KS_SCRATCH_MAX_DEVICES] = {};
+
+// Device address of ks_scratch on the current device (looked up on the first call per device).
+static int ks_scratch_address(void** out) {
+  int dev = -1;
+  MR_CHECK_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= KS_SCRATCH_MAX_DEVICES) {
+    set_error("ks_test_noalloc: device %d outside 0..%d", dev, KS_SCRATCH_MAX_DEVICES - 1);
+    return MR_ERR_UNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> lock(ks_scratch_mutex);
+  if (ks_scratch_dev[dev] == nullptr) {
+    void* p = nullptr;
+    MR_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(ks_scratch)));
+    if (p == nullptr) {
+      set_error("ks_test_noalloc: hipGetSymbolAddress returned a null address on device %d", dev);
+      return MR_ERR_HIP;
+    }
+    ks_scratch_dev[dev] = p;
+  }
+  *out = ks_scratch_dev[dev];
+  return MR_OK;
+}
+
 }  // namespace mr
 
 extern "C" {
+
+int mr_ks_test_f32_noalloc(const float* rmli, int nl, int w, int n, int az, int r, double p_max, float* p_out,
+                           float* dist_out, uint8_t* mask_out, int32_t* shp_num, mr_stream_t stream) {
+  MR_CHECK_ARG(nl > 0 && w > 0 && n >= 1, "ks_test_noalloc: bad sizes (%d x %d x %d)", nl, w, n);
+  const size_t need = mr::ks_workspace_need((size_t)nl * w, n, sizeof(uint32_t));
+  if (need > (size_t)MR_KS_NOALLOC_SCRATCH_BYTES) {
+    mr::set_error("ks_test_noalloc: a %d x %d x %d stack needs %zu bytes of scratch, more than the %zu bytes "
+                  "of the library's static scratch (use mr_ks_test_f32 with a workspace)",
+                  nl, w, n, need, (size_t)MR_KS_NOALLOC_SCRATCH_BYTES);
+    return MR_ERR_UNSUPPORTED;
+  }
+  void* ws = nullptr;
+  if (int rc = mr::ks_scratch_address(&ws)) return rc;
+  return mr::ks_test_impl<float>(rmli, nl, w, n, az, r, p_max, p_out, dist_out, mask_out, shp_num, ws,
+                                 (size_t)MR_KS_NOALLOC_SCRATCH_BYTES, stream);
+}
 
 size_t mr_ks_workspace_bytes(int nl, int w, int n) {
   const size_t npx = (size_t)nl * w;

@@ -122,3 +122,11 @@ def test_refine_argument_rules():
             _check_refine(bad, 60)
     assert _check_refine(1e-3, 2) == 1e-3 and _check_refine(1e-3, 200) == 1e-3
     assert _check_refine(1e-3, 201) is None  # FP64 throughout there
+
+
+def test_noalloc_ks_rejects_oversized_stack_on_host():
+    """mr_ks_test_f32_noalloc checks the stack against its static scratch before any device work."""
+    lib = _lib.load()
+    rc = lib.mr_ks_test_f32_noalloc(None, 2048, 2048, 60, 7, 10, 0.05, None, None, None, None, None)
+    assert rc == 3
+    assert b"static scratch" in lib.mr_last_error_string()
