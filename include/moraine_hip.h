@@ -108,6 +108,11 @@ int mr_ks_tables_f32(int n, double p_max, float* p, float* dist, uint8_t* shp);
 int mr_select_shp_f32(const float* p, long long npix, int win, double p_max, uint8_t* is_shp,
                       int32_t* shp_num, mr_stream_t stream);
 
+/* select_shp on a float64 p array (moraine/shp.py:308-323): the comparison
+ * p < p_max is made in float64. */
+int mr_select_shp_f64(const double* p, long long npix, int win, double p_max, uint8_t* is_shp,
+                      int32_t* shp_num, mr_stream_t stream);
+
 /* ---------------------------------------------------------------------------
  * Coherence estimation.
  * mr_emperical_co replaces moraine.emperical_co (moraine/co.py:117-137,
@@ -128,6 +133,15 @@ int mr_emperical_co_pc(const float* rslc, int nl, int w, int n, const int32_t* i
                        const uint8_t* pc_is_shp, int n_pc, int aw, int rw, const int32_t* pairs,
                        int n_pairs, float* coh, mr_stream_t stream);
 
+/* complex128 forms of the two (moraine/co.py:204: coh has rslc's dtype):
+ * rslc, cov, coh complex128 (interleaved double pairs); FP64 accumulation
+ * on the VALU kernel; n <= 1024; other arguments as above. */
+int mr_emperical_co_c128(const double* rslc, const uint8_t* is_shp, int nl, int w, int n, int aw, int rw,
+                         int row0, int nrows, double* cov, double* coh, mr_stream_t stream);
+int mr_emperical_co_pc_c128(const double* rslc, int nl, int w, int n, const int32_t* idx,
+                            const uint8_t* pc_is_shp, int n_pc, int aw, int rw, const int32_t* pairs,
+                            int n_pairs, double* coh, mr_stream_t stream);
+
 /* ---------------------------------------------------------------------------
  * EMI phase linking.  Replaces moraine.emi (moraine/pl.py:68-77; _emi_cp
  * :18-42; _emi_numba :45-65).
@@ -143,6 +157,15 @@ size_t mr_emi_workspace_bytes(long long n_pts, int n);
 int mr_emi(const float* coh, long long n_pts, int n, int ref, float* ph, float* quality,
            void* workspace, size_t workspace_bytes, mr_stream_t stream);
 
+/* complex128 EMI (moraine/pl.py:45-65 with coh complex128: ph has coh's dtype,
+ * quality float32): every stage in FP64 (|Gamma| inverse by the sweep
+ * operator, Householder tridiagonalisation, eigenpair of T to FP64
+ * resolution), one 256-thread block per point; n >= 2.
+ * workspace >= mr_emi_c128_workspace_bytes(n_pts, n) (0 for n <= 64). */
+size_t mr_emi_c128_workspace_bytes(long long n_pts, int n);
+int mr_emi_c128(const double* coh, long long n_pts, int n, int ref, double* ph, float* quality,
+                void* workspace, size_t workspace_bytes, mr_stream_t stream);
+
 /* ad_intf_pc (moraine/co.py:302-406): adaptively multilooked interferogram of
  * one image pair per point: over the SHPs of the point's aw x rw window that
  * lie inside the nl x w raster, sum ref conj(sec) / sqrt(sum|ref|^2 sum|sec|^2)
@@ -151,12 +174,19 @@ int mr_emi(const float* coh, long long n_pts, int n, int ref, float* ph, float* 
 int mr_ad_intf_pc(const float* ref_rslc, const float* sec_rslc, int nl, int w, const int32_t* idx,
                   const uint8_t* pc_is_shp, long long n_pc, int aw, int rw, float* intf, mr_stream_t stream);
 
+int mr_ad_intf_pc_c128(const double* ref_rslc, const double* sec_rslc, int nl, int w, const int32_t* idx,
+                       const uint8_t* pc_is_shp, long long n_pc, int aw, int rw, double* intf,
+                       mr_stream_t stream);
+
 /* regularize_spectral (moraine/co.py:466-477): out = (1 - beta) coh + beta I
  * for n_mat complex64 n x n matrices; beta (device, FP64) holds one value, or
  * one per matrix when beta_per_matrix != 0; FP64 arithmetic, complex64 out
  * (out may equal coh). */
 int mr_regularize_spectral(const float* coh, long long n_mat, int n, const double* beta, int beta_per_matrix,
                            float* out, mr_stream_t stream);
+
+int mr_regularize_spectral_c128(const double* coh, long long n_mat, int n, const double* beta,
+                                int beta_per_matrix, double* out, mr_stream_t stream);
 
 /* uncompress_coh (moraine/co.py:281-299): (n_pts, n_pairs) -> (n_pts, n, n),
  * [a,b] = c, [b,a] = conj(c), diagonal 1, other entries 0, resolved in the
@@ -165,6 +195,9 @@ int mr_regularize_spectral(const float* coh, long long n_mat, int n, const doubl
  * [0, n) (others are skipped); n <= 16384, n_pairs <= 2^30. */
 int mr_uncompress_coh(const float* coh, long long n_pts, int n, const int32_t* pairs, int n_pairs,
                       float* out, mr_stream_t stream);
+
+int mr_uncompress_coh_c128(const double* coh, long long n_pts, int n, const int32_t* pairs, int n_pairs,
+                           double* out, mr_stream_t stream);
 
 /* ---------------------------------------------------------------------------
  * Temporal coherence.
@@ -178,6 +211,13 @@ int mr_ds_temp_coh(const float* coh, const float* ph, long long n_pts, int n, co
                    int n_pairs, float* t_coh, mr_stream_t stream);
 int mr_temp_coh(const float* intf, const float* rslc, long long n_pts, int n, const int32_t* pairs,
                 int n_pairs, float* t_coh, mr_stream_t stream);
+
+/* complex128 inputs (moraine/pl.py:103-123: the accumulator has the input's
+ * complex type): sums in FP64, t_coh float32; n <= 2048. */
+int mr_ds_temp_coh_c128(const double* coh, const double* ph, long long n_pts, int n, const int32_t* pairs,
+                        int n_pairs, float* t_coh, mr_stream_t stream);
+int mr_temp_coh_c128(const double* intf, const double* rslc, long long n_pts, int n, const int32_t* pairs,
+                     int n_pairs, float* t_coh, mr_stream_t stream);
 
 /* EMI followed by the DS temporal coherence of its own phases: the inner two
  * stages of moraine.emperical_co_emi_temp_coh_pc (moraine/pl.py:148-168),

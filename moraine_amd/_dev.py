@@ -34,6 +34,20 @@ def to_dev(x, dtype: torch.dtype | None = None) -> torch.Tensor:
     return t.contiguous()
 
 
+def to_dev_complex(x) -> torch.Tensor:
+    """Contiguous complex device tensor of ``x``, keeping the reference's dtype
+    rule (outputs have the input's precision): complex128 (or float64) input
+    becomes complex128 and runs the FP64 kernels; anything else becomes
+    complex64 (float32/complex64 exactly)."""
+    t = x if is_torch(x) else torch.from_numpy(np.ascontiguousarray(x))
+    dt = torch.complex128 if t.dtype in (torch.complex128, torch.float64) else torch.complex64
+    return to_dev(t, dt)
+
+
+def is_c128(t: torch.Tensor) -> bool:
+    return t.dtype == torch.complex128
+
+
 def ptr(t: torch.Tensor | None):
     return None if t is None else ctypes.c_void_p(t.data_ptr())
 

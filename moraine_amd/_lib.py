@@ -40,6 +40,16 @@ SIGNATURES = {
     "mr_temp_coh": (_i, [_vp, _vp, _ll, _i, _vp, _i, _vp, _vp]),
     "mr_emi_refined_workspace_bytes": (_sz, [_ll, _i]),
     "mr_emi_refined": (_i, [_vp, _ll, _i, _i, _d, _vp, _i, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "mr_select_shp_f64": (_i, [_vp, _ll, _i, _d, _vp, _vp, _vp]),
+    "mr_emperical_co_c128": (_i, [_vp, _vp, _i, _i, _i, _i, _i, _i, _i, _vp, _vp, _vp]),
+    "mr_emperical_co_pc_c128": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, _i, _i, _vp, _i, _vp, _vp]),
+    "mr_uncompress_coh_c128": (_i, [_vp, _ll, _i, _vp, _i, _vp, _vp]),
+    "mr_emi_c128_workspace_bytes": (_sz, [_ll, _i]),
+    "mr_emi_c128": (_i, [_vp, _ll, _i, _i, _vp, _vp, _vp, _sz, _vp]),
+    "mr_ad_intf_pc_c128": (_i, [_vp, _vp, _i, _i, _vp, _vp, _ll, _i, _i, _vp, _vp]),
+    "mr_regularize_spectral_c128": (_i, [_vp, _ll, _i, _vp, _i, _vp, _vp]),
+    "mr_ds_temp_coh_c128": (_i, [_vp, _vp, _ll, _i, _vp, _i, _vp, _vp]),
+    "mr_temp_coh_c128": (_i, [_vp, _vp, _ll, _i, _vp, _i, _vp, _vp]),
     "mr_ds_batch_workspace_bytes": (_sz, [_ll, _i]),
     "mr_ds_batch": (_i, [_vp, _vp, _vp, _vp, _vp, _ll, _i, _i, _vp, _i, _vp, _vp, _vp, _i, _ll, _vp, _sz, _vp]),
 }

@@ -184,11 +184,6 @@ class DSBatchRunner:
             if n < 2 or n > 1024:
                 raise ValueError(f"ds_batch: N={n} outside [2, 1024]")
             ids = np.nonzero(batch.n_images == n)[0].astype(np.int32)
-            if n > 64 or n <= 32 or per_stage:
-                kmax = int(batch.n_samples[torch.from_numpy(ids).to(dev).long()].max()) if ids.size else 0
-                if kmax > 4096:
-                    raise ValueError(f"ds_batch: N={n} pixels hold up to {kmax} sample rows; at most 4096 "
-                                     "on the N > 64 / per-stage path")
             pairs = _dev.to_dev(_pairs(None, n))
             self.groups.append(_Group(n, torch.from_numpy(ids).to(dev), ids.size, pairs))
         self.max_ws = int(max_workspace_bytes)

@@ -17,28 +17,37 @@ def emperical_co(rslc, is_shp, block_size: int = 128):
     """Full-raster (cov, coh), each (nlines, width, nimages, nimages) complex64.
 
     cov = sum/n_shp, coh = sum/sqrt(P_m P_j) over the in-bounds SHPs of each
-    pixel's window; half windows are (az_win-1)//2, (r_win-1)//2.
+    pixel's window; half windows are (az_win-1)//2, (r_win-1)//2.  The outputs
+    have rslc's dtype (moraine/co.py:132-133): complex128 input runs in FP64.
     """
     as_torch = _dev.is_torch(rslc)
-    z = _dev.to_dev(rslc, torch.complex64)
+    z = _dev.to_dev_complex(rslc)
     m = _dev.to_dev(is_shp, torch.bool)
     nl, w, n = z.shape
     aw, rw = m.shape[-2:]
     if tuple(m.shape[:2]) != (nl, w):
         raise ValueError("emperical_co: is_shp must be (nlines, width, az_win, r_win)")
-    cov = torch.empty((nl, w, n, n), dtype=torch.complex64, device=z.device)
+    cov = torch.empty((nl, w, n, n), dtype=z.dtype, device=z.device)
     coh = torch.empty_like(cov)
-    _lib.call("mr_emperical_co", _dev.ptr(z), _dev.ptr(m), nl, w, n, aw, rw, 0, nl, _dev.ptr(cov), _dev.ptr(coh),
-              _dev.stream())
+    fn = "mr_emperical_co_c128" if _dev.is_c128(z) else "mr_emperical_co"
+    _lib.call(fn, _dev.ptr(z), _dev.ptr(m), nl, w, n, aw, rw, 0, nl, _dev.ptr(cov), _dev.ptr(coh), _dev.stream())
     return _dev.back(cov, as_torch), _dev.back(coh, as_torch)
 
 
 def _pairs(image_pairs, n):
+    """(P, 2) int32 pair array; indices follow NumPy's rules for an axis of n
+    images: negative ones count from the end, others outside [-n, n) raise."""
     if image_pairs is None:
         image_pairs = TempNet.from_bandwidth(n).image_pairs
     if _dev.is_torch(image_pairs):
         image_pairs = image_pairs.cpu().numpy()
-    return np.ascontiguousarray(np.asarray(image_pairs, dtype=np.int32))
+    pr = np.asarray(image_pairs)
+    if pr.size and (pr.ndim != 2 or pr.shape[1] != 2):
+        raise ValueError(f"image_pairs must be (n_pairs, 2), got shape {pr.shape}")
+    pr = pr.reshape(-1, 2).astype(np.int64)
+    if np.any(pr < -n) or np.any(pr >= n):
+        raise IndexError(f"image pair index out of range for {n} images")
+    return np.ascontiguousarray(np.where(pr < 0, pr + n, pr).astype(np.int32))
 
 
 _FULL_PAIRS: dict = {}
@@ -55,27 +64,32 @@ def _full_pairs_dev(n: int, device) -> torch.Tensor:
 
 
 def emperical_co_pc(rslc, idx, pc_is_shp, block_size: int = 128, image_pairs=None):
-    """Coherence of the selected image pairs for a point cloud: (n_pc, n_pairs) complex64."""
+    """Coherence of the selected image pairs for a point cloud: (n_pc, n_pairs),
+    rslc's dtype (complex128 input runs in FP64)."""
     as_torch = _dev.is_torch(rslc)
-    z = _dev.to_dev(rslc, torch.complex64)
+    z = _dev.to_dev_complex(rslc)
     nl, w, n = z.shape
     ix = _dev.to_dev(idx, torch.int32)
     m = _dev.to_dev(pc_is_shp, torch.bool)
     aw, rw = m.shape[-2:]
     pairs = _full_pairs_dev(n, z.device) if image_pairs is None else _dev.to_dev(_pairs(image_pairs, n))
     n_pc = ix.shape[0]
-    coh = torch.empty((n_pc, pairs.shape[0]), dtype=torch.complex64, device=z.device)
-    _lib.call("mr_emperical_co_pc", _dev.ptr(z), nl, w, n, _dev.ptr(ix), _dev.ptr(m), n_pc, aw, rw, _dev.ptr(pairs),
+    coh = torch.empty((n_pc, pairs.shape[0]), dtype=z.dtype, device=z.device)
+    fn = "mr_emperical_co_pc_c128" if _dev.is_c128(z) else "mr_emperical_co_pc"
+    _lib.call(fn, _dev.ptr(z), nl, w, n, _dev.ptr(ix), _dev.ptr(m), n_pc, aw, rw, _dev.ptr(pairs),
               pairs.shape[0], _dev.ptr(coh), _dev.stream())
     return _dev.back(coh, as_torch)
 
 
 def ad_intf_pc(ref_rslc, sec_rslc, idx, pc_is_shp, block_size: int = 128):
     """Adaptively multilooked interferogram of one image pair per point,
-    (n_pc,) complex64 (moraine/co.py:376-406); ``block_size`` is unused."""
+    (n_pc,) complex64, or complex128 for complex128 input (moraine/co.py:376-406);
+    ``block_size`` is unused."""
     as_torch = _dev.is_torch(ref_rslc)
-    a = _dev.to_dev(ref_rslc, torch.complex64).contiguous()
-    b = _dev.to_dev(sec_rslc, torch.complex64).contiguous()
+    a = _dev.to_dev_complex(ref_rslc)
+    b = _dev.to_dev_complex(sec_rslc)
+    if a.dtype != b.dtype:
+        a, b = a.to(torch.complex128), b.to(torch.complex128)
     if a.ndim != 2 or a.shape != b.shape:
         raise ValueError("ad_intf_pc: ref_rslc and sec_rslc must be (nlines, width) of one shape")
     nl, w = a.shape
@@ -83,18 +97,18 @@ def ad_intf_pc(ref_rslc, sec_rslc, idx, pc_is_shp, block_size: int = 128):
     m = _dev.to_dev(pc_is_shp, torch.bool).contiguous()
     aw, rw = m.shape[-2:]
     n_pc = ix.shape[0]
-    out = torch.empty(n_pc, dtype=torch.complex64, device=a.device)
-    _lib.call("mr_ad_intf_pc", _dev.ptr(a), _dev.ptr(b), nl, w, _dev.ptr(ix), _dev.ptr(m), n_pc, aw, rw,
+    out = torch.empty(n_pc, dtype=a.dtype, device=a.device)
+    _lib.call("mr_ad_intf_pc_c128" if _dev.is_c128(a) else "mr_ad_intf_pc", _dev.ptr(a), _dev.ptr(b), nl, w, _dev.ptr(ix), _dev.ptr(m), n_pc, aw, rw,
               _dev.ptr(out), _dev.stream())
     return _dev.back(out, as_torch)
 
 
 def regularize_spectral(coh, beta):
     """(1 - beta) coh + beta I for a stack (..., N, N); beta a number or an array
-    of the stack's leading shape.  complex64 out (the reference's dtype follows
-    its array library's promotion rules)."""
+    of the stack's leading shape.  FP64 arithmetic; the output has coh's
+    precision (complex64, or complex128 for complex128 input)."""
     as_torch = _dev.is_torch(coh)
-    c = _dev.to_dev(coh, torch.complex64).contiguous()
+    c = _dev.to_dev_complex(coh)
     n = c.shape[-1]
     if c.ndim < 2 or c.shape[-2] != n:
         raise ValueError("regularize_spectral: coh must be (..., N, N)")
@@ -107,14 +121,14 @@ def regularize_spectral(coh, beta):
         b, per = torch.broadcast_to(b.reshape(b.shape), lead).reshape(-1), 1
     b = b.to(c.device).contiguous()
     out = torch.empty_like(c)
-    _lib.call("mr_regularize_spectral", _dev.ptr(c), nmat, n, _dev.ptr(b), per, _dev.ptr(out), _dev.stream())
+    _lib.call("mr_regularize_spectral_c128" if _dev.is_c128(c) else "mr_regularize_spectral", _dev.ptr(c), nmat, n, _dev.ptr(b), per, _dev.ptr(out), _dev.stream())
     return _dev.back(out, as_torch)
 
 
 def uncompress_coh(coh, image_pairs=None):
-    """(..., n_pairs) compressed -> (..., n, n) Hermitian with unit diagonal."""
+    """(..., n_pairs) compressed -> (..., n, n) Hermitian with unit diagonal, coh's dtype."""
     as_torch = _dev.is_torch(coh)
-    c = _dev.to_dev(coh, torch.complex64)
+    c = _dev.to_dev_complex(coh)
     if image_pairs is None:
         n = nimage_from_npair(c.shape[-1])
         pairs_np = TempNet.from_bandwidth(n).image_pairs
@@ -133,8 +147,8 @@ def uncompress_coh(coh, image_pairs=None):
     c = c.contiguous()
     lead = tuple(c.shape[:-1])
     npts = int(np.prod(lead)) if lead else 1
-    out = torch.empty(lead + (n, n), dtype=torch.complex64, device=c.device)
+    out = torch.empty(lead + (n, n), dtype=c.dtype, device=c.device)
     pairs = _dev.to_dev(np.ascontiguousarray(pairs_np))
-    _lib.call("mr_uncompress_coh", _dev.ptr(c), npts, n, _dev.ptr(pairs), pairs_np.shape[0], _dev.ptr(out),
+    _lib.call("mr_uncompress_coh_c128" if _dev.is_c128(c) else "mr_uncompress_coh", _dev.ptr(c), npts, n, _dev.ptr(pairs), pairs_np.shape[0], _dev.ptr(out),
               _dev.stream())
     return _dev.back(out, as_torch)

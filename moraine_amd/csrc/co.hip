@@ -39,26 +39,29 @@ __device__ __forceinline__ void tri_index(int p, int n, int& m, int& j) {
 
 // SAMP: batched form, point pt is pixel pix[pix0 + pt] whose K = n_samp[id]
 // stacks are the rows of its sample matrix at rslc + row_off[id] (no window).
-template <bool RASTER, bool SAMP = false>
-__global__ __launch_bounds__(CO_THREADS) void co_kernel(const float2* __restrict__ rslc, int nl, int w, int n,
+// Z: float2 (complex64 stack, FP32 accumulators) or double2 (complex128 stack,
+// FP64 accumulators and outputs, the reference's dtype rule coh.dtype = rslc.dtype).
+template <bool RASTER, bool SAMP = false, typename Z = float2>
+__global__ __launch_bounds__(CO_THREADS) void co_kernel(const Z* __restrict__ rslc, int nl, int w, int n,
                                                         const int32_t* __restrict__ idx,
                                                         const uint8_t* __restrict__ shp, long long npts, int aw,
                                                         int rw, long long pix0, const int32_t* __restrict__ pairs,
-                                                        int npairs, float2* __restrict__ cov,
-                                                        float2* __restrict__ coh,
+                                                        int npairs, Z* __restrict__ cov,
+                                                        Z* __restrict__ coh,
                                                         const long long* __restrict__ row_off = nullptr,
                                                         const int32_t* __restrict__ n_samp = nullptr,
                                                         const int32_t* __restrict__ pix = nullptr) {
   __shared__ int list[CO_MAX_WIN];
   __shared__ int wtot[CO_THREADS / 64];
   __shared__ int s_k;
+  using S = typename ZT<Z>::S;
   extern __shared__ __align__(16) unsigned char smem[];
-  float2* stage = reinterpret_cast<float2*>(smem);
-  float* pw = reinterpret_cast<float*>(smem + CO_STAGE_BYTES);
+  Z* stage = reinterpret_cast<Z*>(smem);
+  S* pw = reinterpret_cast<S*>(smem + CO_STAGE_BYTES);
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int win = aw * rw;
   const int az = (aw - 1) / 2, r = (rw - 1) / 2;
-  const int kc = CO_STAGE_BYTES / (8 * n);  // stacks per LDS chunk
+  const int kc = CO_STAGE_BYTES / ((int)sizeof(Z) * n);  // stacks per LDS chunk
   // raster: the upper triangle m <= j (n(n+1)/2 entries) is accumulated and
   // written with its conjugate mirror, so the output is exactly Hermitian
   const int np = RASTER ? n * (n + 1) / 2 : npairs;
@@ -66,11 +69,11 @@ __global__ __launch_bounds__(CO_THREADS) void co_kernel(const float2* __restrict
   for (long long pt = blockIdx.x; pt < npts; pt += gridDim.x) {
     int ia = 0, ir = 0;
     const uint8_t* ms = nullptr;
-    const float2* base = rslc;
+    const Z* base = rslc;
     if (SAMP) {
       const int id = pix[pix0 + pt];
       base = rslc + row_off[id];
-      if (tid == 0) s_k = min((int)n_samp[id], CO_MAX_WIN);
+      if (tid == 0) s_k = (int)n_samp[id];  // rows are read in place: no SHP list, no limit on K
     } else if (RASTER) {
       const long long px = pix0 + pt;
       ia = (int)(px / w);
@@ -111,14 +114,14 @@ __global__ __launch_bounds__(CO_THREADS) void co_kernel(const float2* __restrict
     }
     const int K = s_k;
     const int nchunk = (K + kc - 1) / kc;
-    float my_pw[4] = {0.f, 0.f, 0.f, 0.f};  // power sums for images tid + 256*q
+    S my_pw[4] = {0, 0, 0, 0};  // power sums for images tid + 256*q
     for (int pb = 0; pb < np; pb += CO_THREADS * CO_QB) {
-      float accr[CO_QB], acci[CO_QB];
+      S accr[CO_QB], acci[CO_QB];
       int pa[CO_QB], pbb[CO_QB];
 #pragma unroll
       for (int q = 0; q < CO_QB; ++q) {
-        accr[q] = 0.f;
-        acci[q] = 0.f;
+        accr[q] = 0;
+        acci[q] = 0;
         const int p = pb + q * CO_THREADS + tid;
         if (p < np) {
           if (RASTER) {
@@ -147,19 +150,19 @@ __global__ __launch_bounds__(CO_THREADS) void co_kernel(const float2* __restrict
               const int m = tid + q * CO_THREADS;
               if (m < n)
                 for (int s = 0; s < kn; ++s) {
-                  const float2 z = stage[s * n + m];
+                  const Z z = stage[s * n + m];
                   my_pw[q] += z.x * z.x + z.y * z.y;
                 }
             }
           }
         }
         for (int s = 0; s < kn; ++s) {
-          const float2* zs = stage + s * n;
+          const Z* zs = stage + s * n;
 #pragma unroll
           for (int q = 0; q < CO_QB; ++q) {
-            const float2 x = zs[pa[q]], y = zs[pbb[q]];
-            accr[q] = fmaf(x.x, y.x, fmaf(x.y, y.y, accr[q]));
-            acci[q] = fmaf(x.y, y.x, fmaf(-x.x, y.y, acci[q]));
+            const Z x = zs[pa[q]], y = zs[pbb[q]];
+            accr[q] = fma(x.x, y.x, fma(x.y, y.y, accr[q]));
+            acci[q] = fma(x.y, y.x, fma(-x.x, y.y, acci[q]));
           }
         }
       }
@@ -176,17 +179,17 @@ __global__ __launch_bounds__(CO_THREADS) void co_kernel(const float2* __restrict
       for (int q = 0; q < CO_QB; ++q) {
         const int p = pb + q * CO_THREADS + tid;
         if (p >= np) continue;
-        const float den = sqrtf(pw[pa[q]] * pw[pbb[q]]);
-        const float2 c = make_float2(accr[q] / den, acci[q] / den);
+        const S den = sqrt(pw[pa[q]] * pw[pbb[q]]);
+        const Z c = mkz<Z>(accr[q] / den, acci[q] / den);
         if (RASTER) {
           const size_t o = (size_t)pt * n * n + (size_t)pa[q] * n + pbb[q];
           const size_t o2 = (size_t)pt * n * n + (size_t)pbb[q] * n + pa[q];
-          const float2 cv = make_float2(accr[q] / (float)K, acci[q] / (float)K);
+          const Z cv = mkz<Z>(accr[q] / (S)K, acci[q] / (S)K);
           coh[o] = c;
           if (cov) cov[o] = cv;
           if (o2 != o) {
-            coh[o2] = make_float2(c.x, -c.y);
-            if (cov) cov[o2] = make_float2(cv.x, -cv.y);
+            coh[o2] = mkz<Z>(c.x, -c.y);
+            if (cov) cov[o2] = mkz<Z>(cv.x, -cv.y);
           }
         } else {
           coh[(size_t)pt * npairs + p] = c;
@@ -218,11 +221,11 @@ constexpr int UNC_THREADS = 256;
 // and gathered from there; otherwise the gathers go through the cache, where
 // the lower triangle's reads (a column of the upper triangle) touch one cache
 // line per lane
-template <bool STAGE>
-__global__ __launch_bounds__(UNC_THREADS) void uncompress_kernel(const float2* __restrict__ coh, long long npts, int n,
+template <bool STAGE, typename Z = float2>
+__global__ __launch_bounds__(UNC_THREADS) void uncompress_kernel(const Z* __restrict__ coh, long long npts, int n,
                                                                  const int32_t* __restrict__ pairs, int npairs,
                                                                  int band_rows, long long pts_per_block, int G,
-                                                                 float2* __restrict__ out) {
+                                                                 Z* __restrict__ out) {
   extern __shared__ unsigned unc_lut[];
   const int row0 = blockIdx.y * band_rows;
   const int rows = min(band_rows, n - row0);
@@ -238,21 +241,22 @@ __global__ __launch_bounds__(UNC_THREADS) void uncompress_kernel(const float2* _
   __syncthreads();
   for (int i = threadIdx.x; i < rows; i += UNC_THREADS) unc_lut[i * n + row0 + i] = 1u;
   __syncthreads();
-  float2* stage = reinterpret_cast<float2*>(unc_lut + ((ne + 1) & ~1));
+  constexpr int al = (int)sizeof(Z) / 4;  // stage aligned to its element
+  Z* stage = reinterpret_cast<Z*>(unc_lut + ((ne + al - 1) & ~(al - 1)));
   const long long p0 = (long long)blockIdx.x * pts_per_block;
   const long long p1 = min(npts, p0 + pts_per_block);
   const long long nn = (long long)n * n;
   for (long long g0 = p0; g0 < p1; g0 += STAGE ? G : p1 - p0) {
     const int gp = (int)min((long long)(STAGE ? G : p1 - p0), p1 - g0);
-    const float2* cb = coh + g0 * npairs;
+    const Z* cb = coh + g0 * npairs;
     if (STAGE) {
       const int m = gp * npairs;
       for (int t0 = threadIdx.x; t0 < m; t0 += 8 * UNC_THREADS) {
-        float2 v[8];
+        Z v[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
           const int t = t0 + u * UNC_THREADS;
-          v[u] = t < m ? cb[t] : make_float2(0.f, 0.f);
+          v[u] = t < m ? cb[t] : mkz<Z>(0, 0);
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
@@ -264,15 +268,15 @@ __global__ __launch_bounds__(UNC_THREADS) void uncompress_kernel(const float2* _
     }
     // the host keeps pts_per_block * ne below 2^31
     const unsigned tot = (unsigned)gp * (unsigned)ne;
-    float2* ob = out + g0 * nn + (long long)row0 * n;
+    Z* ob = out + g0 * nn + (long long)row0 * n;
     constexpr int U = 4;
     for (unsigned t0 = threadIdx.x; t0 < tot; t0 += U * UNC_THREADS) {
-      float2 v[U];
+      Z v[U];
       long long o[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         const unsigned t = t0 + u * UNC_THREADS;
-        v[u] = make_float2(0.f, 0.f);
+        v[u] = mkz<Z>(0, 0);
         o[u] = -1;
         if (t < tot) {
           const unsigned q = t / (unsigned)ne;
@@ -280,12 +284,12 @@ __global__ __launch_bounds__(UNC_THREADS) void uncompress_kernel(const float2* _
           o[u] = (long long)q * nn + e;
           const unsigned code = unc_lut[e];
           if (code == 1u) {
-            v[u].x = 1.f;
+            v[u].x = 1;
           } else if (code >= 2u) {
             const bool cj = code >= 2u + (unsigned)npairs;
             const int pi = (int)q * npairs + (int)(code - 2u - (cj ? (unsigned)npairs : 0u));
-            const float2 c = STAGE ? stage[pi] : cb[pi];
-            v[u] = make_float2(c.x, cj ? -c.y : c.y);
+            const Z c = STAGE ? stage[pi] : cb[pi];
+            v[u] = mkz<Z>(c.x, cj ? -c.y : c.y);
           }
         }
       }
@@ -302,7 +306,7 @@ int co_samples_launch(const float2* samples, const long long* row_off, const int
                       float2* coh, hipStream_t st) {
   const int nb = (int)std::min<long long>(cn, 1 << 16);
   const size_t lds = CO_STAGE_BYTES + (size_t)n * 4;
-  hipLaunchKernelGGL((co_kernel<false, true>), dim3(nb), dim3(CO_THREADS), lds, st, samples, 0, 0, n, nullptr,
+  hipLaunchKernelGGL((co_kernel<false, true, float2>), dim3(nb), dim3(CO_THREADS), lds, st, samples, 0, 0, n, nullptr,
                      nullptr, cn, 1, 1, pix0, pairs, npairs, nullptr, coh, row_off, n_samp, pix);
   return launch_status("ds_batch coherence");
 }
@@ -324,10 +328,12 @@ static int co_check(int n, int aw, int rw) {
 // give sum ref conj(sec) / sqrt(sum |ref|^2 sum |sec|^2).  One wave per point,
 // lanes over the window, FP64 sums as the reference's CPU path (complex128 /
 // float64 accumulators); no SHP gives 0/0 = NaN as there.
-__global__ __launch_bounds__(256) void ad_intf_pc_kernel(const float2* __restrict__ ref, const float2* __restrict__ sec,
+template <typename Z>
+__global__ __launch_bounds__(256) void ad_intf_pc_kernel(const Z* __restrict__ ref, const Z* __restrict__ sec,
                                                          int nl, int w, const int32_t* __restrict__ idx,
                                                          const bool* __restrict__ shp, long long n_pc, int aw, int rw,
-                                                         float2* __restrict__ out) {
+                                                         Z* __restrict__ out) {
+  using S = typename ZT<Z>::S;
   const int lane = threadIdx.x & 63;
   const long long wave0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
@@ -339,7 +345,7 @@ __global__ __launch_bounds__(256) void ad_intf_pc_kernel(const float2* __restric
       const int k = e / rw, l = e - (e / rw) * rw;
       const int az = a0 + k, rg = r0 + l;
       if (az >= 0 && az < nl && rg >= 0 && rg < w && shp[i * win + e]) {
-        const float2 x = ref[(long long)az * w + rg], y = sec[(long long)az * w + rg];
+        const Z x = ref[(long long)az * w + rg], y = sec[(long long)az * w + rg];
         const double xr = x.x, xi = x.y, yr = y.x, yi = y.y;
         ra += xr * xr + xi * xi;
         sa += yr * yr + yi * yi;
@@ -353,7 +359,7 @@ __global__ __launch_bounds__(256) void ad_intf_pc_kernel(const float2* __restric
     sa = wave_sum(sa);
     if (lane == 0) {
       const double d = sqrt(ra * sa);
-      out[i] = make_float2((float)(nr / d), (float)(ni / d));
+      out[i] = mkz<Z>((S)(nr / d), (S)(ni / d));
     }
   }
 }
@@ -363,25 +369,113 @@ __global__ __launch_bounds__(256) void ad_intf_pc_kernel(const float2* __restric
 // in FP64 (product, then sum: no contraction, as NumPy rounds), rounded to
 // complex64.
 #pragma clang fp contract(off)
-__global__ __launch_bounds__(256) void regularize_spectral_kernel(const float2* __restrict__ c, long long nmat, int n,
+template <typename Z>
+__global__ __launch_bounds__(256) void regularize_spectral_kernel(const Z* __restrict__ c, long long nmat, int n,
                                                                   const double* __restrict__ beta, int per_matrix,
-                                                                  float2* __restrict__ out) {
+                                                                  Z* __restrict__ out) {
+  using S = typename ZT<Z>::S;
   const long long nn = (long long)n * n, tot = nmat * nn;
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (long long)gridDim.x * blockDim.x) {
     const long long m = t / nn;
     const int e = (int)(t - m * nn);
     const double b = beta[per_matrix ? m : 0];
-    const float2 v = c[t];
+    const Z v = c[t];
     double re = (1.0 - b) * (double)v.x, im = (1.0 - b) * (double)v.y;
     if (e / n == e % n) re += b;
-    out[t] = make_float2((float)re, (float)im);
+    out[t] = mkz<Z>((S)re, (S)im);
   }
 }
 #pragma clang fp contract(on)
 
+// complex128 forms: the VALU kernels with FP64 accumulators and outputs.
+static int co_c128_launch(bool raster, const double* rslc, int nl, int w, int n, const int32_t* idx,
+                          const uint8_t* shp, long long npts, int aw, int rw, long long pix0, const int32_t* pairs,
+                          int npairs, double* cov, double* coh, hipStream_t st) {
+  const int nb = (int)std::min<long long>(npts, 1 << 16);
+  const size_t lds = CO_STAGE_BYTES + (size_t)n * 8;
+  if (raster)
+    hipLaunchKernelGGL((co_kernel<true, false, double2>), dim3(nb), dim3(CO_THREADS), lds, st,
+                       reinterpret_cast<const double2*>(rslc), nl, w, n, nullptr, shp, npts, aw, rw, pix0, nullptr, 0,
+                       reinterpret_cast<double2*>(cov), reinterpret_cast<double2*>(coh), nullptr, nullptr, nullptr);
+  else
+    hipLaunchKernelGGL((co_kernel<false, false, double2>), dim3(nb), dim3(CO_THREADS), lds, st,
+                       reinterpret_cast<const double2*>(rslc), nl, w, n, idx, shp, npts, aw, rw, 0ll, pairs, npairs,
+                       nullptr, reinterpret_cast<double2*>(coh), nullptr, nullptr, nullptr);
+  return launch_status(raster ? "emperical_co_c128" : "emperical_co_pc_c128");
+}
+
 }  // namespace mr
 
 extern "C" {
+
+int mr_emperical_co_c128(const double* rslc, const uint8_t* is_shp, int nl, int w, int n, int aw, int rw, int row0,
+                         int nrows, double* cov, double* coh, mr_stream_t stream) {
+  MR_CHECK_ARG(rslc && is_shp && coh, "emperical_co_c128: null pointer");
+  MR_CHECK_ARG(nl > 0 && w > 0 && row0 >= 0 && nrows >= 0 && row0 + nrows <= nl, "emperical_co_c128: bad row range");
+  if (int rc = mr::co_check(n, aw, rw)) return rc;
+  const long long npts = (long long)nrows * w;
+  if (npts == 0) return MR_OK;
+  return mr::co_c128_launch(true, rslc, nl, w, n, nullptr, is_shp, npts, aw, rw, (long long)row0 * w, nullptr, 0, cov,
+                            coh, mr::as_stream(stream));
+}
+
+int mr_emperical_co_pc_c128(const double* rslc, int nl, int w, int n, const int32_t* idx, const uint8_t* pc_is_shp,
+                            int n_pc, int aw, int rw, const int32_t* pairs, int n_pairs, double* coh,
+                            mr_stream_t stream) {
+  MR_CHECK_ARG(n_pc >= 0 && n_pairs >= 0, "emperical_co_pc_c128: negative sizes");
+  if (n_pc == 0 || n_pairs == 0) return MR_OK;
+  MR_CHECK_ARG(rslc && idx && pc_is_shp && pairs && coh, "emperical_co_pc_c128: null pointer");
+  if (int rc = mr::co_check(n, aw, rw)) return rc;
+  return mr::co_c128_launch(false, rslc, nl, w, n, idx, pc_is_shp, n_pc, aw, rw, 0, pairs, n_pairs, nullptr, coh,
+                            mr::as_stream(stream));
+}
+
+int mr_uncompress_coh_c128(const double* coh, long long n_pts, int n, const int32_t* pairs, int n_pairs, double* out,
+                           mr_stream_t stream) {
+  MR_CHECK_ARG(n >= 1 && n_pts >= 0 && n_pairs >= 0, "uncompress_coh_c128: bad sizes");
+  MR_CHECK_ARG(n <= mr::UNC_LUT_ENTRIES, "uncompress_coh_c128: n = %d above %d", n, mr::UNC_LUT_ENTRIES);
+  MR_CHECK_ARG(n_pairs <= (1 << 30), "uncompress_coh_c128: %d pairs above 2^30", n_pairs);
+  if (n_pts == 0) return MR_OK;
+  MR_CHECK_ARG(out != nullptr, "uncompress_coh_c128: null output");
+  MR_CHECK_ARG(n_pairs == 0 || (coh && pairs), "uncompress_coh_c128: null input");
+  const int band = std::min(n, mr::UNC_LUT_ENTRIES / n);
+  const int nbands = (int)mr::div_up(n, band);
+  const long long nb = std::min<long long>(n_pts, std::max(1, 1024 / nbands));
+  const long long per = std::min<long long>(mr::div_up((unsigned long long)n_pts, (unsigned long long)nb),
+                                            ((1ll << 31) - 1 - 4 * mr::UNC_THREADS) / ((long long)band * n));
+  const long long blocks = (n_pts + per - 1) / per;
+  MR_CHECK_ARG(blocks < (1ll << 31), "uncompress_coh_c128: %lld points too many", n_pts);
+  const size_t lut = ((size_t)band * n + 3) / 4 * 16;
+  hipLaunchKernelGGL((mr::uncompress_kernel<false, double2>), dim3((unsigned)blocks, (unsigned)nbands),
+                     dim3(mr::UNC_THREADS), lut, mr::as_stream(stream), reinterpret_cast<const double2*>(coh), n_pts,
+                     n, pairs, n_pairs, band, per, 1, reinterpret_cast<double2*>(out));
+  return mr::launch_status("uncompress_coh_c128");
+}
+
+int mr_ad_intf_pc_c128(const double* ref_rslc, const double* sec_rslc, int nl, int w, const int32_t* idx,
+                       const uint8_t* pc_is_shp, long long n_pc, int aw, int rw, double* intf, mr_stream_t stream) {
+  MR_CHECK_ARG(nl > 0 && w > 0 && n_pc >= 0, "ad_intf_pc_c128: bad sizes");
+  MR_CHECK_ARG(aw >= 1 && rw >= 1 && (aw & 1) && (rw & 1), "ad_intf_pc_c128: window %d x %d must be odd", aw, rw);
+  if (n_pc == 0) return MR_OK;
+  MR_CHECK_ARG(ref_rslc && sec_rslc && idx && pc_is_shp && intf, "ad_intf_pc_c128: null pointer");
+  const int nb = (int)std::min<long long>(mr::div_up((unsigned long long)n_pc, 4), 1 << 16);
+  hipLaunchKernelGGL(mr::ad_intf_pc_kernel<double2>, dim3(nb), dim3(256), 0, mr::as_stream(stream),
+                     reinterpret_cast<const double2*>(ref_rslc), reinterpret_cast<const double2*>(sec_rslc), nl, w,
+                     idx, reinterpret_cast<const bool*>(pc_is_shp), n_pc, aw, rw, reinterpret_cast<double2*>(intf));
+  return mr::launch_status("ad_intf_pc_c128");
+}
+
+int mr_regularize_spectral_c128(const double* coh, long long n_mat, int n, const double* beta, int beta_per_matrix,
+                                double* out, mr_stream_t stream) {
+  MR_CHECK_ARG(n_mat >= 0 && n >= 1, "regularize_spectral_c128: bad sizes");
+  if (n_mat == 0) return MR_OK;
+  MR_CHECK_ARG(coh && beta && out, "regularize_spectral_c128: null pointer");
+  const int nb = (int)std::min<long long>(mr::div_up((unsigned long long)n_mat * n * n, 256), 1 << 16);
+  hipLaunchKernelGGL(mr::regularize_spectral_kernel<double2>, dim3(nb), dim3(256), 0, mr::as_stream(stream),
+                     reinterpret_cast<const double2*>(coh), n_mat, n, beta, beta_per_matrix,
+                     reinterpret_cast<double2*>(out));
+  return mr::launch_status("regularize_spectral_c128");
+}
 
 int mr_emperical_co(const float* rslc, const uint8_t* is_shp, int nl, int w, int n, int aw, int rw, int row0,
                     int nrows, float* cov, float* coh, mr_stream_t stream) {
@@ -396,7 +490,7 @@ int mr_emperical_co(const float* rslc, const uint8_t* is_shp, int nl, int w, int
                               reinterpret_cast<float2*>(coh), mr::as_stream(stream));
   const int nb = (int)std::min<long long>(npts, 1 << 16);
   const size_t lds = mr::CO_STAGE_BYTES + (size_t)n * 4;
-  hipLaunchKernelGGL((mr::co_kernel<true>), dim3(nb), dim3(mr::CO_THREADS), lds, mr::as_stream(stream),
+  hipLaunchKernelGGL((mr::co_kernel<true, false, float2>), dim3(nb), dim3(mr::CO_THREADS), lds, mr::as_stream(stream),
                      reinterpret_cast<const float2*>(rslc), nl, w, n, nullptr, is_shp, npts, aw, rw,
                      (long long)row0 * w, nullptr, 0, reinterpret_cast<float2*>(cov), reinterpret_cast<float2*>(coh));
   return mr::launch_status("emperical_co");
@@ -416,7 +510,7 @@ int mr_emperical_co_pc(const float* rslc, int nl, int w, int n, const int32_t* i
                                    pairs, n_pairs, reinterpret_cast<float2*>(coh), mr::as_stream(stream));
   const int nb = std::min(n_pc, 1 << 16);
   const size_t lds = mr::CO_STAGE_BYTES + (size_t)n * 4;
-  hipLaunchKernelGGL((mr::co_kernel<false>), dim3(nb), dim3(mr::CO_THREADS), lds, mr::as_stream(stream),
+  hipLaunchKernelGGL((mr::co_kernel<false, false, float2>), dim3(nb), dim3(mr::CO_THREADS), lds, mr::as_stream(stream),
                      reinterpret_cast<const float2*>(rslc), nl, w, n, idx, pc_is_shp, (long long)n_pc, aw, rw, 0ll,
                      pairs, n_pairs, nullptr, reinterpret_cast<float2*>(coh));
   return mr::launch_status("emperical_co_pc");
@@ -461,7 +555,7 @@ int mr_ad_intf_pc(const float* ref_rslc, const float* sec_rslc, int nl, int w, c
   if (n_pc == 0) return MR_OK;
   MR_CHECK_ARG(ref_rslc && sec_rslc && idx && pc_is_shp && intf, "ad_intf_pc: null pointer");
   const int nb = (int)std::min<long long>(mr::div_up((unsigned long long)n_pc, 4), 1 << 16);
-  hipLaunchKernelGGL(mr::ad_intf_pc_kernel, dim3(nb), dim3(256), 0, mr::as_stream(stream),
+  hipLaunchKernelGGL(mr::ad_intf_pc_kernel<float2>, dim3(nb), dim3(256), 0, mr::as_stream(stream),
                      reinterpret_cast<const float2*>(ref_rslc), reinterpret_cast<const float2*>(sec_rslc), nl, w, idx,
                      reinterpret_cast<const bool*>(pc_is_shp), n_pc, aw, rw, reinterpret_cast<float2*>(intf));
   return mr::launch_status("ad_intf_pc");
@@ -473,7 +567,7 @@ int mr_regularize_spectral(const float* coh, long long n_mat, int n, const doubl
   if (n_mat == 0) return MR_OK;
   MR_CHECK_ARG(coh && beta && out, "regularize_spectral: null pointer");
   const int nb = (int)std::min<long long>(mr::div_up((unsigned long long)n_mat * n * n, 256), 1 << 16);
-  hipLaunchKernelGGL(mr::regularize_spectral_kernel, dim3(nb), dim3(256), 0, mr::as_stream(stream),
+  hipLaunchKernelGGL(mr::regularize_spectral_kernel<float2>, dim3(nb), dim3(256), 0, mr::as_stream(stream),
                      reinterpret_cast<const float2*>(coh), n_mat, n, beta, beta_per_matrix,
                      reinterpret_cast<float2*>(out));
   return mr::launch_status("regularize_spectral");

@@ -200,6 +200,25 @@ __device__ __forceinline__ double rcp_d(double x) {
   return r;
 }
 
+// ---- complex elements of complex64 (float2) / complex128 (double2) arrays ---
+template <typename Z>
+struct ZT;
+template <>
+struct ZT<float2> {
+  using S = float;
+};
+template <>
+struct ZT<double2> {
+  using S = double;
+};
+template <typename Z>
+__host__ __device__ __forceinline__ Z mkz(typename ZT<Z>::S x, typename ZT<Z>::S y) {
+  Z v;
+  v.x = x;
+  v.y = y;
+  return v;
+}
+
 // ---- complex helpers (float2 / double2 as complex) ------------------------
 struct c64 {
   float re, im;

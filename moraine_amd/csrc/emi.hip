@@ -211,10 +211,12 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 // LIST: the FP64 re-solve of flagged points: the points are list[0 .. *count);
 // for n <= 64 the block's slice lives in LDS, and with t_coh set each
 // re-solved point's DS temporal coherence is recomputed from its new phases.
-template <bool LIST>
-__global__ __launch_bounds__(EMI_GEN_THREADS) void emi_general_kernel(const float2* __restrict__ coh,
+// Z = double2: the complex128 form (coherence and phases complex128; every
+// stage FP64 as below; n <= 64 with the slice in LDS).
+template <bool LIST, typename Z = float2>
+__global__ __launch_bounds__(EMI_GEN_THREADS) void emi_general_kernel(const Z* __restrict__ coh,
                                                                       long long npts, int n, int ref,
-                                                                      float2* __restrict__ ph,
+                                                                      Z* __restrict__ ph,
                                                                       float* __restrict__ quality,
                                                                       unsigned char* __restrict__ ws,
                                                                       const int* __restrict__ list = nullptr,
@@ -224,13 +226,15 @@ __global__ __launch_bounds__(EMI_GEN_THREADS) void emi_general_kernel(const floa
   __shared__ double red[EMI_GEN_THREADS / 64];
   __shared__ double sh_key[EMI_GEN_THREADS];
   __shared__ int sh_idx[EMI_GEN_THREADS];
-  __shared__ double2 tri_de[LIST ? 64 : 1];  // LIST: T for the one-wave eigen-solve
-  __shared__ double tri_e[LIST ? 64 : 1];
+  // T for the one-wave eigen-solve of the LDS-slice path (LIST or complex128, n <= 64)
+  constexpr bool kSmallPath = LIST || sizeof(Z) == 16;
+  __shared__ double2 tri_de[kSmallPath ? 64 : 1];
+  __shared__ double tri_e[kSmallPath ? 64 : 1];
   extern __shared__ __align__(16) unsigned char gen_lds[];
   const int P = n * (n - 1) / 2;
   const int tid = threadIdx.x;
   // LIST with n <= 64: the slice in LDS, pivot and eigen-solve by one wave
-  const bool small = LIST && n <= 64;
+  const bool small = kSmallPath && n <= 64;
   unsigned char* base = small ? gen_lds : ws + (size_t)blockIdx.x * emi_gen_slice_bytes(n);
   double* A = reinterpret_cast<double*>(base);
   c128* M = reinterpret_cast<c128*>(A + (size_t)n * n);
@@ -251,17 +255,17 @@ __global__ __launch_bounds__(EMI_GEN_THREADS) void emi_general_kernel(const floa
   const long long nit = LIST ? (long long)*count : npts;
   for (long long it = blockIdx.x; it < nit; it += gridDim.x) {
     const long long pt = LIST ? (long long)list[it] : it;
-    const float2* src = coh + pt * (long long)P;
+    const Z* src = coh + pt * (long long)P;
     for (int t = tid; t < n * n; t += EMI_GEN_THREADS) {
       const int i = t / n, j = t - (t / n) * n;
       c128 g;
       if (i == j) {
         g = {1.0, 0.0};
       } else if (j > i) {
-        const float2 c = src[pair_index(i, j, n)];
+        const Z c = src[pair_index(i, j, n)];
         g = {c.x, c.y};
       } else {
-        const float2 c = src[pair_index(j, i, n)];
+        const Z c = src[pair_index(j, i, n)];
         g = {c.x, -c.y};
       }
       M[t] = g;
@@ -483,16 +487,18 @@ __global__ __launch_bounds__(EMI_GEN_THREADS) void emi_general_kernel(const floa
       const double pr = pv[i].re * vr.re + pv[i].im * vr.im;
       const double pim = pv[i].im * vr.re - pv[i].re * vr.im;
       const double am = sqrt(pr * pr + pim * pim);
-      const float2 v = make_float2((float)(pr / am), (float)(pim / am));
+      const Z v = mkz<Z>((typename ZT<Z>::S)(pr / am), (typename ZT<Z>::S)(pim / am));
       ph[pt * n + i] = v;
-      if (LIST) reinterpret_cast<float2*>(wk)[i] = v;  // LU work space is free again
+      if constexpr (LIST) reinterpret_cast<float2*>(wk)[i] = v;  // LU work space is free again
     }
     __syncthreads();
-    if (LIST && t_coh != nullptr && tid < 64) {
-      const float2* phl = reinterpret_cast<const float2*>(wk);
-      const float tc = tcoh_point<true>([&](int p, int, int) { return src[p]; }, [&](int i) { return phl[i]; }, pairs,
-                                        P, tid);
-      if (tid == 0) t_coh[pt] = tc;
+    if constexpr (LIST) {
+      if (t_coh != nullptr && tid < 64) {
+        const float2* phl = reinterpret_cast<const float2*>(wk);
+        const float tc = tcoh_point<true>([&](int p, int, int) { return src[p]; }, [&](int i) { return phl[i]; },
+                                          pairs, P, tid);
+        if (tid == 0) t_coh[pt] = tc;
+      }
     }
     __syncthreads();
   }
@@ -543,6 +549,30 @@ int mr_emi(const float* coh, long long n_pts, int n, int ref, float* ph, float* 
   hipLaunchKernelGGL(mr::emi_general_kernel<false>, dim3(nb), dim3(mr::EMI_GEN_THREADS), 0, st, c, n_pts, n, ref, p,
                      quality, reinterpret_cast<unsigned char*>(workspace));
   return mr::launch_status("emi_general");
+}
+
+size_t mr_emi_c128_workspace_bytes(long long n_pts, int n) {
+  if (n <= 64) return 0;  // the slice of a point lives in LDS
+  const long long nb = n_pts < mr::EMI_GEN_BLOCKS ? n_pts : mr::EMI_GEN_BLOCKS;
+  return (size_t)(nb > 0 ? nb : 1) * mr::emi_gen_slice_bytes(n);
+}
+
+int mr_emi_c128(const double* coh, long long n_pts, int n, int ref, double* ph, float* quality, void* workspace,
+                size_t workspace_bytes, mr_stream_t stream) {
+  MR_CHECK_ARG(n >= 2, "emi_c128: need at least 2 images (n=%d)", n);
+  MR_CHECK_ARG(ref >= 0 && ref < n, "emi_c128: ref=%d outside [0,%d)", ref, n);
+  MR_CHECK_ARG(n_pts >= 0, "emi_c128: negative point count");
+  if (n_pts == 0) return MR_OK;
+  MR_CHECK_ARG(coh && ph && quality, "emi_c128: null pointer");
+  const size_t need = mr_emi_c128_workspace_bytes(n_pts, n);
+  MR_CHECK_ARG(need == 0 || (workspace != nullptr && workspace_bytes >= need), "emi_c128: workspace %zu < %zu bytes",
+               workspace_bytes, need);
+  const int nb = (int)std::min<long long>(n_pts, mr::EMI_GEN_BLOCKS);
+  hipLaunchKernelGGL((mr::emi_general_kernel<false, double2>), dim3(nb), dim3(mr::EMI_GEN_THREADS),
+                     n <= 64 ? mr::emi_gen_slice_bytes(n) : 0, mr::as_stream(stream),
+                     reinterpret_cast<const double2*>(coh), n_pts, n, ref, reinterpret_cast<double2*>(ph), quality,
+                     reinterpret_cast<unsigned char*>(workspace), nullptr, nullptr, nullptr, nullptr);
+  return mr::launch_status("emi_c128");
 }
 
 int mr_emi_ds_temp_coh(const float* coh, long long n_pts, int n, int ref, const int32_t* pairs, int n_pairs,

@@ -401,7 +401,8 @@ __global__ void fill_i32_kernel(int32_t* __restrict__ x, long long n, int32_t v)
 }
 
 // select_shp on an existing p array: one wave per pixel.
-__global__ __launch_bounds__(256) void select_shp_kernel(const float* __restrict__ p, long long npix, int win,
+template <typename T>
+__global__ __launch_bounds__(256) void select_shp_kernel(const T* __restrict__ p, long long npix, int win,
                                                          double p_max, uint8_t* __restrict__ is_shp,
                                                          int32_t* __restrict__ shp_num) {
   const int lane = threadIdx.x & 63;
@@ -410,8 +411,8 @@ __global__ __launch_bounds__(256) void select_shp_kernel(const float* __restrict
   for (long long px = wave0; px < npix; px += nw) {
     int c = 0;
     for (int i = lane; i < win; i += 64) {
-      const float v = p[px * win + i];
-      const bool s = (double)v < p_max;  // NaN -> false
+      const T v = p[px * win + i];
+      const bool s = (double)v < p_max;  // NaN -> false; float64 p compares in float64
       if (is_shp) is_shp[px * win + i] = s;
       c += s;
     }
@@ -634,9 +635,19 @@ int mr_select_shp_f32(const float* p, long long npix, int win, double p_max, uin
   MR_CHECK_ARG(p != nullptr && npix >= 0 && win > 0, "select_shp: bad arguments");
   if (npix == 0) return MR_OK;
   const int nb = (int)std::min<long long>(mr::div_up(npix, 4), 65536);
-  hipLaunchKernelGGL(mr::select_shp_kernel, dim3(nb), dim3(256), 0, mr::as_stream(stream), p, npix, win, p_max,
+  hipLaunchKernelGGL(mr::select_shp_kernel<float>, dim3(nb), dim3(256), 0, mr::as_stream(stream), p, npix, win, p_max,
                      is_shp, shp_num);
   return mr::launch_status("select_shp");
+}
+
+int mr_select_shp_f64(const double* p, long long npix, int win, double p_max, uint8_t* is_shp, int32_t* shp_num,
+                      mr_stream_t stream) {
+  MR_CHECK_ARG(p != nullptr && npix >= 0 && win > 0, "select_shp_f64: bad arguments");
+  if (npix == 0) return MR_OK;
+  const int nb = (int)std::min<long long>(mr::div_up(npix, 4), 65536);
+  hipLaunchKernelGGL(mr::select_shp_kernel<double>, dim3(nb), dim3(256), 0, mr::as_stream(stream), p, npix, win,
+                     p_max, is_shp, shp_num);
+  return mr::launch_status("select_shp_f64");
 }
 
 }  // extern "C"

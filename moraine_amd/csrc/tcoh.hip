@@ -34,6 +34,53 @@ __global__ __launch_bounds__(256) void tcoh_kernel(const float2* __restrict__ c,
   }
 }
 
+// complex128 inputs: the same sums in FP64 (the reference's accumulator has
+// the input's complex type), the result rounded to float32 as its output is.
+template <bool DS>
+__global__ __launch_bounds__(256) void tcoh64_kernel(const double2* __restrict__ c, const double2* __restrict__ v,
+                                                     long long npts, int n, const int32_t* __restrict__ pairs,
+                                                     int npairs, float* __restrict__ out) {
+  extern __shared__ double2 tcoh_ph64[];
+  const int lane = threadIdx.x & 63;
+  double2* ph = tcoh_ph64 + (threadIdx.x >> 6) * n;
+  const long long w0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
+  for (long long pt = w0; pt < npts; pt += nw) {
+    const double2* cp = c + pt * npairs;
+    const double2* vp = v + pt * n;
+    for (int i = lane; i < n; i += 64) ph[i] = vp[i];
+    __builtin_amdgcn_wave_barrier();
+    double sr = 0.0, si = 0.0;
+    for (int p = lane; p < npairs; p += 64) {
+      const int a = pairs[2 * p], b = pairs[2 * p + 1];
+      const double2 x = cp[p], va = ph[a], vb = ph[b];
+      const double tr = va.x * vb.x + va.y * vb.y, ti = va.x * vb.y - va.y * vb.x;  // conj(va) vb
+      const double den = DS ? sqrt(x.x * x.x + x.y * x.y)
+                            : sqrt((x.x * x.x + x.y * x.y) * (va.x * va.x + va.y * va.y) * (vb.x * vb.x + vb.y * vb.y));
+      sr += (x.x * tr - x.y * ti) / den;
+      si += (x.x * ti + x.y * tr) / den;
+    }
+    sr = wave_sum(sr);
+    si = wave_sum(si);
+    if (lane == 0) out[pt] = (float)(sqrt(sr * sr + si * si) / (double)npairs);
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+template <bool DS>
+static int tcoh64_launch(const double* c, const double* v, long long n_pts, int n, const int32_t* pairs, int n_pairs,
+                         float* out, mr_stream_t stream, const char* name) {
+  MR_CHECK_ARG(n_pts >= 0 && n >= 1 && n_pairs >= 0, "%s: bad sizes", name);
+  MR_CHECK_ARG(n <= 2048, "%s: n=%d above 2048 (the phases of a point are staged in LDS)", name, n);
+  if (n_pts == 0) return MR_OK;
+  MR_CHECK_ARG(c && v && pairs && out, "%s: null pointer", name);
+  const int nb = (int)std::min<long long>(div_up((unsigned long long)n_pts, 4), 1 << 16);
+  hipLaunchKernelGGL((tcoh64_kernel<DS>), dim3(nb), dim3(256), (size_t)4 * n * sizeof(double2), as_stream(stream),
+                     reinterpret_cast<const double2*>(c), reinterpret_cast<const double2*>(v), n_pts, n, pairs,
+                     n_pairs, out);
+  return launch_status(name);
+}
+
 template <bool DS>
 static int tcoh_launch(const float* c, const float* v, long long n_pts, int n, const int32_t* pairs, int n_pairs,
                        float* out, mr_stream_t stream, const char* name) {
@@ -54,6 +101,14 @@ extern "C" {
 int mr_ds_temp_coh(const float* coh, const float* ph, long long n_pts, int n, const int32_t* pairs, int n_pairs,
                    float* t_coh, mr_stream_t stream) {
   return mr::tcoh_launch<true>(coh, ph, n_pts, n, pairs, n_pairs, t_coh, stream, "ds_temp_coh");
+}
+int mr_ds_temp_coh_c128(const double* coh, const double* ph, long long n_pts, int n, const int32_t* pairs,
+                        int n_pairs, float* t_coh, mr_stream_t stream) {
+  return mr::tcoh64_launch<true>(coh, ph, n_pts, n, pairs, n_pairs, t_coh, stream, "ds_temp_coh_c128");
+}
+int mr_temp_coh_c128(const double* intf, const double* rslc, long long n_pts, int n, const int32_t* pairs,
+                     int n_pairs, float* t_coh, mr_stream_t stream) {
+  return mr::tcoh64_launch<false>(intf, rslc, n_pts, n, pairs, n_pairs, t_coh, stream, "temp_coh_c128");
 }
 int mr_temp_coh(const float* intf, const float* rslc, long long n_pts, int n, const int32_t* pairs, int n_pairs,
                 float* t_coh, mr_stream_t stream) {

@@ -46,8 +46,13 @@ def _emi_refined_dev(c: torch.Tensor, n: int, ref: int, tol: float, ph: torch.Te
 
 def _emi_dev(c: torch.Tensor, n: int, ref: int, tol=None):
     npts = c.shape[0]
-    ph = torch.empty((npts, n), dtype=torch.complex64, device=c.device)
+    ph = torch.empty((npts, n), dtype=c.dtype, device=c.device)
     q = torch.empty(npts, dtype=torch.float32, device=c.device)
+    if _dev.is_c128(c):  # FP64 throughout: refine has nothing to do
+        ws = _dev.workspace(_lib.query("mr_emi_c128_workspace_bytes", npts, n), c.device)
+        _lib.call("mr_emi_c128", _dev.ptr(c), npts, n, int(ref), _dev.ptr(ph), _dev.ptr(q), _dev.ptr(ws), ws.numel(),
+                  _dev.stream())
+        return ph, q
     if tol is not None:
         _emi_refined_dev(c, n, ref, tol, ph, q)
         return ph, q
@@ -59,15 +64,16 @@ def _emi_dev(c: torch.Tensor, n: int, ref: int, tol=None):
 
 
 def emi(coh, ref: int = 0, *, refine=None):
-    """EMI phase history (n_pts, N) complex64 and quality (n_pts,) float32 from
-    compressed coherence (n_pts, N(N-1)/2) in triu order.
+    """EMI phase history (n_pts, N) with coh's dtype and quality (n_pts,) float32
+    from compressed coherence (n_pts, N(N-1)/2) in triu order (moraine/pl.py:45-65).
+    complex128 coherence runs every stage in FP64.
 
     refine (not in moraine): a phase tolerance in radians, e.g. 1e-3.  Points
     whose estimated phase error from the FP32 stages exceeds it are solved
     again in FP64 throughout; the others are unchanged.  (Above 200 images
     EMI is FP64 throughout and refine has nothing to do.)"""
     as_torch = _dev.is_torch(coh)
-    c = _dev.to_dev(coh, torch.complex64)
+    c = _dev.to_dev_complex(coh)
     n = nimage_from_npair(c.shape[-1])
     tol = _check_refine(refine, n)
     lead = tuple(c.shape[:-1])
@@ -77,8 +83,12 @@ def emi(coh, ref: int = 0, *, refine=None):
 
 def _tcoh(kind: str, a, b, image_pairs):
     as_torch = _dev.is_torch(a)
-    x = _dev.to_dev(a, torch.complex64)
-    y = _dev.to_dev(b, torch.complex64)
+    x = _dev.to_dev_complex(a)
+    y = _dev.to_dev_complex(b)
+    if x.dtype != y.dtype:  # complex128 if either is: the FP64 kernel
+        x, y = x.to(torch.complex128), y.to(torch.complex128)
+    if _dev.is_c128(x):
+        kind = kind + "_c128"
     n = y.shape[-1]
     lead = tuple(y.shape[:-1])
     x2 = x.reshape(-1, x.shape[-1])
@@ -101,6 +111,14 @@ def ds_temp_coh(coh, ph, image_pairs=None, block_size: int = 128):
 def _emi_tcoh_dev(c: torch.Tensor, n: int, ref: int, ph: torch.Tensor, q: torch.Tensor, t: torch.Tensor, tol=None):
     """emi then ds_temp_coh on the same full-network rows, into ph, q, t (one
     kernel for 32 < n <= 64; results equal the two calls bit for bit)."""
+    if _dev.is_c128(c):
+        p2, q2 = _emi_dev(c, n, ref)
+        ph.copy_(p2)
+        q.copy_(q2)
+        pairs = _full_pairs_dev(n, c.device)
+        _lib.call("mr_ds_temp_coh_c128", _dev.ptr(c), _dev.ptr(ph), c.shape[0], n, _dev.ptr(pairs), pairs.shape[0],
+                  _dev.ptr(t), _dev.stream())
+        return
     if tol is not None:
         _emi_refined_dev(c, n, ref, tol, ph, q, t)
         return
@@ -117,10 +135,10 @@ def emi_ds_temp_coh(coh, ref: int = 0, *, refine=None):
     the inner stages of moraine.emperical_co_emi_temp_coh_pc (moraine/pl.py:165-167)
     for full-network coherence rows (n_pts, N(N-1)/2) in triu order."""
     as_torch = _dev.is_torch(coh)
-    c = _dev.to_dev(coh, torch.complex64).reshape(-1, coh.shape[-1])
+    c = _dev.to_dev_complex(coh).reshape(-1, coh.shape[-1])
     n = nimage_from_npair(c.shape[-1])
     tol = _check_refine(refine, n)
-    ph = torch.empty((c.shape[0], n), dtype=torch.complex64, device=c.device)
+    ph = torch.empty((c.shape[0], n), dtype=c.dtype, device=c.device)
     q = torch.empty(c.shape[0], dtype=torch.float32, device=c.device)
     t = torch.empty(c.shape[0], dtype=torch.float32, device=c.device)
     _emi_tcoh_dev(c, n, ref, ph, q, t, tol)
@@ -129,25 +147,43 @@ def emi_ds_temp_coh(coh, ref: int = 0, *, refine=None):
             _dev.back(t.reshape(lead), as_torch))
 
 
+#: a batch of the batch driver holds at least this many points (fewer only
+#: lengthen the run: every point is independent) ...
+MIN_BATCH_POINTS = 1 << 16
+#: ... and its coherence buffer takes at most this share of the free HBM
+BATCH_HBM_SHARE = 0.25
+
+
+def batch_points(batch_size: int, n: int, n_pc: int, free_bytes: int) -> int:
+    """Points per batch of emperical_co_emi_temp_coh_pc: the caller's batch_size
+    raised to MIN_BATCH_POINTS, then capped so that the batch's coherence rows
+    (8 N(N-1)/2 bytes per point, 16 for complex128 adds at most 2x: the cap uses
+    16) stay within BATCH_HBM_SHARE of free_bytes; at least 1 point."""
+    per_point = 16 * (n * (n - 1) // 2) + 16 * n
+    cap = max(1, int(BATCH_HBM_SHARE * free_bytes) // per_point)
+    return max(1, min(max(int(batch_size), MIN_BATCH_POINTS), cap, max(n_pc, 1)))
+
+
 def emperical_co_emi_temp_coh_pc(rslc, idx, pc_is_shp, batch_size: int = 1000, *, refine=None):
     """Coherence -> EMI -> temporal coherence for a point cloud, in batches.
 
     Every point is independent and every kernel deterministic, so the result
-    does not depend on the batch size; batches below 65536 points are merged
-    (they only bound memory: 8*N(N-1)/2 bytes of coherence per point).
-    refine: as for emi (not in moraine).
+    does not depend on the batch size.  The batch is batch_points(batch_size,
+    ...): at least 65536 points, at most what a quarter of the free HBM holds.
+    refine: as for emi (not in moraine).  complex128 input runs in FP64 and
+    returns complex128 phases.
     """
     as_torch = _dev.is_torch(rslc)
-    z = _dev.to_dev(rslc, torch.complex64)
+    z = _dev.to_dev_complex(rslc)
     ix = _dev.to_dev(idx, torch.int32)
     m = _dev.to_dev(pc_is_shp, torch.bool)
     n = z.shape[2]
     tol = _check_refine(refine, n)
     n_pc = ix.shape[0]
-    ph = torch.empty((n_pc, n), dtype=torch.complex64, device=z.device)
+    ph = torch.empty((n_pc, n), dtype=z.dtype, device=z.device)
     q = torch.empty(n_pc, dtype=torch.float32, device=z.device)
     t = torch.empty(n_pc, dtype=torch.float32, device=z.device)
-    bs = max(int(batch_size), 1 << 16)
+    bs = batch_points(batch_size, n, n_pc, torch.cuda.mem_get_info(z.device)[0])
     for s in range(0, n_pc, bs):
         e = min(n_pc, s + bs)
         c = emperical_co_pc(z, ix[s:e], m[s:e])

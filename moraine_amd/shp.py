@@ -65,14 +65,17 @@ def ks_test_select_shp(rmli, az_half_win: int, r_half_win: int, p_max: float, re
 
 
 def select_shp(p, p_max: float):
-    """is_shp = p < p_max (NaN -> False) and shp_num = count per pixel (int32)."""
+    """is_shp = p < p_max (NaN -> False) and shp_num = count per pixel (int32).
+    float64 p is compared in float64 (other dtypes as float32)."""
     as_torch = _dev.is_torch(p)
-    x = _dev.to_dev(p, torch.float32)
+    x = _dev.to_dev(p)
+    if x.dtype != torch.float64:
+        x = x.to(torch.float32)
     if x.ndim != 4:
         raise ValueError("select_shp: p must be 4-D (n_az, n_r, az_win, r_win)")
     npix = x.shape[0] * x.shape[1]
     win = x.shape[2] * x.shape[3]
     m = torch.empty(x.shape, dtype=torch.bool, device=x.device)
     cnt = torch.empty(x.shape[:2], dtype=torch.int32, device=x.device)
-    _lib.call("mr_select_shp_f32", _dev.ptr(x), npix, win, float(p_max), _dev.ptr(m), _dev.ptr(cnt), _dev.stream())
+    _lib.call("mr_select_shp_f64" if x.dtype == torch.float64 else "mr_select_shp_f32", _dev.ptr(x), npix, win, float(p_max), _dev.ptr(m), _dev.ptr(cnt), _dev.stream())
     return _dev.back(m, as_torch), _dev.back(cnt, as_torch)

@@ -35,6 +35,9 @@ emi64                       closure-free Gamma: ph = theta*conj(theta_ref)/|.|,
 emi_ref32                   same closure-free case at FP32 tolerance
 ds_temp_coh / temp_coh      t = 1 for consistent inputs; hand-computed 3-image
                             case; FP64 variant agreement
+ad_intf_pc64 / temp_coh64   closed forms (sec = ref e^{i phi} -> e^{-i phi};
+                            consistent phases -> t = 1) and agreement with the
+                            complex64 restatements on complex64 data
 ==========================  ===================================================
 """
 from __future__ import annotations
@@ -90,6 +93,24 @@ def ad_intf_pc(ref_rslc, sec_rslc, idx, pc_is_shp) -> np.ndarray:
                     sa += abs(sec[y, x]) ** 2
         with np.errstate(invalid="ignore", divide="ignore"):
             out[i] = num / np.sqrt(ra * sa) if ra * sa > 0 else complex(np.nan, np.nan)
+    return out
+
+
+def ad_intf_pc64(ref_rslc, sec_rslc, idx, pc_is_shp) -> np.ndarray:
+    """FP64 truth of ``ad_intf_pc`` (moraine/co.py:302-406) for complex128 input:
+    the same window sums as ``ad_intf_pc`` on the unrounded complex128 samples,
+    complex128 out."""
+    ref = np.asarray(ref_rslc).astype(np.complex128)
+    sec = np.asarray(sec_rslc).astype(np.complex128)
+    nl, w = ref.shape
+    aw, rw = pc_is_shp.shape[1:]
+    out = np.empty(idx.shape[0], np.complex128)
+    for i, (a, r) in enumerate(idx):
+        sa, sr, ok = _window_valid(nl, w, aw // 2, rw // 2, int(a), int(r), aw, rw)
+        sel = ok & pc_is_shp[i].astype(bool)
+        x, y = ref[sa[sel], sr[sel]], sec[sa[sel], sr[sel]]
+        with np.errstate(invalid="ignore", divide="ignore"):
+            out[i] = np.sum(x * np.conj(y)) / np.sqrt(np.sum(np.abs(x) ** 2) * np.sum(np.abs(y) ** 2))
     return out
 
 
@@ -511,6 +532,21 @@ def temp_coh(intf: np.ndarray, rslc: np.ndarray, image_pairs=None):
         acc = (acc + term[:, j]).astype(np.complex64)
     out = (np.abs(acc) / image_pairs.shape[0]).astype(np.float32)
     return out.reshape(shape) if is_ras else out
+
+
+def temp_coh64(intf, rslc, image_pairs=None):
+    """FP64 truth of ``temp_coh`` (moraine/pqm.py:46-121) on point-cloud input:
+    t = |sum_p intf_p conj(r_a) r_b / (|intf_p| |r_a| |r_b|)| / n_pairs."""
+    n = rslc.shape[-1]
+    if image_pairs is None:
+        image_pairs = image_pairs_from_bandwidth(n)
+    image_pairs = np.asarray(image_pairs)
+    f = np.asarray(intf).astype(np.complex128)
+    r = np.asarray(rslc).astype(np.complex128)
+    a, b = image_pairs[:, 0], image_pairs[:, 1]
+    with np.errstate(invalid="ignore", divide="ignore"):
+        t = np.sum(f * np.conj(r[:, a]) * r[:, b] / (np.abs(f) * np.abs(r[:, a]) * np.abs(r[:, b])), axis=1)
+    return np.abs(t) / image_pairs.shape[0]
 
 
 def emperical_co_emi_temp_coh_pc(rslc, idx, pc_is_shp, batch_size=1000):

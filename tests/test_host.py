@@ -130,3 +130,35 @@ def test_noalloc_ks_rejects_oversized_stack_on_host():
     rc = lib.mr_ks_test_f32_noalloc(None, 2048, 2048, 60, 7, 10, 0.05, None, None, None, None, None)
     assert rc == 3
     assert b"static scratch" in lib.mr_last_error_string()
+
+
+def test_pair_indices_follow_numpy_rules():
+    from moraine_amd.co import _pairs
+
+    got = _pairs(np.array([[0, 1], [-3, -1], [2, 3]]), 4)
+    np.testing.assert_array_equal(got, [[0, 1], [1, 3], [2, 3]])
+    assert got.dtype == np.int32
+    with pytest.raises(IndexError):
+        _pairs(np.array([[0, 4]]), 4)
+    with pytest.raises(IndexError):
+        _pairs(np.array([[-5, 1]]), 4)
+    with pytest.raises(ValueError):
+        _pairs(np.array([0, 1, 2]), 4)
+
+
+def test_batch_points_respects_hbm_budget():
+    """The batch driver's batch: at least 65536 points, never more coherence
+    bytes than a quarter of the free HBM, at least one point."""
+    from moraine_amd.pl import BATCH_HBM_SHARE, MIN_BATCH_POINTS, batch_points
+
+    gib = 1 << 30
+    for n in (17, 60, 200, 1000):
+        for free in (1 * gib, 16 * gib, 250 * gib):
+            for bsz in (1, 1000, 1 << 20):
+                bs = batch_points(bsz, n, 10 ** 7, free)
+                per = 16 * (n * (n - 1) // 2) + 16 * n
+                assert bs >= 1
+                assert bs == 1 or bs * per <= BATCH_HBM_SHARE * free
+                assert bs >= min(max(bsz, MIN_BATCH_POINTS), int(BATCH_HBM_SHARE * free) // per, 10 ** 7) or bs == 1
+    assert batch_points(1000, 60, 5000, 250 * gib) == 5000  # never more than the point count
+    assert batch_points(1000, 1000, 10 ** 6, 16 * gib) == 1 * 16 * gib // 4 // (16 * 499500 + 16000)

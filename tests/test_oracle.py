@@ -364,3 +364,26 @@ def test_regularize_spectral_closed_forms():
     b = rng.random((3, 4))
     r = R.regularize_spectral(c, b)
     np.testing.assert_allclose(r[1, 2], (1 - b[1, 2]) * c[1, 2] + b[1, 2] * np.eye(5), atol=1e-12)
+
+
+def test_fp64_variants_closed_forms_and_agreement():
+    """ad_intf_pc64: sec = ref e^{i phi} -> e^{-i phi} on complex128 data, equals
+    ad_intf_pc on complex64 data to complex64 rounding.  temp_coh64: consistent
+    phases -> 1, equals temp_coh to float32 rounding."""
+    rng = np.random.default_rng(9)
+    ref = rng.standard_normal((7, 9)) + 1j * rng.standard_normal((7, 9))
+    sec = ref * np.exp(1j * 1.3)
+    idx = np.array([[0, 0], [3, 4], [6, 8]], np.int32)
+    shp = rng.random((3, 3, 5)) < 0.7
+    shp[:, 1, 2] = True
+    np.testing.assert_allclose(R.ad_intf_pc64(ref, sec, idx, shp), np.exp(-1j * 1.3), atol=1e-14)
+    r64, s64 = ref.astype(np.complex64), (rng.standard_normal((7, 9)) + 1j).astype(np.complex64)
+    np.testing.assert_allclose(R.ad_intf_pc64(r64, s64, idx, shp), R.ad_intf_pc(r64, s64, idx, shp), atol=2e-7)
+    n = 7
+    th = np.exp(1j * rng.uniform(-np.pi, np.pi, (5, n)))
+    pairs = R.image_pairs_from_bandwidth(n)
+    f = 2.5 * th[:, pairs[:, 0]] * np.conj(th[:, pairs[:, 1]])
+    np.testing.assert_allclose(R.temp_coh64(f, 3 * th), 1.0, atol=1e-14)
+    f2 = (rng.standard_normal(f.shape) + 1j * rng.standard_normal(f.shape)).astype(np.complex64)
+    th2 = th.astype(np.complex64)
+    np.testing.assert_allclose(R.temp_coh64(f2, th2), R.temp_coh(f2, th2), atol=2e-6)
