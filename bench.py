@@ -12,8 +12,10 @@ pixel taken through the whole path.  One step = one pass over the tile:
 
 Inputs are generated on the device before timing (synthetic, seeded).  Prints
 one JSON line.  Algorithmic units per pixel: KS 4N + 5W + 4 bytes, coherence 4K N(N+1)
-flops, EMI (2 + 16/3) N^3 flops, temporal coherence 8P + 8N + 4 bytes (W window entries,
-K SHPs, P = N(N-1)/2 pairs).
+flops, EMI N^3 (FP64) + (4/3) N^3 (FP32) + 8 N^2 x 2 iterations flops (the formulation-
+independent count: Cholesky and inverse of |Gamma|, a complex LDL^H factorisation and two
+inverse-iteration sweeps), temporal coherence 8P + 8N + 4 bytes (W window entries, K SHPs,
+P = N(N-1)/2 pairs).
 """
 from __future__ import annotations
 
@@ -30,6 +32,18 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 (vector and matrix); also FP32 vector without packed math
 
 
+def emi_algorithmic_flops(n: int) -> float:
+    """Algorithmic EMI flops per point: N^3 (FP64 Cholesky + SPD inverse of |Gamma|)
+    + (4/3) N^3 (FP32 complex LDL^H) + 8 N^2 per inverse-iteration sweep, two sweeps."""
+    return n ** 3 + (4.0 / 3.0) * n ** 3 + 2 * 8.0 * n ** 2
+
+
+def emi_executed_flops(n: int) -> float:
+    """Flops the built EMI kernel executes per point: 2N^3 (FP64 sweep inverse of
+    |Gamma|) + (16/3) N^3 (FP32 Householder tridiagonalisation of the Hermitian M)."""
+    return (2.0 + 16.0 / 3.0) * n ** 3
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -44,12 +58,14 @@ def parse():
     ap.add_argument("--cpu-sample", type=int, default=160, help="CPU baseline: interior sub-tile edge")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--err-sample", type=int, default=2048)
-    ap.add_argument("--forms", default="plain,batched,serving,rows,streamed",
+    ap.add_argument("--forms", default="plain,batched,serving,rows,streamed,precision",
                     help="comma list: plain (drop-in pipeline, the headline), batched, serving, rows (each "
-                         "drop-in function timed alone), streamed (the same tile from host memory in bands)")
+                         "drop-in function timed alone), streamed (the same tile from host memory in bands), "
+                         "precision (the N=120, 21x41, 1000x4000 mixed-coherence workload)")
     ap.add_argument("--batched-px", type=int, default=200000, help="pixels of the ragged batch")
     ap.add_argument("--serve-requests", type=int, default=16)
-    ap.add_argument("--serve-edge", type=int, default=512, help="largest tile edge of the request stream")
+    ap.add_argument("--serve-edge", type=int, default=1024, help="largest tile edge of the request stream")
+    ap.add_argument("--c3-lines", type=int, default=1000, help="precision workload: tile lines (x 4000 columns)")
     return ap.parse_args()
 
 
@@ -228,8 +244,8 @@ def rows_leg(args, torch):
                       "per_call": what}
 
     def flops(name, t, nflop, peak, what):
-        rows[name] = {"s": t, "TFLOP/s": nflop / t / 1e12, "frac": nflop / t / (peak * 1e12), "bound": "compute",
-                      "per_call": what}
+        rows[name] = {"s": t, "TFLOP/s": nflop / t / 1e12, "frac": nflop / t / (peak * 1e12),
+                      "bound": "mfma" if peak > 1000 else "valu", "per_call": what}
 
     rows = {}
     N, az, r = 60, 7, 10
@@ -275,7 +291,7 @@ def rows_leg(args, torch):
     ch = coh[:65536].contiguous()
     hbm("uncompress_coh", dtime(lambda: mr.uncompress_coh(ch)), 65536 * (8 * P + 8 * N * N),
         "65536 points, (P) -> (N, N)")
-    emi_fl = (2.0 + 16.0 / 3.0) * N ** 3
+    emi_fl = emi_algorithmic_flops(N)
     flops("emi", dtime(lambda: mr.emi(coh)), npt * emi_fl, FP64_PEAK_TFLOPS, f"{npt} points")
     ph, q = mr.emi(coh)
     hbm("ds_temp_coh", dtime(lambda: mr.ds_temp_coh(coh, ph)), npt * (8 * P + 8 * N + 4), f"{npt} points")
@@ -287,11 +303,92 @@ def rows_leg(args, torch):
     rows["emperical_co_emi_temp_coh_pc"] = {"s": t, "points/s": npt / t, "per_call": f"{npt} points, one batch"}
     for n2, npt2 in ((120, 32768), (200, 16384)):
         c2 = torch.randn(npt2, n2 * (n2 - 1) // 2, dtype=torch.complex64, device="cuda") * 0.1
-        flops(f"emi_n{n2}", dtime(lambda: mr.emi(c2), reps=2), npt2 * (2.0 + 16.0 / 3.0) * n2 ** 3,
+        flops(f"emi_n{n2}", dtime(lambda: mr.emi(c2), reps=2), npt2 * emi_algorithmic_flops(n2),
               FP64_PEAK_TFLOPS, f"{npt2} points, N={n2} (random coherence rows)")
         del c2
     return {"workload": f"N={N}, 15x21 windows, synthetic DS stack; each function timed alone (HIP events)",
             "rows": rows}
+
+
+def _phase_rms(ph, ph64, torch):
+    err = torch.angle(ph.to(torch.complex128) * ph64.conj())
+    return torch.sqrt(torch.nanmean(err ** 2, dim=1))
+
+
+def _rms_stats(rms, torch):
+    ok = rms[~torch.isnan(rms)]
+    return {"median": float(ok.median()), "p99": float(torch.quantile(ok, 0.99)), "max": float(ok.max()),
+            "all": float(torch.sqrt(torch.mean(ok ** 2))), "points": int(ok.numel())}
+
+
+def config3_leg(args, torch):
+    """Precision workload: 1000x4000 tile, N=120, 21x41 windows, gamma0 ~ U[0.2, 0.9] per pixel and
+    2% near-singular pixels (gamma0 in [0.98, 0.999] at every lag).  The plain drop-in pass (KS +
+    selection with p materialised, coherence, EMI + temporal coherence in 65536-point batches) and
+    the same pass with emi(refine=1e-3); per-point phase error against an FP64 solve (complex128
+    inverse and eigh) on a sample of the last batch, beside the reference's own arithmetic (float32
+    inv, complex64 eigh on the CPU) on the same points."""
+    import moraine_amd as mr
+    from moraine_amd import synth
+    from moraine_amd.pl import _emi_tcoh_dev as emi_tcoh
+
+    nl, w, n, az, r = args.c3_lines, 4000, 120, 10, 20
+    aw, rw = 2 * az + 1, 2 * r + 1
+    z, ns = synth.ds_stack_mixed_torch(nl, w, n, seed=3, device="cuda")
+    npx = nl * w
+    ii, jj = torch.meshgrid(torch.arange(nl, device="cuda", dtype=torch.int32),
+                            torch.arange(w, device="cuda", dtype=torch.int32), indexing="ij")
+    idx = torch.stack([ii.reshape(-1), jj.reshape(-1)], -1).contiguous()
+    del ii, jj
+    ph = torch.empty((npx, n), dtype=torch.complex64, device="cuda")
+    q = torch.empty(npx, dtype=torch.float32, device="cuda")
+    tc = torch.empty(npx, dtype=torch.float32, device="cuda")
+    bs = 1 << 16
+    st = {}
+
+    def run(tol):
+        x = torch.view_as_real(z)
+        rmli = (x[..., 0] * x[..., 0] + x[..., 1] * x[..., 1]).contiguous()
+        p, m, cnt = mr.ks_test_select_shp(rmli, az, r, 0.05, return_p=True)
+        del rmli, p
+        pcs = m.view(npx, aw, rw)
+        for s0 in range(0, npx, bs):
+            e0 = min(npx, s0 + bs)
+            c = mr.emperical_co_pc(z, idx[s0:e0], pcs[s0:e0])
+            emi_tcoh(c, n, 0, ph[s0:e0], q[s0:e0], tc[s0:e0], tol)
+        st["coh"], st["s0"], st["cnt"] = c, s0, cnt
+
+    # warm-up on a few lines (kernel loading), then one timed pass per form
+    out = {"workload": f"{nl}x{w} tile, N={n}, {aw}x{rw} windows, gamma0~U[0.2,0.9], 2% near-singular, "
+                       "p_max=0.05, all pixels", "pixels": npx}
+    for name, tol in (("plain", None), ("refine_1e-3", 1e-3)):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        run(tol)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        c, s0 = st["coh"], st["s0"]
+        m = min(512, c.shape[0])
+        if "fp64" not in st:
+            G = mr.uncompress_coh(c[:m]).to(torch.complex128)
+            M = torch.linalg.inv(G.abs()).to(torch.complex128) * G
+            _, V = torch.linalg.eigh(M)
+            v = V[..., :, 0] * V[..., 0:1, 0].conj()
+            st["fp64"] = v / v.abs()
+            G32 = G.to(torch.complex64).cpu()
+            _, V32 = torch.linalg.eigh(torch.linalg.inv(G32.abs()).to(torch.complex64) * G32)
+            v32 = (V32[..., :, 0] * V32[..., 0:1, 0].conj()).to(torch.complex128)
+            out["phase_rms_vs_fp64_reference_fp32_lapack"] = _rms_stats(
+                _phase_rms(v32 / v32.abs(), st["fp64"].cpu(), torch), torch)
+        rms = _phase_rms(ph[s0:s0 + m], st["fp64"], torch)
+        nsm = ns.reshape(-1)[s0:s0 + m]
+        out[name] = {"s": dt, "value": npx / dt, "unit": "DS pixels/s", "phase_rms_vs_fp64": _rms_stats(rms, torch),
+                     "near_singular_points_max_rms": float(rms[nsm].max()) if bool(nsm.any()) else None}
+        if tol is not None:
+            out[name]["re_solved_last_batch"] = int(mr.pl.last_refined)
+    out["mean_shp_count"] = float(st["cnt"].float().mean())
+    out["max_shp_count"] = int(st["cnt"].max())
+    return out
 
 
 def serving_leg(args, torch):
@@ -413,7 +510,7 @@ def main():
     ks_bytes = 4 * N + 4 * aw * rw + aw * rw + 4  # rmli in, p out, mask out, count out
     co_flops = 4.0 * kmean * N * (N + 1)  # Hermitian rank-K update (unique entries)
     co_bytes = 8 * P
-    emi_flops = 2.0 * N ** 3 + (16.0 / 3.0) * N ** 3  # FP64 sweep inverse + Hermitian tridiagonalisation
+    emi_flops = emi_algorithmic_flops(N)
     tc_bytes = 8 * P + 8 * N + 4
     stages = {
         "rmli": {"s": t_rmli},
@@ -427,11 +524,14 @@ def main():
     dom = max(["ks_test+select_shp", "emperical_co_pc", "emi+ds_temp_coh"], key=lambda k: stages[k]["s"])
     if dom == "emi+ds_temp_coh":
         ach = emi_flops * (npx / nb) / (t_emi / nb) / 1e12
-        roof = {"kernel": "emi_fast_kernel<64, true, false>", "bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
+        roof = {"kernel": "emi_fast_kernel<64, true, false>", "bound": "valu", "achieved": ach,
+                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
                 "traffic": measured_traffic("emi_fast_kernel<64,tc>", args),
-                "note": "EMI with the DS temporal coherence fused; VALU-bound (FP64 sweep + FP32 Householder); "
-                        "peak = MI355X FP64 = FP32 non-packed vector rate; flops count EMI only"}
+                "executed_TFLOP/s": emi_executed_flops(N) * (npx / nb) / (t_emi / nb) / 1e12,
+                "note": "EMI with the DS temporal coherence fused; VALU-bound (no MFMA); achieved = algorithmic "
+                        "flops N^3 + (4/3)N^3 + 16N^2 per point over the kernel's HIP-event time; "
+                        "executed_TFLOP/s counts what the kernel runs (2N^3 FP64 sweep + (16/3)N^3 FP32 "
+                        "Householder); peak = MI355X FP64 = FP32 non-packed vector rate"}
     elif dom == "ks_test+select_shp":
         ach = ks_bytes * npx / t_ks / 1e9
         roof = {"kernel": "ks_pairs_kernel<float, 4>", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
@@ -457,6 +557,22 @@ def main():
     phase = {"rms_median_rad": float(torch.nanmedian(rms)), "rms_p99_rad": float(torch.nanquantile(rms, 0.99)),
              "rms_max_rad": float(rms[~torch.isnan(rms)].max()),
              "rms_all_rad": float(torch.sqrt(torch.nanmean(rms ** 2))), "points": m}
+    # the reference's own arithmetic on the same points (moraine/pl.py:57-59: float32 inv of |Gamma|,
+    # complex64 eigh; CPU LAPACK through torch) against the same FP64 solve
+    G32 = G.to(torch.complex64).cpu()
+    M32 = torch.linalg.inv(G32.abs()).to(torch.complex64) * G32
+    _, V32 = torch.linalg.eigh(M32)
+    v32 = (V32[..., :, 0] * V32[..., 0:1, 0].conj()).to(torch.complex128)
+    err32 = torch.angle((v32 / v32.abs()) * ph64.conj().cpu())
+    rms32 = torch.sqrt(torch.nanmean(err32 ** 2, dim=1))
+    phase["reference_fp32_lapack"] = {
+        "rms_median_rad": float(torch.nanmedian(rms32)), "rms_p99_rad": float(torch.nanquantile(rms32, 0.99)),
+        "rms_max_rad": float(rms32[~torch.isnan(rms32)].max()),
+        "rms_all_rad": float(torch.sqrt(torch.nanmean(rms32 ** 2))),
+        "note": "float32 inv(|Gamma|) and complex64 eigh (LAPACK, CPU) on the same coherence rows"}
+    phase["gpu_over_reference"] = {
+        "p99": phase["rms_p99_rad"] / phase["reference_fp32_lapack"]["rms_p99_rad"],
+        "max": phase["rms_max_rad"] / phase["reference_fp32_lapack"]["rms_max_rad"]}
     # precision form on the same batch: FP64 re-solve of points whose estimated
     # phase error exceeds tol (emi(..., refine=tol)); error on the same sample
     if 32 < args.n <= 64:
@@ -493,6 +609,9 @@ def main():
         torch.cuda.empty_cache()
     if "streamed" in wanted:
         forms["streamed"] = streamed_leg(args, torch)
+        torch.cuda.empty_cache()
+    if "precision" in wanted:
+        forms["precision"] = config3_leg(args, torch)
         torch.cuda.empty_cache()
     value = npx / dt
     out = {

@@ -231,3 +231,50 @@ def ds_stack_mixed(nl: int, w: int, n: int, seed: int = 0, g0_range=(0.2, 0.9), 
             wv = (rng.standard_normal(n) + 1j * rng.standard_normal(n)) / math.sqrt(2.0)
             z[i, j] = (L @ wv) * np.exp(1j * ph[i, j])
     return z.astype(np.complex64), ns
+
+
+def ds_stack_mixed_torch(nl: int, w: int, n: int, seed: int = 0, g0_range=(0.2, 0.9), near_singular_frac: float = 0.02,
+                         tau: float = 24.0, levels: int = 64, device="cuda", row_block: int = 128):
+    """The ``ds_stack_mixed`` model generated on a torch device (large tiles).
+
+    g0 ~ U[g0_range] per pixel, quantised to ``levels`` values (one Cholesky
+    factor per level); near-singular pixels take Gamma_ik = g0, g0 ~ U[0.98,
+    0.999], in 16 levels.  Linear deformation phase as ``ds_stack``.  Returns
+    (complex64 (nl, w, n) tensor, near-singular bool (nl, w) tensor).
+    """
+    import torch
+
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    lv = np.linspace(g0_range[0], g0_range[1], levels)
+    Ls = [np.linalg.cholesky(coherence_model(n, float(a), tau)) for a in lv]
+    ns_lv = np.linspace(0.98, 0.999, 16)
+    for a in ns_lv:
+        G = np.full((n, n), a)
+        np.fill_diagonal(G, 1.0)
+        Ls.append(np.linalg.cholesky(G))
+    L = torch.from_numpy(np.stack(Ls).astype(np.float32)).to(device)  # (levels + 16, n, n)
+    t = torch.arange(n, device=device, dtype=torch.float32) * DT_DAYS
+    out = torch.empty((nl, w, n), dtype=torch.complex64, device=device)
+    ns_all = torch.empty((nl, w), dtype=torch.bool, device=device)
+    for r0 in range(0, nl, row_block):
+        rows = min(row_block, nl - r0)
+        m = rows * w
+        ns = torch.rand(m, generator=g, device=device) < near_singular_frac
+        li = torch.randint(0, levels, (m,), generator=g, device=device)
+        li = torch.where(ns, levels + torch.randint(0, 16, (m,), generator=g, device=device), li)
+        wr = torch.randn((m, n), generator=g, device=device) / math.sqrt(2.0)
+        wi = torch.randn((m, n), generator=g, device=device) / math.sqrt(2.0)
+        zr = torch.empty_like(wr)
+        zi = torch.empty_like(wi)
+        for lvl in torch.unique(li).tolist():  # one factor per level: z = L w for its pixels
+            sel = (li == lvl).nonzero()[:, 0]
+            Lt = L[lvl].T.contiguous()
+            zr[sel] = wr[sel] @ Lt
+            zi[sel] = wi[sel] @ Lt
+        v = torch.rand(m, generator=g, device=device) * 10.0
+        ph = (4 * math.pi / WAVELENGTH_M) * (v / 1000.0 / 365.25)[:, None] * t
+        c, s = torch.cos(ph), torch.sin(ph)
+        out[r0:r0 + rows] = torch.complex(zr * c - zi * s, zr * s + zi * c).reshape(rows, w, n)
+        ns_all[r0:r0 + rows] = ns.reshape(rows, w)
+    return out, ns_all

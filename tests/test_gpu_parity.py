@@ -380,6 +380,14 @@ def test_emi_ds_vs_fp64(n, win):
     # linked-phase RMS error vs FP64 <= 1e-3 rad (median far below)
     assert np.nanmax(rms) < 1e-3, np.nanmax(rms)
     np.testing.assert_allclose(q, q64, rtol=1e-3, atol=1e-3 * np.abs(q64).max())
+    # the reference's own arithmetic (float32 inv of |Gamma|, complex64 eigh) on the same points:
+    # the GPU's per-point tail is no worse (floor 2e-6 rad: both below it is a tie)
+    ph32, _ = R.emi_ref32(c)
+    rms32 = np.sqrt(np.nanmean(np.angle(ph32 * np.conj(ph64)) ** 2, axis=1))
+    g99, r99 = np.nanquantile(rms, 0.99), np.nanquantile(rms32, 0.99)
+    print(f"  per-point phase RMS vs FP64: GPU p99 {g99:.2e} max {np.nanmax(rms):.2e}; "
+          f"reference FP32 LAPACK p99 {r99:.2e} max {np.nanmax(rms32):.2e}")
+    assert g99 <= max(r99, 2e-6) and np.nanmax(rms) <= max(np.nanmax(rms32), 2e-6)
 
 
 def test_emi_golden_and_ref():

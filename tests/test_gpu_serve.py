@@ -116,3 +116,47 @@ def test_poisson_stream_and_serial_baseline():
     for a, b in zip(res, ser):
         assert torch.equal(torch.view_as_real(a.ph).view(torch.int32), torch.view_as_real(b.ph).view(torch.int32))
     print(stream_stats(ser, specs))
+
+
+def test_config5_shaped_mix_against_oracle(port):
+    """A served mix at the serving workload's shapes: an N=200 request, 21x41 and
+    11x15 windows, tiles of 128^2 and larger.  SHP counts equal the oracle's KS
+    (C restatement of the reference's numba path) bit for bit; sampled points'
+    phases are within 1e-3 rad RMS of FP64 EMI on the same coherence rows."""
+    import torch
+
+    import moraine_amd as mr
+    from moraine_amd import synth
+    from moraine_amd.serve import DSServer, PatchSpec
+    from oracle import moraine_ref as R
+
+    shapes = [(200, 128, 128, 10, 20), (30, 160, 128, 5, 7), (60, 128, 144, 7, 10), (120, 136, 128, 10, 20)]
+    specs, stacks = [], {}
+    for i, (n, h, w, az, r) in enumerate(shapes):
+        z = synth.ds_stack(h, w, n, seed=900 + i, ps_frac=0.1)
+        stacks[i] = z
+        specs.append(PatchSpec(i, n, h, w, az, r, seed=900 + i, rslc=z))
+    _case("serve: config-5-shaped mix (N=200 128x128 21x41, N=30 11x15, N=60 15x21, N=120 21x41)")
+    srv = DSServer(coh_budget=1 << 30, max_tick_px=1 << 15)
+    for s in specs:
+        srv.submit(s)
+    srv.drain()
+    torch.cuda.synchronize()
+    res = {x.rid: x for x in srv.results}
+    assert sorted(res) == list(range(len(specs)))
+    rng = np.random.default_rng(5)
+    for s in specs:
+        z = stacks[s.rid]
+        rmli = z.real.astype(np.float32) ** 2 + z.imag.astype(np.float32) ** 2
+        _case(f"serve: request {s.rid} N={s.n} {s.h}x{s.w} window {2*s.az+1}x{2*s.r+1} against the oracle")
+        p_ref = port.ks_test(rmli, s.az, s.r)
+        m_ref, n_ref = R.select_shp(p_ref, 0.05)
+        np.testing.assert_array_equal(res[s.rid].shp_num.cpu().numpy().reshape(s.h, s.w), n_ref)
+        pts = rng.choice(s.npx, 48, replace=False)
+        idx = np.stack([pts // s.w, pts % s.w], -1).astype(np.int32)
+        c = mr.emperical_co_pc(z, idx, m_ref[idx[:, 0], idx[:, 1]])
+        ph64, _ = R.emi64(c)
+        ph = res[s.rid].ph.reshape(s.npx, s.n).cpu().numpy()[pts]
+        rms = np.sqrt(np.nanmean(np.angle(ph * np.conj(ph64)) ** 2, axis=1))
+        print(f"  phase RMS vs FP64 on 48 points: max {np.nanmax(rms):.2e}")
+        assert np.nanmax(rms) < 1e-3
