@@ -188,6 +188,27 @@ int mr_regularize_spectral(const float* coh, long long n_mat, int n, const doubl
 int mr_regularize_spectral_c128(const double* coh, long long n_mat, int n, const double* beta,
                                 int beta_per_matrix, double* out, mr_stream_t stream);
 
+/* isPD (moraine/co.py:409-414) and nearestPD (moraine/co.py:429-463) for a
+ * stack of n_mat real n x n matrices (float32 or float64, row-major),
+ * 1 <= n <= 1024.  isPD: is_pd[i] = 1 when the Cholesky factorisation of
+ * matrix i succeeds (FP64, every pivot positive and finite), as the
+ * reference's CuPy path reports it.  nearestPD: Higham's method as the
+ * reference writes it, its undefined `wherePD` read as isPD: B = (A + A^T)/2,
+ * H = V S V^T from the SVD of A, A3 = sym((B + H)/2), then while A3 (rounded
+ * to the output dtype) is not PD and k < 100: A3 += (-lambda_min(A3) k^2 +
+ * spacing) I, spacing = the gap from ||A||_F to the next number of the dtype.
+ * FP64 throughout (one-sided Jacobi SVD), output in the input's dtype (out may
+ * not alias a).  workspace >= mr_pd_workspace_bytes(n_mat, n). */
+size_t mr_pd_workspace_bytes(long long n_mat, int n);
+int mr_is_pd_f32(const float* a, long long n_mat, int n, uint8_t* is_pd, void* workspace,
+                 size_t workspace_bytes, mr_stream_t stream);
+int mr_is_pd_f64(const double* a, long long n_mat, int n, uint8_t* is_pd, void* workspace,
+                 size_t workspace_bytes, mr_stream_t stream);
+int mr_nearest_pd_f32(const float* a, long long n_mat, int n, float* out, void* workspace,
+                      size_t workspace_bytes, mr_stream_t stream);
+int mr_nearest_pd_f64(const double* a, long long n_mat, int n, double* out, void* workspace,
+                      size_t workspace_bytes, mr_stream_t stream);
+
 /* uncompress_coh (moraine/co.py:281-299): (n_pts, n_pairs) -> (n_pts, n, n),
  * [a,b] = c, [b,a] = conj(c), diagonal 1, other entries 0, resolved in the
  * reference's order when entries collide: all pairs, then all conjugates, a

@@ -125,6 +125,43 @@ def regularize_spectral(coh, beta):
     return _dev.back(out, as_torch)
 
 
+def _pd_call(co, nearest: bool):
+    as_torch = _dev.is_torch(co)
+    a = _dev.to_dev(co)
+    if a.is_complex():
+        raise TypeError("isPD/nearestPD: real matrices expected (the reference passes |coherence|)")
+    if a.dtype != torch.float64:
+        a = a.to(torch.float32)
+    if a.ndim < 2 or a.shape[-1] != a.shape[-2]:
+        raise ValueError("isPD/nearestPD: co must be a stack of square matrices (..., N, N)")
+    a = a.contiguous()
+    n = a.shape[-1]
+    lead = tuple(a.shape[:-2])
+    nmat = int(np.prod(lead)) if lead else 1
+    ws = _dev.workspace(_lib.query("mr_pd_workspace_bytes", nmat, n), a.device)
+    sfx = "f64" if a.dtype == torch.float64 else "f32"
+    if nearest:
+        out = torch.empty_like(a)
+        _lib.call(f"mr_nearest_pd_{sfx}", _dev.ptr(a), nmat, n, _dev.ptr(out), _dev.ptr(ws), ws.numel(), _dev.stream())
+    else:
+        out = torch.empty(lead, dtype=torch.bool, device=a.device)
+        _lib.call(f"mr_is_pd_{sfx}", _dev.ptr(a), nmat, n, _dev.ptr(out), _dev.ptr(ws), ws.numel(), _dev.stream())
+    return _dev.back(out, as_torch)
+
+
+def isPD(co):
+    """Whether each matrix of a real stack (..., N, N) is positive definite (its
+    Cholesky factorisation succeeds, FP64): bool (...) (moraine/co.py:409-414)."""
+    return _pd_call(co, False)
+
+
+def nearestPD(co):
+    """Nearest positive-definite matrix of each matrix of a real stack (..., N, N)
+    by Higham's method as moraine/co.py:429-463 writes it (its undefined
+    ``wherePD`` read as isPD); FP64 arithmetic, the input's dtype out."""
+    return _pd_call(co, True)
+
+
 def uncompress_coh(coh, image_pairs=None):
     """(..., n_pairs) compressed -> (..., n, n) Hermitian with unit diagonal, coh's dtype."""
     as_torch = _dev.is_torch(coh)

@@ -35,6 +35,12 @@ emi64                       closure-free Gamma: ph = theta*conj(theta_ref)/|.|,
 emi_ref32                   same closure-free case at FP32 tolerance
 ds_temp_coh / temp_coh      t = 1 for consistent inputs; hand-computed 3-image
                             case; FP64 variant agreement
+isPD / nearestPD            Cholesky success on PD / indefinite matrices;
+                            Higham's theorem: symmetric input -> Q max(L, 0) Q^T
+                            (+ a shift of the order of the spacing), PD input
+                            unchanged, output symmetric PD, no random PSD matrix
+                            closer to a symmetric input (the reference's call is
+                            undefined: parity pinned by mathematics only)
 ad_intf_pc64 / temp_coh64   closed forms (sec = ref e^{i phi} -> e^{-i phi};
                             consistent phases -> t = 1) and agreement with the
                             complex64 restatements on complex64 data
@@ -112,6 +118,50 @@ def ad_intf_pc64(ref_rslc, sec_rslc, idx, pc_is_shp) -> np.ndarray:
         with np.errstate(invalid="ignore", divide="ignore"):
             out[i] = np.sum(x * np.conj(y)) / np.sqrt(np.sum(np.abs(x) ** 2) * np.sum(np.abs(y) ** 2))
     return out
+
+
+def isPD(co) -> np.ndarray:
+    """Restates ``isPD`` (moraine/co.py:409-414) with the CuPy path's semantics
+    (a failed factorisation gives a non-finite factor, not an exception):
+    True where the Cholesky factorisation of the matrix succeeds (FP64)."""
+    co = np.asarray(co, np.float64)
+    out = np.empty(co.shape[:-2], bool)
+    for i in np.ndindex(out.shape):
+        try:
+            out[i] = bool(np.isfinite(np.linalg.cholesky(co[i])).all())
+        except np.linalg.LinAlgError:
+            out[i] = False
+    return out
+
+
+def nearestPD(co) -> np.ndarray:
+    """Restates ``nearestPD`` (moraine/co.py:429-463), Higham's method, in FP64
+    with the undefined ``wherePD`` read as ``isPD``: B = (A + A^T)/2; A = U S Vh
+    (SVD of the input, as the reference takes it); H = Vh^T S Vh; A3 = the
+    symmetric part of (B + H)/2; while some matrix is not PD (k < 100): k += 1,
+    A3 += (-lambda_min(A3) k^2 + spacing) I on those matrices, spacing the
+    distance from ||A||_F to the next number of the input's dtype.  Returns the
+    input's dtype."""
+    a = np.asarray(co)
+    dt = a.dtype if a.dtype in (np.float32, np.float64) else np.float64
+    A = a.astype(np.float64)
+    B = (A + np.swapaxes(A, -1, -2)) / 2
+    _, s, Vh = np.linalg.svd(A)
+    H = np.swapaxes(Vh, -1, -2) @ (s[..., :, None] * Vh)
+    A2 = (B + H) / 2
+    A3 = (A2 + np.swapaxes(A2, -1, -2)) / 2
+    I = np.eye(A.shape[-1])
+    nrm = np.linalg.norm(a.astype(dt), axis=(-2, -1)).astype(dt)
+    spacing = (np.nextafter(nrm, np.asarray(np.inf, dt)) - nrm).astype(np.float64)
+    k = 0
+    while True:
+        pd = isPD(A3.astype(dt))
+        if pd.all() or k >= 100:
+            break
+        k += 1
+        mineig = np.min(np.linalg.eigvalsh(A3), axis=-1)
+        A3 = A3 + (~pd)[..., None, None] * I * (-mineig * k ** 2 + spacing)[..., None, None]
+    return A3.astype(dt)
 
 
 def regularize_spectral(coh, beta) -> np.ndarray:

@@ -387,3 +387,40 @@ def test_fp64_variants_closed_forms_and_agreement():
     f2 = (rng.standard_normal(f.shape) + 1j * rng.standard_normal(f.shape)).astype(np.complex64)
     th2 = th.astype(np.complex64)
     np.testing.assert_allclose(R.temp_coh64(f2, th2), R.temp_coh(f2, th2), atol=2e-6)
+
+
+def test_nearest_pd_higham_properties():
+    """nearestPD (moraine/co.py:429-463; its wherePD is undefined, read as isPD):
+    pinned by Higham's theorem.  For symmetric A = Q L Q^T the nearest PSD matrix
+    in the Frobenius norm is Q max(L, 0) Q^T; the result adds at most a shift of
+    the order of the spacing and is PD; PD input comes back unchanged; no random
+    PSD matrix is closer; non-symmetric input gives a symmetric PD result."""
+    rng = np.random.default_rng(4)
+    n = 12
+    X = rng.standard_normal((5, n, n))
+    S = (X + np.swapaxes(X, -1, -2)) / 2  # symmetric indefinite
+    assert not R.isPD(S).any()
+    out = R.nearestPD(S)
+    assert R.isPD(out).all()
+    np.testing.assert_array_equal(out, np.swapaxes(out, -1, -2))
+    lam, Q = np.linalg.eigh(S)
+    clip = Q @ (np.maximum(lam, 0)[..., :, None] * np.swapaxes(Q, -1, -2))
+    nrm = np.linalg.norm(S, axis=(-2, -1))
+    assert (np.linalg.norm(out - clip, axis=(-2, -1)) < 1e-12 * nrm).all()
+    for _ in range(20):  # optimality against random PSD matrices
+        Y = rng.standard_normal((5, n, n))
+        P = Y @ np.swapaxes(Y, -1, -2) * rng.uniform(0.01, 1.0)
+        assert (np.linalg.norm(S - out, axis=(-2, -1)) <= np.linalg.norm(S - P, axis=(-2, -1)) + 1e-12).all()
+    # PD input: unchanged to rounding
+    Apd = X @ np.swapaxes(X, -1, -2) + n * np.eye(n)
+    assert R.isPD(Apd).all()
+    np.testing.assert_allclose(R.nearestPD(Apd), Apd, rtol=1e-13, atol=1e-13 * np.abs(Apd).max())
+    # non-symmetric: symmetric and PD; float32 keeps its dtype
+    o2 = R.nearestPD(X.astype(np.float32))
+    assert o2.dtype == np.float32 and R.isPD(o2).all()
+    np.testing.assert_array_equal(o2, np.swapaxes(o2, -1, -2))
+    # a planted mistake (the polar factor of B instead of A) is seen on non-symmetric input
+    B = (X + np.swapaxes(X, -1, -2)) / 2
+    _, s, Vh = np.linalg.svd(B)
+    Hb = np.swapaxes(Vh, -1, -2) @ (s[..., :, None] * Vh)
+    assert np.abs(R.nearestPD(X) - (B + Hb) / 2).max() > 1e-3
