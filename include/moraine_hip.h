@@ -47,13 +47,14 @@ int mr_abi_version(void);
  *   mask_out  same shape, uint8 (0/1), or NULL                   (is_shp)
  *   shp_num   (nl, w) int32, or NULL                             (shp_num)
  *   p_max     SHP threshold used for mask_out/shp_num (is_shp = p < p_max)
- *   workspace >= mr_ks_workspace_bytes(nl, w, n) bytes of device memory
+ *   workspace >= mr_ks_workspace_bytes(nl, w, n, az, r, sizeof(element)) bytes
+ *             of device memory (sorted keys, NaN flags, walk results)
  * n must be 1..MR_KS_MAX_N.  p and dist are bit-exact with the reference CPU
  * path: p = dtype(ks_p_f64(lambda(k))) with k = N*D the integer merge-walk step
  * count.
  * ------------------------------------------------------------------------- */
 #define MR_KS_MAX_N 256
-size_t mr_ks_workspace_bytes(int nl, int w, int n);
+size_t mr_ks_workspace_bytes(int nl, int w, int n, int az, int r, int elem_bytes);
 int mr_ks_test_f32(const float* rmli, int nl, int w, int n, int az, int r, double p_max,
                    float* p_out, float* dist_out, uint8_t* mask_out, int32_t* shp_num,
                    void* workspace, size_t workspace_bytes, mr_stream_t stream);
@@ -68,7 +69,7 @@ int mr_ks_test_f64(const double* rmli, int nl, int w, int n, int az, int r, doub
  * sorted-key workspace is the library's static device scratch of
  * MR_KS_NOALLOC_SCRATCH_BYTES per GPU (allocated when the library's code is
  * loaded on the GPU, held until the process ends): a stack needs
- * align256(4 nl w n) + align256(nl w) bytes of it, and a larger one is
+ * mr_ks_workspace_bytes(nl, w, n, az, r, 4) bytes of it, and a larger one is
  * rejected with MR_ERR_UNSUPPORTED before any device work.  Nothing is
  * allocated and nothing needs clearing: every call writes the scratch before
  * reading it.  All calls on one GPU share its scratch, so calls that overlap on

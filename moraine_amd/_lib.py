@@ -19,7 +19,7 @@ _vp, _i, _ll, _d, _sz = _c.c_void_p, _c.c_int, _c.c_longlong, _c.c_double, _c.c_
 SIGNATURES = {
     "mr_last_error_string": (_c.c_char_p, []),
     "mr_abi_version": (_i, []),
-    "mr_ks_workspace_bytes": (_sz, [_i, _i, _i]),
+    "mr_ks_workspace_bytes": (_sz, [_i, _i, _i, _i, _i, _i]),
     "mr_ks_test_f32": (_i, [_vp, _i, _i, _i, _i, _i, _d, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
     "mr_ks_test_f64": (_i, [_vp, _i, _i, _i, _i, _i, _d, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
     "mr_ks_test_f32_noalloc": (_i, [_vp, _i, _i, _i, _i, _i, _d, _vp, _vp, _vp, _vp, _vp]),

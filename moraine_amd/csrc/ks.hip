@@ -263,38 +263,61 @@ __device__ __forceinline__ void stage_stacks(Key* __restrict__ dst, const Key* _
   for (int s = threadIdx.x; s < nstk; s += KS_THREADS) dst[s * ns + n] = ~(Key)0;  // sentinel
 }
 
-template <typename T, int KS_ILP>
+// Walk results: the integer step count k = N*D of each walked (pixel, offset)
+// pair, one element per entry of the half window the pair kernel walks, in
+// the layout kbuf[line][l][column][m] (l = 0..az row offsets below the pixel,
+// m = 0..rw-1 range offsets; for l = 0 only m >= r is walked).  KB is uint8
+// for n <= 254 and uint16 above; its all-ones value marks a pair the
+// reference writes as NaN (a NaN stack or a partner outside the raster).
+template <typename KB>
+__host__ __device__ constexpr KB ks_invalid() {
+  return (KB)~(KB)0;
+}
+
+// XCD-aware tile order: the hardware hands workgroup i to XCD i % 8.  The
+// (tile, line) units are ordered tile-column-major (every line of tile 0, then
+// of tile 1, ...) and XCD x runs the x-th of eight equal contiguous chunks of
+// that order, so all XCDs get the same number of full-width tiles and each
+// walks down its tile columns line by line: the stacks a block stages (its
+// line and the az lines below, tile plus range halo) were staged by the
+// blocks of the lines just above on the same XCD and sit in its L2.  Returns
+// false for the padding workgroups of the last chunk.
+constexpr int KS_XCDS = 8;
+__device__ __forceinline__ bool ks_tile_of(int ntx, int nl, long long chunk, int& tile, int& line) {
+  const int x = blockIdx.x % KS_XCDS;
+  const long long j = blockIdx.x / KS_XCDS;
+  const long long f = x * chunk + j;
+  if (j >= chunk || f >= (long long)ntx * nl) return false;
+  tile = (int)(f / nl);
+  line = (int)(f - (long long)tile * nl);
+  return true;
+}
+
+template <typename T, int KS_ILP, typename KB>
 __global__ __launch_bounds__(KS_THREADS) void ks_pairs_kernel(const typename KeyOf<T>::type* __restrict__ keys,
                                                               const uint8_t* __restrict__ valid, int nl, int w,
-                                                              int n, int az, int r, int tw, KsTables<T> tab,
-                                                              T* __restrict__ p_out, T* __restrict__ d_out,
-                                                              uint8_t* __restrict__ m_out,
-                                                              int32_t* __restrict__ shp_num) {
+                                                              int n, int az, int r, int tw, int ntx,
+                                                              long long chunk, KB* __restrict__ kbuf) {
   using Key = typename KeyOf<T>::type;
   extern __shared__ __align__(16) unsigned char smem[];
-  const int ns = n + 1;
-  const int rw = 2 * r + 1, aw = 2 * az + 1;
-  const int a = blockIdx.y;
-  const int c0 = blockIdx.x * tw;
+  int tile, a;
+  if (!ks_tile_of(ntx, nl, chunk, tile, a)) return;
+  const int c0 = tile * tw;
   const int tcols = min(tw, w - c0);
+  const int ns = n + 1;
+  const int rw = 2 * r + 1;
   const int scols = tcols + 2 * r;
   Key* ref = reinterpret_cast<Key*>(smem);
   Key* sec = ref + tw * ns;
-  int* cnt = reinterpret_cast<int*>(sec + (tw + 2 * r) * ns);
-  uint8_t* refv = reinterpret_cast<uint8_t*>(cnt + tw);
+  uint8_t* refv = reinterpret_cast<uint8_t*>(sec + (tw + 2 * r) * ns);
   uint8_t* secv = refv + tw;
 
   stage_stacks(ref, keys + ((size_t)a * w + c0) * n, tcols, n, ns);
-  for (int c = threadIdx.x; c < tw; c += KS_THREADS) {
-    refv[c] = c < tcols ? valid[(size_t)a * w + c0 + c] : 0;
-    cnt[c] = 0;
-  }
+  for (int c = threadIdx.x; c < tw; c += KS_THREADS) refv[c] = c < tcols ? valid[(size_t)a * w + c0 + c] : 0;
   // The KS distance of two pixels does not depend on which one is the
   // reference (the walk is symmetric), so each unordered pair is walked once:
   // offsets (l, m - r) with l > 0, or l = 0 and m >= r (m = r is the pixel
-  // itself).  The result goes to this pixel's entry and, mirrored, to the
-  // partner's entry for the opposite offset; a mirrored entry whose source
-  // pixel lies outside the raster gets NaN from this block.
+  // itself).  The emit kernel reads the result for both pixels.
   for (int l = 0; l <= az; ++l) {
     __syncthreads();
     const int sa = a + l;
@@ -315,10 +338,11 @@ __global__ __launch_bounds__(KS_THREADS) void ks_pairs_kernel(const typename Key
     const int m0 = l == 0 ? r : 0;
     const int mc = rw - m0;
     const int npairs = tcols * mc;
+    KB* kb = kbuf + (((size_t)a * (az + 1) + l) * w + c0) * rw;
     for (int t0 = threadIdx.x; t0 < npairs; t0 += KS_THREADS * KS_ILP) {
       int ia[KS_ILP], ib[KS_ILP], ra[KS_ILP], rb[KS_ILP], km[KS_ILP], cc[KS_ILP], mm[KS_ILP];
       bool ok[KS_ILP], in[KS_ILP];
-      Key ka[KS_ILP], kb[KS_ILP];
+      Key ka[KS_ILP], kb_[KS_ILP];
 #pragma unroll
       for (int q = 0; q < KS_ILP; ++q) {
         const int t = t0 + q * KS_THREADS;
@@ -334,7 +358,7 @@ __global__ __launch_bounds__(KS_THREADS) void ks_pairs_kernel(const typename Key
         ib[q] = 0;
         km[q] = 0;
         ka[q] = ref[ra[q]];
-        kb[q] = ref[rb[q]];
+        kb_[q] = ref[rb[q]];
       }
       // merge walk of moraine/shp.py:35-44 on keys: '<=' advances ref, '>=' advances sec
       for (int it = 0; it < 2 * n; it += 4) {
@@ -342,14 +366,14 @@ __global__ __launch_bounds__(KS_THREADS) void ks_pairs_kernel(const typename Key
         for (int u = 0; u < 4; ++u) {
 #pragma unroll
           for (int q = 0; q < KS_ILP; ++q) {
-            const int le = ka[q] <= kb[q];
-            const int ge = ka[q] >= kb[q];
+            const int le = ka[q] <= kb_[q];
+            const int ge = ka[q] >= kb_[q];
             ia[q] = min(ia[q] + le, n);
             ib[q] = min(ib[q] + ge, n);
             const int d = ib[q] - ia[q];
             km[q] = max(km[q], max(d, -d));
             ka[q] = ref[ra[q] + ia[q]];
-            kb[q] = ref[rb[q] + ib[q]];
+            kb_[q] = ref[rb[q] + ib[q]];
           }
         }
         bool alive = false;
@@ -358,46 +382,114 @@ __global__ __launch_bounds__(KS_THREADS) void ks_pairs_kernel(const typename Key
         if (!__any(alive)) break;
       }
 #pragma unroll
-      for (int q = 0; q < KS_ILP; ++q) {
-        if (!in[q]) continue;
-        const int k = km[q];
-        const int col = c0 + cc[q];
-        const T pv = ok[q] ? tab.p[k] : (T)NAN;
-        const T dv = ok[q] ? tab.d[k] : (T)NAN;
-        const bool s = ok[q] && ((tab.shp[k >> 5] >> (k & 31)) & 1u);
-        const size_t o = (((size_t)a * w + col) * aw + (l + az)) * rw + mm[q];
-        if (p_out) p_out[o] = pv;
-        if (d_out) d_out[o] = dv;
-        if (m_out) m_out[o] = s ? 1 : 0;
-        if (s) atomicAdd(&cnt[cc[q]], 1);
-        if (l == 0 && mm[q] == r) continue;  // the pixel itself
-        const int mo = rw - 1 - mm[q];      // index of the opposite offset
-        const int col2 = col + mm[q] - r;   // partner pixel (a + l, col2)
-        if (sa < nl && col2 >= 0 && col2 < w) {
-          const size_t o2 = (((size_t)sa * w + col2) * aw + (az - l)) * rw + mo;
-          if (p_out) p_out[o2] = pv;
-          if (d_out) d_out[o2] = dv;
-          if (m_out) m_out[o2] = s ? 1 : 0;
-          if (s && shp_num) atomicAdd(&shp_num[(size_t)sa * w + col2], 1);
-        }
-        const int cs = col - (mm[q] - r);  // source pixel of this pixel's opposite offset: (a - l, cs)
-        if (a - l < 0 || cs < 0 || cs >= w) {
-          const size_t o3 = (((size_t)a * w + col) * aw + (az - l)) * rw + mo;
-          if (p_out) p_out[o3] = (T)NAN;
-          if (d_out) d_out[o3] = (T)NAN;
-          if (m_out) m_out[o3] = 0;
+      for (int q = 0; q < KS_ILP; ++q)
+        if (in[q]) kb[cc[q] * rw + mm[q]] = ok[q] ? (KB)km[q] : ks_invalid<KB>();
+    }
+  }
+}
+
+// Emit: one block per (line, column tile) writes its pixels' whole p, dist and
+// mask rows (W contiguous entries per pixel, consecutive threads on
+// consecutive entries) and SHP counts.  Entry (lw, m) of pixel (a, c):
+//   lw >= az (own walk, l = lw - az):  kbuf[a][l][c][m]  for l > 0 or m >= r,
+//     kbuf[a][0][c + m - r][rw - 1 - m] for l = 0, m < r (the left neighbour's walk);
+//   lw < az (l = az - lw): the pixel (a - l, c + m - r) walked it as its offset
+//     (l, rw - 1 - m): kbuf[a - l][l][c + m - r][rw - 1 - m];
+// a source outside the raster gives NaN, as the reference writes.
+constexpr int KS_EMIT_THREADS = 256;
+constexpr int KS_EMIT_TILE = 32;
+constexpr int KS_EMIT_U = 4;
+constexpr int KS_EMIT_MAX_W = 4096;  // window entries the per-block offset table holds
+template <typename T, typename KB>
+__global__ __launch_bounds__(KS_EMIT_THREADS) void ks_emit_kernel(const KB* __restrict__ kbuf, int nl, int w, int az,
+                                                                  int r, KsTables<T> tab, T* __restrict__ p_out,
+                                                                  T* __restrict__ d_out, uint8_t* __restrict__ m_out,
+                                                                  int32_t* __restrict__ shp_num) {
+  __shared__ int cnt[KS_EMIT_TILE];
+  __shared__ int e_src[KS_EMIT_MAX_W];         // window entry -> ((column shift + 2048) << 12) | sm
+  __shared__ uint16_t e_row[KS_EMIT_MAX_W];    // window entry -> l
+  __shared__ T t_p[MR_KS_MAX_N + 1], t_d[MR_KS_MAX_N + 1];
+  __shared__ uint32_t t_s[(MR_KS_MAX_N + 32) / 32];
+  const int a = blockIdx.y;
+  const int c0 = blockIdx.x * KS_EMIT_TILE;
+  const int tcols = min(KS_EMIT_TILE, w - c0);
+  const int rw = 2 * r + 1, aw = 2 * az + 1;
+  const int W = aw * rw;
+  if (threadIdx.x < KS_EMIT_TILE) cnt[threadIdx.x] = 0;
+  constexpr int NT = (int)(sizeof(tab.p) / sizeof(T));
+  for (int i = threadIdx.x; i < NT; i += KS_EMIT_THREADS) {
+    t_p[i] = tab.p[i];
+    t_d[i] = tab.d[i];
+  }
+  for (int i = threadIdx.x; i < (int)(sizeof(tab.shp) / 4); i += KS_EMIT_THREADS) t_s[i] = tab.shp[i];
+  // per window entry (lw, m): the row offset l of the walk that holds it, the
+  // column shift of the walking pixel and the walk's range index sm; the line
+  // of the walk is a for l = lw - az >= 0, a - l for entries above the pixel
+  for (int e = threadIdx.x; e < W; e += KS_EMIT_THREADS) {
+    const int lw = e / rw, m = e - lw * rw;
+    int l, dc, sm;
+    if (lw > az || (lw == az && m >= r)) {  // the pixel's own walk
+      l = lw - az;
+      dc = 0;
+      sm = m;
+    } else {  // the left neighbour (same line) or the pixel above walked it, mirrored
+      l = az - lw;
+      dc = m - r;
+      sm = rw - 1 - m;
+    }
+    e_src[e] = ((dc + 2048) << 12) | sm;
+    e_row[e] = (uint16_t)l;
+  }
+  __syncthreads();
+  const size_t plane = (size_t)w * rw;  // one (line, l) plane of kbuf
+  const size_t obase = ((size_t)a * w + c0) * W;
+  // consecutive threads on consecutive entries of the tile's contiguous rows,
+  // KS_EMIT_U entries per thread per trip with their gathers in flight together;
+  // (c, e) advance by the block size without divisions
+  const int qd = KS_EMIT_THREADS / W, qr = KS_EMIT_THREADS - (KS_EMIT_THREADS / W) * W;
+  int c = threadIdx.x / W, e = threadIdx.x - (threadIdx.x / W) * W;
+  const int tot = tcols * W;
+  for (int t0 = threadIdx.x; t0 < tot; t0 += KS_EMIT_U * KS_EMIT_THREADS) {
+    int cc[KS_EMIT_U], kk[KS_EMIT_U];
+    KB v[KS_EMIT_U];
+#pragma unroll
+    for (int u = 0; u < KS_EMIT_U; ++u) {
+      const int t = t0 + u * KS_EMIT_THREADS;
+      cc[u] = c;
+      kk[u] = -1;  // -1: no source (outside the raster or beyond the tile)
+      v[u] = ks_invalid<KB>();
+      if (t < tot) {
+        const int code = e_src[e];
+        const int l = e_row[e], dc = (code >> 12) - 2048, sm = code & 4095;
+        const bool own = (e >= az * rw + r);  // lw > az, or lw == az and m >= r
+        const int line = own || l == 0 ? a : a - l;
+        const int src = c0 + c + dc;
+        if (line >= 0 && src >= 0 && src < w) {
+          kk[u] = 0;
+          v[u] = kbuf[((size_t)line * (az + 1) + l) * plane + (size_t)src * rw + sm];
         }
       }
+      c += qd;
+      e += qr;
+      if (e >= W) {
+        e -= W;
+        ++c;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < KS_EMIT_U; ++u) {
+      const int t = t0 + u * KS_EMIT_THREADS;
+      if (t >= tot) break;
+      const int k = (kk[u] == 0 && v[u] != ks_invalid<KB>()) ? (int)v[u] : -1;
+      const bool s = k >= 0 && ((t_s[k >> 5] >> (k & 31)) & 1u);
+      if (p_out) p_out[obase + t] = k >= 0 ? t_p[k] : (T)NAN;
+      if (d_out) d_out[obase + t] = k >= 0 ? t_d[k] : (T)NAN;
+      if (m_out) m_out[obase + t] = s ? 1 : 0;
+      if (s && shp_num) atomicAdd(&cnt[cc[u]], 1);
     }
   }
   __syncthreads();
-  if (shp_num)
-    for (int c = threadIdx.x; c < tcols; c += KS_THREADS) atomicAdd(&shp_num[(size_t)a * w + c0 + c], cnt[c]);
-}
-
-__global__ void fill_i32_kernel(int32_t* __restrict__ x, long long n, int32_t v) {
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-    x[i] = v;
+  if (shp_num && threadIdx.x < tcols) shp_num[(size_t)a * w + c0 + threadIdx.x] = cnt[threadIdx.x];
 }
 
 // select_shp on an existing p array: one wave per pixel.
@@ -423,9 +515,14 @@ __global__ __launch_bounds__(256) void select_shp_kernel(const T* __restrict__ p
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-// Workspace of ks_test_impl: sorted keys (sizeof(Key) per sample) + one NaN flag per pixel.
-static size_t ks_workspace_need(size_t npx, int n, size_t key_bytes) {
-  return align256(npx * (size_t)n * key_bytes) + align256(npx);
+// Walk-result element of n images: one byte up to n = 254 (k <= n, all ones = NaN), two above.
+static size_t ks_kb_bytes(int n) { return n <= 254 ? 1 : 2; }
+
+// Workspace of ks_test_impl: sorted keys (key_bytes per sample), one NaN flag
+// per pixel, and the walk results ((az + 1) x rw elements per pixel).
+static size_t ks_workspace_need(size_t npx, int n, int az, int r, size_t key_bytes) {
+  return align256(npx * (size_t)n * key_bytes) + align256(npx) +
+         align256(npx * (size_t)(az + 1) * (2 * (size_t)r + 1) * ks_kb_bytes(n));
 }
 
 // Walks per thread for a window row of `pairs` (pixel, offset) walks per block:
@@ -453,11 +550,12 @@ static int ks_test_impl(const T* rmli, int nl, int w, int n, int az, int r, doub
   }
   MR_CHECK_ARG(az >= 0 && r >= 0, "ks_test: negative half window");
   const size_t npx = (size_t)nl * w;
-  const size_t need = ks_workspace_need(npx, n, sizeof(Key));
+  const size_t need = ks_workspace_need(npx, n, az, r, sizeof(Key));
   MR_CHECK_ARG(ws_bytes >= need, "ks_test: workspace %zu < %zu bytes", ws_bytes, need);
   hipStream_t st = as_stream(stream_);
   Key* keys = reinterpret_cast<Key*>(ws);
   uint8_t* valid = reinterpret_cast<uint8_t*>(ws) + align256(npx * n * sizeof(Key));
+  void* kbuf = valid + align256(npx);
 
   const int nb = (int)std::min<size_t>(div_up(npx, 4), 65536);
   if constexpr (sizeof(T) == 4) {
@@ -486,7 +584,7 @@ static int ks_test_impl(const T* rmli, int nl, int w, int n, int az, int r, doub
 
   // column tile: as wide as 64 pixels while LDS stays <= 48 KiB (>= 3 blocks/CU)
   const size_t stk = (size_t)(n + 1) * sizeof(Key);
-  auto lds_for = [&](int tw) { return (2 * (size_t)tw + 2 * r) * stk + (size_t)tw * 4 + 2 * (size_t)tw + 2 * r + 16; };
+  auto lds_for = [&](int tw) { return (2 * (size_t)tw + 2 * r) * stk + 2 * (size_t)tw + 2 * r + 16; };
   // one trip of four walks per thread covers a window row when tw * rw <= 4 * KS_THREADS: a narrower tile
   // than 64 columns then also leaves LDS for more blocks per CU
   int tw = std::max(1, std::min(64, 4 * KS_THREADS / (2 * r + 1)));
@@ -497,21 +595,48 @@ static int ks_test_impl(const T* rmli, int nl, int w, int n, int az, int r, doub
     set_error("ks_test: window too large for LDS (n=%d, r_half=%d)", n, r);
     return MR_ERR_UNSUPPORTED;
   }
+  // pass 1: the walks, XCD-chunked tile order (ks_tile_of); pass 2: whole output rows
+  const int ntx = (int)div_up(w, tw);
+  const long long chunk = ((long long)ntx * nl + KS_XCDS - 1) / KS_XCDS;  // (tile, line) units per XCD
+  const long long nwg = KS_XCDS * chunk;
+  if ((2 * az + 1) * (2 * r + 1) > KS_EMIT_MAX_W) {
+    set_error("ks_test: window %d x %d above %d entries", 2 * az + 1, 2 * r + 1, KS_EMIT_MAX_W);
+    return MR_ERR_UNSUPPORTED;
+  }
+  if (nwg >= (1ll << 31)) {
+    set_error("ks_test: raster too large (%d x %d)", nl, w);
+    return MR_ERR_UNSUPPORTED;
+  }
+  const bool ilp6 = ks_pick_ilp(tw * (2 * r + 1)) == 6;
+  const dim3 g1((unsigned)nwg);
+  if (ks_kb_bytes(n) == 1) {
+    uint8_t* kb = reinterpret_cast<uint8_t*>(kbuf);
+    if (ilp6)
+      hipLaunchKernelGGL((ks_pairs_kernel<T, 6, uint8_t>), g1, dim3(KS_THREADS), lds, st, keys, valid, nl, w, n, az,
+                         r, tw, ntx, chunk, kb);
+    else
+      hipLaunchKernelGGL((ks_pairs_kernel<T, 4, uint8_t>), g1, dim3(KS_THREADS), lds, st, keys, valid, nl, w, n, az,
+                         r, tw, ntx, chunk, kb);
+  } else {
+    uint16_t* kb = reinterpret_cast<uint16_t*>(kbuf);
+    if (ilp6)
+      hipLaunchKernelGGL((ks_pairs_kernel<T, 6, uint16_t>), g1, dim3(KS_THREADS), lds, st, keys, valid, nl, w, n,
+                         az, r, tw, ntx, chunk, kb);
+    else
+      hipLaunchKernelGGL((ks_pairs_kernel<T, 4, uint16_t>), g1, dim3(KS_THREADS), lds, st, keys, valid, nl, w, n,
+                         az, r, tw, ntx, chunk, kb);
+  }
+  if (int rc2 = launch_status("ks_pairs")) return rc2;
   KsTables<T> tab;
   build_tables<T>(n, p_max, tab);
-  if (shp_num) {  // counts accumulate with atomics (a fill kernel, not a memset: safe under graph capture)
-    hipLaunchKernelGGL(fill_i32_kernel, dim3((unsigned)std::min<size_t>(div_up(npx, 256), 65536)), dim3(256), 0, st,
-                       shp_num, (long long)npx, 0);
-    if (int rc2 = launch_status("ks_test count init")) return rc2;
-  }
-  dim3 grid(div_up(w, tw), nl);
-  if (ks_pick_ilp(tw * (2 * r + 1)) == 6)
-    hipLaunchKernelGGL((ks_pairs_kernel<T, 6>), grid, dim3(KS_THREADS), lds, st, keys, valid, nl, w, n, az, r, tw,
-                       tab, p_out, d_out, m_out, shp_num);
+  const dim3 g2(div_up(w, KS_EMIT_TILE), nl);
+  if (ks_kb_bytes(n) == 1)
+    hipLaunchKernelGGL((ks_emit_kernel<T, uint8_t>), g2, dim3(KS_EMIT_THREADS), 0, st,
+                       reinterpret_cast<const uint8_t*>(kbuf), nl, w, az, r, tab, p_out, d_out, m_out, shp_num);
   else
-    hipLaunchKernelGGL((ks_pairs_kernel<T, 4>), grid, dim3(KS_THREADS), lds, st, keys, valid, nl, w, n, az, r, tw,
-                       tab, p_out, d_out, m_out, shp_num);
-  return launch_status("ks_pairs");
+    hipLaunchKernelGGL((ks_emit_kernel<T, uint16_t>), g2, dim3(KS_EMIT_THREADS), 0, st,
+                       reinterpret_cast<const uint16_t*>(kbuf), nl, w, az, r, tab, p_out, d_out, m_out, shp_num);
+  return launch_status("ks_emit");
 }
 
 // ---------------------------------------------------------------------------
@@ -557,7 +682,8 @@ extern "C" {
 int mr_ks_test_f32_noalloc(const float* rmli, int nl, int w, int n, int az, int r, double p_max, float* p_out,
                            float* dist_out, uint8_t* mask_out, int32_t* shp_num, mr_stream_t stream) {
   MR_CHECK_ARG(nl > 0 && w > 0 && n >= 1, "ks_test_noalloc: bad sizes (%d x %d x %d)", nl, w, n);
-  const size_t need = mr::ks_workspace_need((size_t)nl * w, n, sizeof(uint32_t));
+  MR_CHECK_ARG(az >= 0 && r >= 0, "ks_test_noalloc: negative half window");
+  const size_t need = mr::ks_workspace_need((size_t)nl * w, n, az, r, sizeof(uint32_t));
   if (need > (size_t)MR_KS_NOALLOC_SCRATCH_BYTES) {
     mr::set_error("ks_test_noalloc: a %d x %d x %d stack needs %zu bytes of scratch, more than the %zu bytes "
                   "of the library's static scratch (use mr_ks_test_f32 with a workspace)",
@@ -570,9 +696,9 @@ int mr_ks_test_f32_noalloc(const float* rmli, int nl, int w, int n, int az, int 
                                  (size_t)MR_KS_NOALLOC_SCRATCH_BYTES, stream);
 }
 
-size_t mr_ks_workspace_bytes(int nl, int w, int n) {
-  const size_t npx = (size_t)nl * w;
-  return mr::align256(npx * n * sizeof(uint64_t)) + mr::align256(npx);
+size_t mr_ks_workspace_bytes(int nl, int w, int n, int az, int r, int elem_bytes) {
+  if (nl <= 0 || w <= 0 || n < 1 || az < 0 || r < 0) return 0;
+  return mr::ks_workspace_need((size_t)nl * w, n, az, r, elem_bytes == 8 ? sizeof(uint64_t) : sizeof(uint32_t));
 }
 
 int mr_ks_test_f32(const float* rmli, int nl, int w, int n, int az, int r, double p_max, float* p_out,

@@ -31,7 +31,7 @@ def _ks(rmli, az_half_win: int, r_half_win: int, want_p: bool, want_dist: bool, 
     d = torch.empty(shape, dtype=rdt, device=x.device) if want_dist else None
     m = torch.empty(shape, dtype=torch.bool, device=x.device) if want_mask else None
     cnt = torch.empty((nl, w), dtype=torch.int32, device=x.device) if want_mask else None
-    wsb = _lib.query("mr_ks_workspace_bytes", nl, w, n)
+    wsb = _lib.query("mr_ks_workspace_bytes", nl, w, n, az_half_win, r_half_win, 8 if x.dtype == torch.float64 else 4)
     ws = _dev.workspace(wsb, x.device)
     _lib.call(_KS[x.dtype], _dev.ptr(x), nl, w, n, az_half_win, r_half_win,
               float(p_max if p_max is not None else 0.0), _dev.ptr(p), _dev.ptr(d), _dev.ptr(m), _dev.ptr(cnt),

@@ -28,7 +28,7 @@ def _ks(x, az, r, p_max, noalloc):
         _lib.call("mr_ks_test_f32_noalloc", _dev.ptr(x), nl, w, n, az, r, p_max, _dev.ptr(p), _dev.ptr(d),
                   _dev.ptr(m), _dev.ptr(c), _dev.stream())
     else:
-        ws = _dev.workspace(_lib.query("mr_ks_workspace_bytes", nl, w, n), x.device)
+        ws = _dev.workspace(_lib.query("mr_ks_workspace_bytes", nl, w, n, az, r, 4), x.device)
         _lib.call("mr_ks_test_f32", _dev.ptr(x), nl, w, n, az, r, p_max, _dev.ptr(p), _dev.ptr(d), _dev.ptr(m),
                   _dev.ptr(c), _dev.ptr(ws), ws.numel(), _dev.stream())
     return p, d, m, c
