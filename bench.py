@@ -69,6 +69,11 @@ def parse():
     return ap.parse_args()
 
 
+def progress(msg):
+    """One line per bench stage on stderr (the JSON result is the only stdout line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def measured_traffic(kernel, args):
     """HBM bytes per launch of `kernel` from the committed FETCH_SIZE/WRITE_SIZE counter
     summary (copy-kernel calibrated), valid only for the workload it was measured on."""
@@ -161,8 +166,8 @@ def batched_leg(args, torch):
     t_stage = best(per_stage.run)
     groups = {}
     for g in fused.groups:
-        groups[str(g.n)] = {"pixels": g.count, "s": best(lambda g=g: fused._call(g, -1 if fused._staged(g) else
-                                                                               g.blocks, g.chunk), 2)}
+        groups[str(g.n)] = {"pixels": g.count, "staged": g.staged,
+                            "s": best(lambda g=g: fused._call(g, -1 if g.staged else g.blocks, g.chunk), 2)}
     # phase error vs FP64 (complex128 coherence, inverse and eigh) on a sample per group
     ph, ph_off, q, _ = fused.replay()
     torch.cuda.synchronize()
@@ -184,7 +189,10 @@ def batched_leg(args, torch):
             e = torch.angle(ph[off[i]:off[i] + n].to(torch.complex128) * (v / v.abs()).conj())
             rms_all.append(float(torch.sqrt(torch.mean(e ** 2))))
     rms_all = np.array(rms_all)
+    progress("batched form: N=60 window batch")
+    n60 = _n60_window_batch(args, torch)
     return {
+        "n60_15x21_window_batch": n60,
         "workload": f"{args.batched_px} pixels, N in {{30,60,90,150}}, K in [20,315], gamma0~U[0.3,0.9], "
                     f"tau~U[12,96] d",
         "value": args.batched_px / t_graph, "unit": "DS pixels/s", "s_per_batch": t_graph,
@@ -193,6 +201,68 @@ def batched_leg(args, torch):
         "phase_rms_vs_fp64": {"median": float(np.median(rms_all)), "p99": float(np.quantile(rms_all, 0.99)),
                               "max": float(rms_all.max()), "pixels": int(rms_all.size)},
     }
+
+
+def _n60_window_batch(args, torch):
+    """The batched form on the plain workload's own pixels: N = 60, SHP sets from the KS test over
+    15x21 windows of a 384x384 DS stack; each pixel's sample matrix is its in-bounds SHP stacks in
+    window order, gathered on the device.  Fused kernel (autotuned, HIP-graph replay) against the
+    plain drop-in on the same pixels (emperical_co_pc on the raster, then EMI + DS temporal
+    coherence): both compute the same outputs (bit for bit, tests/test_gpu_batched.py)."""
+    import numpy as np
+
+    import moraine_amd as mr
+    from moraine_amd import synth
+    from moraine_amd.batched import AutoTuner, DSBatch, DSBatchRunner
+    from moraine_amd.pl import emi_ds_temp_coh
+
+    S, N, az, r = 384, 60, 7, 10
+    aw, rw = 2 * az + 1, 2 * r + 1
+    z = synth.ds_stack_torch(S, S, N, seed=7, ps_frac=args.ps, device="cuda")
+    x = torch.view_as_real(z)
+    rmli = (x[..., 0] * x[..., 0] + x[..., 1] * x[..., 1]).contiguous()
+    _, shp, cnt = mr.ks_test_select_shp(rmli, az, r, 0.05, return_p=False)
+    del rmli, x
+    npx = S * S
+    ii, jj = torch.meshgrid(torch.arange(S, device="cuda"), torch.arange(S, device="cuda"), indexing="ij")
+    ka, kr = torch.meshgrid(torch.arange(aw, device="cuda") - az, torch.arange(rw, device="cuda") - r, indexing="ij")
+    sa = ii.reshape(-1, 1) + ka.reshape(1, -1)
+    sr = jj.reshape(-1, 1) + kr.reshape(1, -1)
+    valid = shp.reshape(npx, -1) & (sa >= 0) & (sa < S) & (sr >= 0) & (sr < S)
+    src = (sa * S + sr)[valid]  # window order within each pixel, pixels in index order
+    k = valid.sum(1).to(torch.int64)
+    samples = z.reshape(-1, N)[src].reshape(-1).contiguous()
+    row_off = torch.zeros(npx, dtype=torch.int64, device="cuda")
+    row_off[1:] = torch.cumsum(k * N, 0)[:-1]
+    b = DSBatch.from_tensors(samples, row_off, np.full(npx, N, np.int32), k.to(torch.int32))
+    del sa, sr, valid, src
+    idx = torch.stack([ii.reshape(-1), jj.reshape(-1)], -1).to(torch.int32).contiguous()
+    pcs = shp.reshape(npx, aw, rw)
+
+    def best(fn, reps=3):
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return min(ts)
+
+    def plain():
+        for s0 in range(0, npx, 1 << 16):
+            c = mr.emperical_co_pc(z, idx[s0:s0 + (1 << 16)], pcs[s0:s0 + (1 << 16)])
+            emi_ds_temp_coh(c)
+
+    runner = DSBatchRunner(b, tuner=AutoTuner())
+    runner.capture()
+    t_b = best(runner.replay)
+    t_p = best(plain)
+    return {"workload": f"{S}x{S} pixels, N={N}, 15x21 windows, SHP sets from the KS test (mean K "
+                        f"{float(k.float().mean()):.1f})", "pixels": npx,
+            "batched_px_per_s": npx / t_b, "plain_px_per_s": npx / t_p, "batched_vs_plain": t_p / t_b,
+            "note": "coherence + EMI + temporal coherence only (KS excluded from both)"}
 
 
 def streamed_leg(args, torch):
@@ -301,6 +371,32 @@ def rows_leg(args, torch):
           f"{npt} points, EMI flops only")
     t = dtime(lambda: mr.emperical_co_emi_temp_coh_pc(z, idx, pcs, batch_size=npt), reps=2)
     rows["emperical_co_emi_temp_coh_pc"] = {"s": t, "points/s": npt / t, "per_call": f"{npt} points, one batch"}
+    # the reference's own GPU EMI algorithm (moraine/pl.py:18-42: float32 inv of |Gamma|, complex64
+    # eigh of inv o Gamma) run through the vendor library (rocSOLVER via torch) on the same rows:
+    # what a library-call port of moraine's CuPy path gives on this GPU
+    nref = 8192  # rocSOLVER's batched eigh takes about a millisecond per matrix at N = 60
+    cr = coh[:nref].contiguous()
+
+    def emi_reference_algorithm():
+        G = mr.uncompress_coh(cr)
+        M = torch.linalg.inv(G.abs()).to(torch.complex64) * G
+        _, V = torch.linalg.eigh(M)
+        v = V[..., :, 0] * V[..., 0:1, 0].conj()
+        return v / v.abs()
+
+    progress("rows: reference EMI algorithm through rocSOLVER")
+    emi_reference_algorithm()  # library warm-up
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    emi_reference_algorithm()
+    e1.record()
+    e1.synchronize()
+    t_ref = e0.elapsed_time(e1) / 1e3
+    t_ours = dtime(lambda: mr.emi(cr))
+    rows["emi_reference_algorithm_rocsolver"] = {
+        "s": t_ref, "points/s": nref / t_ref, "vs_emi_kernel": t_ref / t_ours,
+        "per_call": f"{nref} points, N=60: uncompress, float32 inv(|Gamma|), complex64 eigh (rocSOLVER, batched)"}
     for n2, npt2 in ((120, 32768), (200, 16384)):
         c2 = torch.randn(npt2, n2 * (n2 - 1) // 2, dtype=torch.complex64, device="cuda") * 0.1
         flops(f"emi_n{n2}", dtime(lambda: mr.emi(c2), reps=2), npt2 * emi_algorithmic_flops(n2),
@@ -362,6 +458,7 @@ def config3_leg(args, torch):
     out = {"workload": f"{nl}x{w} tile, N={n}, {aw}x{rw} windows, gamma0~U[0.2,0.9], 2% near-singular, "
                        "p_max=0.05, all pixels", "pixels": npx}
     for name, tol in (("plain", None), ("refine_1e-3", 1e-3)):
+        progress(f"precision workload: {name} pass")
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         run(tol)
@@ -570,6 +667,13 @@ def main():
         "rms_max_rad": float(rms32[~torch.isnan(rms32)].max()),
         "rms_all_rad": float(torch.sqrt(torch.nanmean(rms32 ** 2))),
         "note": "float32 inv(|Gamma|) and complex64 eigh (LAPACK, CPU) on the same coherence rows"}
+    # how often the matrices EMI factors are positive definite on this workload: a Cholesky-based
+    # formulation (FP64 Cholesky of |Gamma|, shift-invert iteration on M) needs both
+    Gs = mr.uncompress_coh(coh[:m])
+    phase["abs_gamma_not_pd_frac"] = float((~mr.isPD(Gs.abs().to(torch.float64))).float().mean())
+    phase["lambda_min_negative_frac"] = float((q[s0:s0 + m] < 0).float().mean())
+    phase["k_below_n_frac"] = float((state["shp_num"].reshape(-1)[s0:s0 + m] < N).float().mean())
+    del Gs
     phase["gpu_over_reference"] = {
         "p99": phase["rms_p99_rad"] / phase["reference_fp32_lapack"]["rms_p99_rad"],
         "max": phase["rms_max_rad"] / phase["reference_fp32_lapack"]["rms_max_rad"]}
@@ -593,24 +697,30 @@ def main():
                             "rms_all_rad": float(torch.sqrt(torch.nanmean(rms_r ** 2))),
                             "note": "EMI + temporal coherence of the last batch with refine=tol, HIP events"}
 
+    progress("plain pass timed; CPU baseline")
     cpu = None if args.no_cpu_baseline else cpu_baseline(args)
     forms = {}
     wanted = {f.strip() for f in args.forms.split(",") if f.strip()}
     del state, rslc, idx, ph, q, tc, coh
     torch.cuda.empty_cache()
     if "batched" in wanted:
+        progress("batched form")
         forms["batched"] = batched_leg(args, torch)
         torch.cuda.empty_cache()
     if "serving" in wanted:
+        progress("serving form")
         forms["serving"] = serving_leg(args, torch)
         torch.cuda.empty_cache()
     if "rows" in wanted:
+        progress("per-function rows")
         forms["rows"] = rows_leg(args, torch)
         torch.cuda.empty_cache()
     if "streamed" in wanted:
+        progress("streamed form")
         forms["streamed"] = streamed_leg(args, torch)
         torch.cuda.empty_cache()
     if "precision" in wanted:
+        progress("precision workload")
         forms["precision"] = config3_leg(args, torch)
         torch.cuda.empty_cache()
     value = npx / dt

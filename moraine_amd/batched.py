@@ -104,7 +104,7 @@ class AutoTuner:
     """
 
     FUSED_BLOCKS = (0, 4096, 16384, 65536)  # 0 = one block per pixel (capped)
-    CHUNKS = (4096, 8192, 16384, 32768)
+    CHUNKS = (4096, 8192, 16384, 32768, 65536)
 
     def __init__(self, path: str | None = None, reps: int = 2, timer=None):
         self.path = path or os.environ.get("MORAINE_AMD_TUNE_CACHE")
@@ -156,6 +156,7 @@ class _Group:
     pairs: torch.Tensor
     blocks: int = 0
     chunk: int = 0
+    staged: bool = True
 
 
 class DSBatchRunner:
@@ -208,11 +209,17 @@ class DSBatchRunner:
 
     def tune(self):
         """Choose each group's launch shape (autotuner if given, else defaults)
-        and allocate the shared large-N workspace."""
+        and allocate the shared workspace.  For 32 < N <= 64 the tuner also
+        chooses between the fused launch (one of FUSED_BLOCKS grid sizes) and
+        the staged one (coherence chunk through HBM, then the EMI + temporal
+        coherence kernel; one of CHUNKS), encoded as -chunk: both give the same
+        outputs bit for bit, and which is faster depends on N and the SHP
+        counts."""
         dev = self.batch.samples.device
         need = 0
         for g in self.groups:
-            if self._staged(g):
+            g.staged = self._staged(g)
+            if g.staged or self.tuner is not None:
                 cands = [c for c in AutoTuner.CHUNKS if self._chunk_fits(g.n, c)] or [1024]
                 g.chunk = min(cands[-1], max(g.count, 1))
                 need = max(need, _lib.query("mr_ds_batch_workspace_bytes", g.chunk, g.n))
@@ -221,12 +228,18 @@ class DSBatchRunner:
             for g in self.groups:
                 if g.count == 0:
                     continue
-                if not self._staged(g):
-                    g.blocks = self.tuner.choose(AutoTuner.key("fused", g.n, g.count), AutoTuner.FUSED_BLOCKS,
-                                                 lambda b, g=g: self._call(g, b, 0))
+                chunks = [c for c in AutoTuner.CHUNKS if self._chunk_fits(g.n, c) and c <= g.chunk] or [g.chunk]
+                if not g.staged:
+                    cands = list(AutoTuner.FUSED_BLOCKS) + [-c for c in chunks]
+                    pick = self.tuner.choose(
+                        AutoTuner.key("fused_or_staged", g.n, g.count), cands,
+                        lambda v, g=g: self._call(g, v, 0) if v >= 0 else self._call(g, -1, -v))
+                    if pick >= 0:
+                        g.blocks = pick
+                    else:
+                        g.staged, g.chunk = True, -pick
                 else:
-                    cands = [c for c in AutoTuner.CHUNKS if self._chunk_fits(g.n, c) and c <= g.chunk] or [g.chunk]
-                    g.chunk = self.tuner.choose(AutoTuner.key("chunk", g.n, g.count), cands,
+                    g.chunk = self.tuner.choose(AutoTuner.key("chunk", g.n, g.count), chunks,
                                                 lambda c, g=g: self._call(g, -1, c))
         self._tuned = True
 
@@ -236,7 +249,7 @@ class DSBatchRunner:
             self.tune()
         for g in self.groups:
             if g.count:
-                self._call(g, -1 if self._staged(g) else g.blocks, g.chunk)
+                self._call(g, -1 if g.staged else g.blocks, g.chunk)
 
     def run(self):
         """Eager run; returns (ph, ph_off, quality, t_coh)."""

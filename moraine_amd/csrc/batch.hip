@@ -152,11 +152,10 @@ int mr_ds_batch(const float* samples, const long long* row_off, const int32_t* n
         : wide  ? mr::co_mfma_wide_samples_launch(smp, row_off, n_samples, pix, c0, cn, n, pairs, n_pairs, coh, st)
                 : mr::co_samples_launch(smp, row_off, n_samples, pix, c0, cn, n, pairs, n_pairs, coh, st);
     if (rc0) return rc0;
-    if (int rc = mr_emi(reinterpret_cast<const float*>(coh), cn, n, ref, reinterpret_cast<float*>(ph_t), q_t, emi_ws,
-                        emi_ws_bytes, stream))
-      return rc;
-    if (int rc = mr_ds_temp_coh(reinterpret_cast<const float*>(coh), reinterpret_cast<const float*>(ph_t), cn, n,
-                                pairs, n_pairs, t_t, stream))
+    // EMI and the temporal coherence of the chunk: one kernel for 32 < n <= 64 (equal, bit for bit, to
+    // mr_emi followed by mr_ds_temp_coh)
+    if (int rc = mr_emi_ds_temp_coh(reinterpret_cast<const float*>(coh), cn, n, ref, pairs, n_pairs,
+                                    reinterpret_cast<float*>(ph_t), q_t, t_t, emi_ws, emi_ws_bytes, stream))
       return rc;
     const int nb = (int)std::min<long long>(mr::div_up((unsigned long long)cn * n, 256), 1 << 16);
     hipLaunchKernelGGL(mr::ds_batch_scatter_kernel, dim3(nb), dim3(256), 0, st, ph_t, q_t, t_t, pix, c0, cn, n,

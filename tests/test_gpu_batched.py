@@ -112,10 +112,13 @@ def test_batched_launch_shapes_do_not_change_results():
     torch.cuda.synchronize()
     assert len(tuner.picks) == 3
     for g in r.groups:
-        if not r._staged(g):
-            assert g.blocks in AutoTuner.FUSED_BLOCKS
+        if not g.staged:
+            assert g.blocks in AutoTuner.FUSED_BLOCKS and 32 < g.n <= 64
         else:
             assert g.chunk in AutoTuner.CHUNKS or g.chunk == g.count
+    # the N = 45 group was timed fused and staged
+    k45 = [k for k in tuner.trials if k.startswith("fused_or_staged:n=45")]
+    assert k45 and any(v < 0 for v in tuner.trials[k45[0]]) and any(v >= 0 for v in tuner.trials[k45[0]])
     assert torch.equal(out[0].view(torch.int64), ref[0].view(torch.int64))
     assert torch.equal(out[2].view(torch.int32), ref[2].view(torch.int32))
     assert torch.equal(out[3].view(torch.int32), ref[3].view(torch.int32))
