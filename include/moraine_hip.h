@@ -280,8 +280,10 @@ int mr_emi_refined(const float* coh, long long n_pts, int n, int ref, double tol
  *   pairs      the full triu network (n_pairs = n(n-1)/2);
  *   ph         complex64, pixel id's n phases at complex element ph_off[id];
  *   quality, t_coh  float32 indexed by pixel id.
- * 32 < n <= 64 runs one fused launch (coherence stays on chip; `blocks` sets
- * its grid, 0 = default); n <= 32, n > 64, or blocks < 0 (per-stage mode:
+ * 32 < n <= 64 runs the fused launch (coherence stays on chip; `blocks` sets
+ * its grid, 0 = default; workspace >= mr_ds_batch_fused_workspace_bytes(n_pix)
+ * for the list of pixels the shift-invert EMI leaves to the Householder
+ * pass); n <= 32, n > 64, or blocks < 0 (per-stage mode:
  * coherence, EMI and temporal coherence as separate launches through HBM, the
  * drop-in sequence), runs per chunk of `chunk` pixels with
  * workspace >= mr_ds_batch_workspace_bytes(chunk, n).  For a sample matrix
@@ -289,6 +291,7 @@ int mr_emi_refined(const float* coh, long long n_pts, int n, int ref, double tol
  * composition exactly.
  * ------------------------------------------------------------------------- */
 size_t mr_ds_batch_workspace_bytes(long long chunk, int n);
+size_t mr_ds_batch_fused_workspace_bytes(long long n_pix);
 int mr_ds_batch(const float* samples, const long long* row_off, const int32_t* n_samples,
                 const long long* ph_off, const int32_t* pix, long long n_pix, int n, int ref,
                 const int32_t* pairs, int n_pairs, float* ph, float* quality, float* t_coh, int blocks,

@@ -219,6 +219,8 @@ class DSBatchRunner:
         need = 0
         for g in self.groups:
             g.staged = self._staged(g)
+            if not g.staged:  # the fused launch's fallback list
+                need = max(need, _lib.query("mr_ds_batch_fused_workspace_bytes", g.count))
             if g.staged or self.tuner is not None:
                 cands = [c for c in AutoTuner.CHUNKS if self._chunk_fits(g.n, c)] or [1024]
                 g.chunk = min(cands[-1], max(g.count, 1))

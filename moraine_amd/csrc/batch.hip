@@ -35,12 +35,19 @@ int co_mfma_wide_samples_launch(const float2* samples, const long long* row_off,
                                 const int32_t* pix, long long pix0, long long cn, int n, const int32_t* pairs,
                                 int npairs, float2* coh, hipStream_t st);
 constexpr int BATCH_DEFAULT_BLOCKS = 1 << 15;
+constexpr int BATCH_FALLBACK_BLOCKS = 8192;
+
+__global__ void batch_zero_count_kernel(unsigned* c) { *c = 0u; }
 
 __host__ __device__ inline size_t batch_fused_lds(bool two) {
   return (size_t)com_tile_floats(two) * 4 + 128 * 4 /* scales */ + EMI_FAST_LDS_BYTES;
 }
 
-template <int NM, bool TWO>
+// MODE 1: EMI by the shift-invert formulation; a pixel it declines is
+// appended to list[*count] and left unwritten.  MODE 2: the Householder
+// formulation for the pixels of that list (the plain kernels' two passes, so
+// the outputs equal the staged path's bit for bit).
+template <int NM, bool TWO, int MODE>
 __global__ __launch_bounds__(64) void ds_batch_fused_kernel(const float2* __restrict__ samples,
                                                             const long long* __restrict__ row_off,
                                                             const int32_t* __restrict__ n_samp,
@@ -48,13 +55,16 @@ __global__ __launch_bounds__(64) void ds_batch_fused_kernel(const float2* __rest
                                                             const int32_t* __restrict__ pix, long long npix, int n,
                                                             int ref, const int32_t* __restrict__ pairs, int npairs,
                                                             float2* __restrict__ ph, float* __restrict__ quality,
-                                                            float* __restrict__ tcoh) {
+                                                            float* __restrict__ tcoh, unsigned* __restrict__ count,
+                                                            int* __restrict__ list) {
   extern __shared__ __align__(16) unsigned char smem[];
   float2* tiles = reinterpret_cast<float2*>(smem);
   float* sc = reinterpret_cast<float*>(smem + (size_t)com_tile_floats(TWO) * 4);  // scales, 1/sqrt(diag)
   unsigned char* work = reinterpret_cast<unsigned char*>(sc + 128);  // EMI_FAST_LDS_BYTES
   float2* ub = reinterpret_cast<float2*>(work + 512);               // linked phases after EMI
-  for (long long t = blockIdx.x; t < npix; t += gridDim.x) {
+  const long long nit = MODE == 2 ? (long long)*count : npix;
+  for (long long it = blockIdx.x; it < nit; it += gridDim.x) {
+    const long long t = MODE == 2 ? (long long)list[it] : it;
     int lane = threadIdx.x;
     __asm__ volatile("" : "+v"(lane));  // per-pixel opaque lane (see emi.hip)
     const int id = pix[t];
@@ -62,7 +72,15 @@ __global__ __launch_bounds__(64) void ds_batch_fused_kernel(const float2* __rest
     const CohTiles<TWO> ct{tiles, sc + 64};
     float vr, vi;
     double rq;
-    emi_fast_point<NM>(ct, n, lane, work, vr, vi, rq);
+    if constexpr (MODE == 1) {
+      if (!emi_inv_point<NM>(ct, n, lane, work, vr, vi, rq)) {
+        if (lane == 0) list[atomicAdd(count, 1u)] = (int)t;
+        __builtin_amdgcn_wave_barrier();
+        continue;
+      }
+    } else {
+      emi_fast_point<NM>(ct, n, lane, work, vr, vi, rq);
+    }
     const float2 pv = emi_link_phase(vr, vi, ref);
     __builtin_amdgcn_wave_barrier();
     ub[lane] = pv;  // linked phases for the temporal coherence
@@ -100,6 +118,10 @@ static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 extern "C" {
 
+size_t mr_ds_batch_fused_workspace_bytes(long long n_pix) {
+  return 256 + (((size_t)(n_pix > 0 ? n_pix : 0) * 4 + 255) & ~(size_t)255);
+}
+
 size_t mr_ds_batch_workspace_bytes(long long chunk, int n) {
   if (n < 2 || chunk <= 0) return 0;
   const long long P = (long long)n * (n - 1) / 2;
@@ -123,10 +145,23 @@ int mr_ds_batch(const float* samples, const long long* row_off, const int32_t* n
   const float2* smp = reinterpret_cast<const float2*>(samples);
   float2* phc = reinterpret_cast<float2*>(ph);
   if (n > 32 && n <= 64 && blocks >= 0) {
+    const size_t need = mr_ds_batch_fused_workspace_bytes(n_pix);
+    MR_CHECK_ARG(workspace && workspace_bytes >= need, "ds_batch: fused launch workspace %zu < %zu bytes",
+                 workspace_bytes, need);
+    unsigned* count = reinterpret_cast<unsigned*>(workspace);
+    int* list = reinterpret_cast<int*>(reinterpret_cast<unsigned char*>(workspace) + 256);
+    hipLaunchKernelGGL(mr::batch_zero_count_kernel, dim3(1), dim3(1), 0, st, count);
+    if (int rc = mr::launch_status("ds_batch count init")) return rc;
     const int nb = (int)std::min<long long>(n_pix, blocks > 0 ? blocks : mr::BATCH_DEFAULT_BLOCKS);
-    hipLaunchKernelGGL((mr::ds_batch_fused_kernel<64, true>), dim3(nb), dim3(64), mr::batch_fused_lds(true), st, smp,
-                       row_off, n_samples, ph_off, pix, n_pix, n, ref, pairs, n_pairs, phc, quality, t_coh);
-    return mr::launch_status("ds_batch");
+    hipLaunchKernelGGL((mr::ds_batch_fused_kernel<64, true, 1>), dim3(nb), dim3(64), mr::batch_fused_lds(true), st,
+                       smp, row_off, n_samples, ph_off, pix, n_pix, n, ref, pairs, n_pairs, phc, quality, t_coh, count,
+                       list);
+    if (int rc = mr::launch_status("ds_batch shift-invert")) return rc;
+    const int nb2 = (int)std::min<long long>(n_pix, mr::BATCH_FALLBACK_BLOCKS);
+    hipLaunchKernelGGL((mr::ds_batch_fused_kernel<64, true, 2>), dim3(nb2), dim3(64), mr::batch_fused_lds(true), st,
+                       smp, row_off, n_samples, ph_off, pix, n_pix, n, ref, pairs, n_pairs, phc, quality, t_coh, count,
+                       list);
+    return mr::launch_status("ds_batch householder");
   }
   MR_CHECK_ARG(chunk > 0, "ds_batch: chunk must be positive for n <= 32, n > 64 or the per-stage mode");
   const size_t need = mr_ds_batch_workspace_bytes(chunk, n);

@@ -124,7 +124,11 @@ constexpr int EMI_FAST_THREADS = 64;
 // (moraine/pl.py:126-145), as mr_ds_temp_coh computes it from (coh, ph, pairs).
 // CHK: points whose conditioning test fails (tau) are appended to
 // flag_list[*flag_count] for the FP64 re-solve.
-template <int NM, bool TC, bool CHK = false>
+// MODE 0: the Householder formulation for every point (with CHK: the precision
+// form's flags).  MODE 1: the shift-invert formulation; a point it declines is
+// appended to flag_list[*flag_count] and left unwritten.  MODE 2: the
+// Householder formulation for the points of that list.
+template <int NM, bool TC, bool CHK = false, int MODE = 0>
 __global__ __launch_bounds__(EMI_FAST_THREADS) void emi_fast_kernel(const float2* __restrict__ coh, long long npts,
                                                                     int n, int ref, float2* __restrict__ ph,
                                                                     float* __restrict__ quality,
@@ -138,7 +142,9 @@ __global__ __launch_bounds__(EMI_FAST_THREADS) void emi_fast_kernel(const float2
   float2* cs = reinterpret_cast<float2*>(smem);                          // staged coh, P entries
   unsigned char* work = smem + ((P * 8 + 15) & ~15);                      // EMI_FAST_LDS_BYTES
 
-  for (long long pt = blockIdx.x; pt < npts; pt += gridDim.x) {
+  const long long nit = MODE == 2 ? (long long)*flag_count : npts;
+  for (long long it = blockIdx.x; it < nit; it += gridDim.x) {
+    const long long pt = MODE == 2 ? (long long)flag_list[it] : it;
     // Opaque per-point copy of the lane id: keeps the compiler from hoisting
     // NM per-lane gather addresses out of the point loop (they would stay live
     // in VGPRs for the whole kernel).
@@ -164,7 +170,15 @@ __global__ __launch_bounds__(EMI_FAST_THREADS) void emi_fast_kernel(const float2
     float vr, vi;
     double rq;
     bool flag = false;
-    emi_fast_point<NM, CHK>(CohPacked{cs, n}, n, lane, work, vr, vi, rq, tau, &flag);
+    if constexpr (MODE == 1) {
+      if (!emi_inv_point<NM>(CohPacked{cs, n}, n, lane, work, vr, vi, rq)) {
+        if (lane == 0) flag_list[atomicAdd(flag_count, 1u)] = (int)pt;
+        __builtin_amdgcn_wave_barrier();
+        continue;
+      }
+    } else {
+      emi_fast_point<NM, CHK>(CohPacked{cs, n}, n, lane, work, vr, vi, rq, tau, &flag);
+    }
     // ---- output: ph = v conj(v_ref) / |.|, quality --------------------------
     const float2 pv = emi_link_phase(vr, vi, ref);
     if (lane < n) ph[pt * n + lane] = pv;
@@ -509,9 +523,33 @@ int emi_wide_launch(const float2* coh, long long npts, int n, int ref, float2* p
 
 __global__ void emi_zero_count_kernel(unsigned* c) { *c = 0u; }
 
+// count + list of the points the shift-invert pass leaves to the Householder pass
+inline size_t emi_list_bytes(long long npts) { return 256 + (((size_t)(npts > 0 ? npts : 0) * 4 + 255) & ~(size_t)255); }
+
 static size_t emi_fast_lds(int n, int nm) {
   const int P = n * (n - 1) / 2;
   return (((size_t)P * 8 + 15) & ~(size_t)15) + EMI_FAST_LDS_BYTES;
+}
+
+// 32 < n <= 64: the shift-invert pass over every point, then the Householder
+// pass over the points it left (count and list at the start of ws).
+constexpr int EMI_FALLBACK_BLOCKS = 8192;
+template <bool TC>
+static int emi_two_pass(const float2* c, long long n_pts, int n, int ref, float2* p, float* quality,
+                        const int32_t* pairs, float* t_coh, unsigned char* ws, hipStream_t st) {
+  unsigned* count = reinterpret_cast<unsigned*>(ws);
+  int* list = reinterpret_cast<int*>(ws + 256);
+  const size_t lds = emi_fast_lds(n, 64);
+  hipLaunchKernelGGL(emi_zero_count_kernel, dim3(1), dim3(1), 0, st, count);
+  if (int rc = launch_status("emi count init")) return rc;
+  const int nb = (int)std::min<long long>(n_pts, 1 << 20);
+  hipLaunchKernelGGL((emi_fast_kernel<64, TC, false, 1>), dim3(nb), dim3(EMI_FAST_THREADS), lds, st, c, n_pts, n,
+                     ref, p, quality, pairs, t_coh, 0.0, count, list);
+  if (int rc = launch_status("emi_fast shift-invert")) return rc;
+  const int nb2 = (int)std::min<long long>(n_pts, EMI_FALLBACK_BLOCKS);
+  hipLaunchKernelGGL((emi_fast_kernel<64, TC, false, 2>), dim3(nb2), dim3(EMI_FAST_THREADS), lds, st, c, n_pts, n,
+                     ref, p, quality, pairs, t_coh, 0.0, count, list);
+  return launch_status("emi_fast householder");
 }
 
 }  // namespace mr
@@ -519,6 +557,7 @@ static size_t emi_fast_lds(int n, int nm) {
 extern "C" {
 
 size_t mr_emi_workspace_bytes(long long n_pts, int n) {
+  if (n > 32 && n <= 64) return mr::emi_list_bytes(n_pts);  // the Householder-fallback list
   if (n <= mr::EMI_WIDE_MAX_N) return 0;
   const long long nb = n_pts < mr::EMI_GEN_BLOCKS ? n_pts : mr::EMI_GEN_BLOCKS;
   return (size_t)(nb > 0 ? nb : 1) * mr::emi_gen_slice_bytes(n);
@@ -535,11 +574,11 @@ int mr_emi(const float* coh, long long n_pts, int n, int ref, float* ph, float* 
   const float2* c = reinterpret_cast<const float2*>(coh);
   float2* p = reinterpret_cast<float2*>(ph);
   if (n > 32 && n <= 64) {
-    const size_t lds = mr::emi_fast_lds(n, 64);
-    const int nb = (int)std::min<long long>(n_pts, 1 << 20);
-    hipLaunchKernelGGL((mr::emi_fast_kernel<64, false>), dim3(nb), dim3(mr::EMI_FAST_THREADS), lds, st, c, n_pts, n,
-                       ref, p, quality, nullptr, nullptr);
-    return mr::launch_status("emi_fast");
+    const size_t need = mr_emi_workspace_bytes(n_pts, n);
+    MR_CHECK_ARG(workspace != nullptr && workspace_bytes >= need, "emi: workspace %zu < %zu bytes", workspace_bytes,
+                 need);
+    return mr::emi_two_pass<false>(c, n_pts, n, ref, p, quality, nullptr, nullptr,
+                                   reinterpret_cast<unsigned char*>(workspace), st);
   }
   if (n <= mr::EMI_WIDE_MAX_N) return mr::emi_wide_launch(c, n_pts, n, ref, p, quality, st);
   const size_t need = mr_emi_workspace_bytes(n_pts, n);
@@ -586,13 +625,12 @@ int mr_emi_ds_temp_coh(const float* coh, long long n_pts, int n, int ref, const 
   if (n_pts == 0) return MR_OK;
   MR_CHECK_ARG(coh && pairs && ph && quality && t_coh, "emi_ds_temp_coh: null pointer");
   if (n > 32 && n <= 64) {
-    hipStream_t st = mr::as_stream(stream);
-    const size_t lds = mr::emi_fast_lds(n, 64);
-    const int nb = (int)std::min<long long>(n_pts, 1 << 20);
-    hipLaunchKernelGGL((mr::emi_fast_kernel<64, true>), dim3(nb), dim3(mr::EMI_FAST_THREADS), lds, st,
-                       reinterpret_cast<const float2*>(coh), n_pts, n, ref, reinterpret_cast<float2*>(ph), quality,
-                       pairs, t_coh);
-    return mr::launch_status("emi_ds_temp_coh");
+    const size_t need = mr_emi_workspace_bytes(n_pts, n);
+    MR_CHECK_ARG(workspace != nullptr && workspace_bytes >= need, "emi_ds_temp_coh: workspace %zu < %zu bytes",
+                 workspace_bytes, need);
+    return mr::emi_two_pass<true>(reinterpret_cast<const float2*>(coh), n_pts, n, ref, reinterpret_cast<float2*>(ph),
+                                  quality, pairs, t_coh, reinterpret_cast<unsigned char*>(workspace),
+                                  mr::as_stream(stream));
   }
   const int rc = mr_emi(coh, n_pts, n, ref, ph, quality, workspace, workspace_bytes, stream);
   if (rc != MR_OK) return rc;

@@ -463,6 +463,265 @@ __device__ __forceinline__ void emi_fast_point(const Src& coh_at, int n, int lan
   }
 }
 
+// Steps 1-2: M = inv(|Gamma|) o Gamma into mr_, mi_ (row `lane`).  Returns
+// whether |Gamma| is positive definite: the sweep pivots are the pivots of a
+// symmetric elimination with symmetric pivoting, so their signs are |Gamma|'s
+// inertia (Sylvester); all positive <=> PD.
+template <int NM, class Src>
+__device__ __forceinline__ bool emi_fast_form_m(const Src& coh_at, int n, int lane, unsigned char* lds,
+                                                RegArr<float, NM>& mr_, RegArr<float, NM>& mi_) {
+  double* colb = reinterpret_cast<double*>(lds);        // 64: sweep column broadcast
+  bool abs_pd = true;
+  // ---- step 1: A = |Gamma| (row per lane, FP64), sweep inverse ----------
+  RegArr<double, NM> a;
+#pragma unroll
+  for (int j = 0; j < NM; ++j) {
+    double v = 0.0;
+    if (lane < n && j < n) {
+      if (j == lane) {
+        v = 1.0;
+      } else {
+        const float2 g = coh_at(j > lane ? lane : j, j > lane ? j : lane);
+        v = sqrt((double)g.x * g.x + (double)g.y * g.y);
+      }
+    }
+    a.set(j, v);
+    if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+  }
+  double dg = lane < n ? 1.0 : 0.0;  // own diagonal entry, tracked through the sweep
+  bool swept = lane >= n;
+  for (int s = 0; s < n; ++s) {
+    int k = 63 - (wmax_i(pivot_key<6>(fabs(dg), !swept, lane)) & 63);  // unswept row, largest |diagonal|
+    __asm__ volatile("" : "+s"(k));  // an opaque index keeps the row in VGPRs (M0-relative read, no scratch copy)
+    const double c = a.dyn(k);
+    // Broadcasting d-1 in place of the pivot d lets the rank-1 update itself
+    // write column k:  c - (c/d)(d-1) = c/d.  The pivot lane's own diagonal
+    // comes out as 2 - 1/d instead of -1/d; that entry is never read again
+    // by the sweep, so the +2 offset is removed once, when M is formed.
+    colb[lane] = (lane == k) ? c - 1.0 : c;
+    __builtin_amdgcn_wave_barrier();
+    const double dk = rl_d(c, k);
+    abs_pd = abs_pd && dk > 0.0;
+    const double rd = rcp_d(dk);
+    const double f = (lane == k) ? (1.0 - rd) : c * rd;
+    {
+      // software-pipelined broadcast of colb: group g+1 is loaded before the
+      // FMAs of group g, so LDS latency overlaps the FP64 FMAs
+      double cur[8], nxt[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) cur[t] = colb[t];
+#pragma unroll
+      for (int g = 0; g < NM / 8; ++g) {
+        if (g + 1 < NM / 8) {
+#pragma unroll
+          for (int t = 0; t < 8; ++t) nxt[t] = colb[(g + 1) * 8 + t];
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t) a.set(g * 8 + t, fma(-f, cur[t], a.get(g * 8 + t)));
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int t = 0; t < 8; ++t) cur[t] = nxt[t];
+      }
+    }
+    dg = fma(-f, c, dg);
+    swept = swept || (lane == k);
+    __builtin_amdgcn_wave_barrier();
+  }
+  // a now holds -inv(A) row `lane` (diagonal entry offset by +2)
+
+  // ---- step 2: M = inv(A) o Gamma (FP32 complex row per lane) ------------
+#pragma unroll
+  for (int j = 0; j < NM; ++j) {
+    float re = 0.f, im = 0.f;
+    if (lane < n && j < n) {
+      const double ai = (j == lane) ? 2.0 - a.get(j) : -a.get(j);  // undo the sweep's +2 on the diagonal
+      if (j == lane) {
+        re = (float)ai;
+      } else if (j > lane) {
+        const float2 g = coh_at(lane, j);
+        re = (float)(ai * g.x);
+        im = (float)(ai * g.y);
+      } else {
+        const float2 g = coh_at(j, lane);
+        re = (float)(ai * g.x);
+        im = (float)(-ai * g.y);
+      }
+    }
+    mr_.set(j, re);
+    mi_.set(j, im);
+    if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+  }
+  return abs_pd;
+}
+
+// Shift-invert formulation (|Gamma| PD): B = M - sigma I with sigma = 0.98
+// below the smallest eigenvalue (M is near a closure-free matrix, whose
+// smallest eigenvalue is 1), X = inv(B) by the Hermitian sweep in FP32 (the
+// pivots certify B PD: one that is not positive means sigma >= lambda_min and
+// the point goes back to the Householder path), then inverse iteration
+// x <- X x / |X x| from the closure-free guess x_j = conj(Gamma_0j)/|Gamma_0j|
+// until successive unit iterates differ by less than EMI_INV_TOL in every
+// entry (at least EMI_INV_MIN, at most EMI_INV_MAX sweeps; not converged ->
+// Householder path).  lambda_min = sigma + 1 / (x^H X x).  Returns false when
+// the point must take the Householder path (M in mr_, mi_ is destroyed).
+constexpr float EMI_INV_SIGMA = 0.98f;
+constexpr float EMI_INV_TOL = 4e-7f;
+constexpr int EMI_INV_MIN = 3;
+constexpr int EMI_INV_MAX = 16;
+
+__device__ __forceinline__ float wmax_f(float v) {
+  v = fmaxf(v, dpp_f<DPP_XOR1>(v));
+  v = fmaxf(v, dpp_f<DPP_XOR2>(v));
+  v = fmaxf(v, dpp_f<DPP_HALF_MIRROR>(v));
+  v = fmaxf(v, dpp_f<DPP_MIRROR>(v));
+  return fmaxf(fmaxf(rl_f(v, 0), rl_f(v, 16)), fmaxf(rl_f(v, 32), rl_f(v, 48)));
+}
+
+template <int NM, class Src>
+__device__ __forceinline__ bool emi_shift_invert(const Src& coh_at, int n, int lane, unsigned char* lds,
+                                                 RegArr<float, NM>& br, RegArr<float, NM>& bi, float& vr, float& vi,
+                                                 double& rq) {
+  float2* colb = reinterpret_cast<float2*>(lds);        // 64: sweep column broadcast
+  float2* xb = reinterpret_cast<float2*>(lds + 512);    // 64: iterate broadcast
+  // B = M - sigma I, and this lane's diagonal entry (real) for the pivot search
+  float dg = 0.f;
+#pragma unroll
+  for (int j = 0; j < NM; ++j) {
+    const bool dj = j == lane && lane < n;
+    const float b = br.get(j) - (dj ? EMI_INV_SIGMA : 0.f);
+    br.set(j, b);
+    if (dj) dg = b;
+  }
+  // ---- X = inv(B): Hermitian sweep with largest-diagonal pivoting ---------
+  // Row i:  B_ij -= f_i conj(B_jk) (row k = conj(column k)), f_i = B_ik / d,
+  // d = B_kk.  Broadcasting d - 1 in place of the pivot lets the update write
+  // column k itself (B_ik - f_i (d - 1) = B_ik / d); the pivot lane uses
+  // f = 1 - 1/d, so its row becomes B_kj / d and its diagonal 2 - 1/d (the
+  // +2 is undone in the products below).  After all steps B holds -inv(B)
+  // off the diagonal and 2 - inv(B)_ii on it.
+  bool swept = lane >= n;
+  for (int s = 0; s < n; ++s) {
+    int k = 63 - (wmax_i(pivot_key<6>(fabs((double)dg), !swept, lane)) & 63);
+    __asm__ volatile("" : "+s"(k));
+    const float cr = br.dyn(k), ci = bi.dyn(k);  // B[lane][k]
+    const float d = rl_f(cr, k);
+    if (!(d > 0.f)) return false;  // sigma >= lambda_min (or B singular): Householder path
+    colb[lane] = (lane == k) ? make_float2(cr - 1.f, 0.f) : make_float2(cr, ci);
+    __builtin_amdgcn_wave_barrier();
+    float rd = __builtin_amdgcn_rcpf(d);
+    rd = fmaf(rd, fmaf(-d, rd, 1.f), rd);
+    const float fr = (lane == k) ? 1.f - rd : cr * rd;
+    const float fi = (lane == k) ? 0.f : ci * rd;
+    {
+      float2 cur[4], nxt[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) cur[t] = colb[t];
+#pragma unroll
+      for (int g = 0; g < NM / 4; ++g) {
+        if (g + 1 < NM / 4) {
+#pragma unroll
+          for (int t = 0; t < 4; ++t) nxt[t] = colb[(g + 1) * 4 + t];
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const int j = g * 4 + t;
+          float re = br.get(j), im = bi.get(j);
+          re = fmaf(-fr, cur[t].x, re);  // -= Re(f conj(c))
+          re = fmaf(-fi, cur[t].y, re);
+          im = fmaf(-fi, cur[t].x, im);  // -= Im(f conj(c))
+          im = fmaf(fr, cur[t].y, im);
+          br.set(j, re);
+          bi.set(j, im);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) cur[t] = nxt[t];
+      }
+    }
+    dg = fmaf(-fr, cr, fmaf(-fi, ci, dg));
+    swept = swept || (lane == k);
+    __builtin_amdgcn_wave_barrier();
+  }
+  // ---- inverse iteration: y = X x = 2 x - (B x) ---------------------------
+  float xr = 0.f, xi = 0.f;
+  if (lane == 0) {
+    xr = 1.f;
+  } else if (lane < n) {
+    const float2 g = coh_at(0, lane);
+    const float a = sqrtf(g.x * g.x + g.y * g.y);
+    xr = a > 0.f ? g.x / a : 1.f;
+    xi = a > 0.f ? -g.y / a : 0.f;
+  }
+  {
+    const float nrm = sqrtf(wsum_f(xr * xr + xi * xi));
+    xr /= nrm;
+    xi /= nrm;
+  }
+  float mu = 0.f;
+  bool conv = false;
+  for (int it = 0; it < EMI_INV_MAX; ++it) {
+    xb[lane] = make_float2(xr, xi);
+    __builtin_amdgcn_wave_barrier();
+    float ar = 0.f, ai = 0.f;
+    {
+      float2 cur[4], nxt[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) cur[t] = xb[t];
+#pragma unroll
+      for (int g = 0; g < NM / 4; ++g) {
+        if (g + 1 < NM / 4) {
+#pragma unroll
+          for (int t = 0; t < 4; ++t) nxt[t] = xb[(g + 1) * 4 + t];
+        }
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const int j = g * 4 + t;
+          ar = fmaf(br.get(j), cur[t].x, ar);
+          ar = fmaf(-bi.get(j), cur[t].y, ar);
+          ai = fmaf(br.get(j), cur[t].y, ai);
+          ai = fmaf(bi.get(j), cur[t].x, ai);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) cur[t] = nxt[t];
+      }
+    }
+    const float yr = lane < n ? 2.f * xr - ar : 0.f, yi = lane < n ? 2.f * xi - ai : 0.f;
+    mu = wsum_f(xr * yr + xi * yi);  // x^H X x (x unit, X Hermitian: real)
+    const float nrm = sqrtf(wsum_f(yr * yr + yi * yi));
+    const float ny = 1.f / nrm;
+    const float zr = yr * ny, zi = yi * ny;
+    const float dr = zr - xr, di = zi - xi;
+    const float dmax = wmax_f(dr * dr + di * di);
+    xr = zr;
+    xi = zi;
+    __builtin_amdgcn_wave_barrier();
+    if (it + 1 >= EMI_INV_MIN && dmax < EMI_INV_TOL * EMI_INV_TOL) {
+      conv = true;
+      break;
+    }
+  }
+  if (!conv || !(mu > 0.f) || !isfinite(mu)) return false;
+  vr = lane < n ? xr : 0.f;
+  vi = lane < n ? xi : 0.f;
+  rq = (double)EMI_INV_SIGMA + 1.0 / (double)mu;
+  return true;
+}
+
+// One wave, 32 < n <= 64 plain path, first pass: the shift-invert formulation
+// when |Gamma| is positive definite.  Returns false when the point needs the
+// Householder formulation instead (|Gamma| indefinite, sigma not below the
+// smallest eigenvalue, or no convergence): the caller lists it for a second
+// launch running emi_fast_point (one kernel holding both formulations spills
+// the register-resident matrix to scratch).
+template <int NM, class Src>
+__device__ __forceinline__ bool emi_inv_point(const Src& coh_at, int n, int lane, unsigned char* lds, float& vr,
+                                              float& vi, double& rq) {
+  RegArr<float, NM> mr_, mi_;
+  if (!emi_fast_form_m<NM>(coh_at, n, lane, lds, mr_, mi_)) return false;
+  return emi_shift_invert<NM>(coh_at, n, lane, lds, mr_, mi_, vr, vi, rq);
+}
+
 // ph_i = v_i conj(v_ref) / |v_i conj(v_ref)|
 __device__ __forceinline__ float2 emi_link_phase(float vr, float vi, int ref) {
   const float rr = rl_f(vr, ref), ri = rl_f(vi, ref);

@@ -17,6 +17,9 @@ from .tnet import nimage_from_npair
 #: after a call with ``refine`` set: a 0-d int32 device tensor, the number of
 #: points that were solved again in FP64 (reading it synchronises the stream)
 last_refined = None
+#: ... and an int32 device tensor whose first ``last_refined`` entries are those
+#: points' indices (in no particular order)
+last_refined_points = None
 
 
 def _check_refine(refine, n: int):
@@ -41,7 +44,9 @@ def _emi_refined_dev(c: torch.Tensor, n: int, ref: int, tol: float, ph: torch.Te
     _lib.call("mr_emi_refined", _dev.ptr(c), npts, n, int(ref), tol, _dev.ptr(pairs) if pairs is not None else None,
               pairs.shape[0] if pairs is not None else 0, _dev.ptr(ph), _dev.ptr(q),
               _dev.ptr(t) if t is not None else None, _dev.ptr(ws), ws.numel(), _dev.stream())
+    global last_refined_points
     last_refined = ws[:4].view(torch.int32)[0]
+    last_refined_points = ws[256:256 + 4 * npts].view(torch.int32)
 
 
 def _emi_dev(c: torch.Tensor, n: int, ref: int, tol=None):

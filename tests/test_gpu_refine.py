@@ -2,8 +2,11 @@
 
 Against the FP64 oracle (oracle.moraine_ref.emi64) on a synthetic DS stack
 whose SHP sets include fewer samples than images (K < N: |Gamma| close to
-singular, where the FP32 stages lose the most).  Unflagged points must equal
-the plain call bit for bit; every point must be within the stated phase
+singular, where the FP32 stages lose the most).  The precision form runs the
+Householder formulation with its a-priori flag: where that is also the plain
+call's formulation (n <= 32, n > 64) unflagged points equal the plain call bit
+for bit; for 32 < n <= 64 the plain call runs the shift-invert formulation, and
+both are held to the FP64 oracle.  Every point must be within the stated phase
 tolerance of FP64 after refinement.
 """
 import sys
@@ -54,15 +57,10 @@ def test_refine_flags_fixes_and_leaves_the_rest(coh60):
     ph1, q1 = mr.emi(coh60, refine=tol)
     ph1, q1 = ph1.cpu().numpy(), q1.cpu().numpy()
     n_ref = int(mr.pl.last_refined)
-    changed = np.any(ph0 != ph1, axis=1) | (q0 != q1)
-    # flagged points are re-solved; everything else is the plain call's output, bit for bit
-    assert 0 < n_ref < 0.05 * ph0.shape[0]
-    assert changed.sum() <= n_ref
-    same = ~changed
-    np.testing.assert_array_equal(ph0[same].view(np.uint32), ph1[same].view(np.uint32))
-    # FP64 oracle on the re-solved points and on the plain call's worst points
+    fix = np.sort(mr.pl.last_refined_points[:n_ref].cpu().numpy())
+    assert 0 < n_ref < 0.05 * ph0.shape[0] and np.unique(fix).size == n_ref
+    # FP64 oracle on the re-solved points and on a sample of all points
     c = coh60.cpu().numpy()
-    fix = np.flatnonzero(changed)
     p64_fix, q64_fix = R.emi64(c[fix])
     err_fix = _rms_phase(ph1[fix], p64_fix)
     assert err_fix.max() < 1e-5, err_fix.max()  # FP64 throughout, phases rounded to complex64
@@ -71,9 +69,11 @@ def test_refine_flags_fixes_and_leaves_the_rest(coh60):
     p64, _ = R.emi64(c[sample])
     e0 = _rms_phase(ph0[sample], p64)
     e1 = _rms_phase(ph1[sample], p64)
-    # tolerance: per-point RMS phase error vs FP64 stays near tol after refinement
+    # tolerance: per-point RMS phase error vs FP64 stays near tol after refinement; the
+    # plain call (shift-invert for this n) within 1e-3 rad everywhere
+    print(f"  sample of {sample.size}: plain max {e0.max():.2e}, refined max {e1.max():.2e}")
     assert e1.max() <= 2 * tol, e1.max()
-    assert e1.max() <= e0.max()
+    assert e0.max() < 1e-3, e0.max()
 
 
 def test_refine_fused_temporal_coherence(coh60):
