@@ -573,6 +573,8 @@ def main():
             emi_tcoh(coh, N, 0, ph[s:e], q[s:e], tc[s:e])
             mark("emi1")
         state["p"], state["shp_num"], state["coh_last"] = p, shp_num, coh
+        if mr.pl.last_householder is not None:
+            state["householder_last"] = mr.pl.last_householder
 
     for _ in range(args.warmup):
         step()
@@ -673,6 +675,8 @@ def main():
     phase["abs_gamma_not_pd_frac"] = float((~mr.isPD(Gs.abs().to(torch.float64))).float().mean())
     phase["lambda_min_negative_frac"] = float((q[s0:s0 + m] < 0).float().mean())
     phase["k_below_n_frac"] = float((state["shp_num"].reshape(-1)[s0:s0 + m] < N).float().mean())
+    if "householder_last" in state:  # points of the last batch the shift-invert EMI left to Householder
+        phase["householder_pass_frac"] = int(state["householder_last"]) / coh.shape[0]
     del Gs
     phase["gpu_over_reference"] = {
         "p99": phase["rms_p99_rad"] / phase["reference_fp32_lapack"]["rms_p99_rad"],

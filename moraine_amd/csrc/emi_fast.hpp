@@ -463,15 +463,15 @@ __device__ __forceinline__ void emi_fast_point(const Src& coh_at, int n, int lan
   }
 }
 
-// Steps 1-2: M = inv(|Gamma|) o Gamma into mr_, mi_ (row `lane`).  Returns
-// whether |Gamma| is positive definite: the sweep pivots are the pivots of a
-// symmetric elimination with symmetric pivoting, so their signs are |Gamma|'s
-// inertia (Sylvester); all positive <=> PD.
+// Steps 1-2 of emi_fast_point for the shift-invert path: M = inv(|Gamma|) o
+// Gamma into mr_, mi_ (row `lane`).  Returns false, as soon as a sweep pivot is
+// not positive, when |Gamma| is not positive definite: the sweep pivots are
+// those of a symmetric elimination with symmetric pivoting, so their signs are
+// |Gamma|'s inertia (Sylvester); all positive <=> PD.
 template <int NM, class Src>
 __device__ __forceinline__ bool emi_fast_form_m(const Src& coh_at, int n, int lane, unsigned char* lds,
                                                 RegArr<float, NM>& mr_, RegArr<float, NM>& mi_) {
   double* colb = reinterpret_cast<double*>(lds);        // 64: sweep column broadcast
-  bool abs_pd = true;
   // ---- step 1: A = |Gamma| (row per lane, FP64), sweep inverse ----------
   RegArr<double, NM> a;
 #pragma unroll
@@ -501,7 +501,7 @@ __device__ __forceinline__ bool emi_fast_form_m(const Src& coh_at, int n, int la
     colb[lane] = (lane == k) ? c - 1.0 : c;
     __builtin_amdgcn_wave_barrier();
     const double dk = rl_d(c, k);
-    abs_pd = abs_pd && dk > 0.0;
+    if (!(dk > 0.0)) return false;  // |Gamma| not PD: the point goes to the Householder pass
     const double rd = rcp_d(dk);
     const double f = (lane == k) ? (1.0 - rd) : c * rd;
     {
@@ -551,7 +551,7 @@ __device__ __forceinline__ bool emi_fast_form_m(const Src& coh_at, int n, int la
     mi_.set(j, im);
     if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
   }
-  return abs_pd;
+  return true;
 }
 
 // Shift-invert formulation (|Gamma| PD): B = M - sigma I with sigma = 0.98
@@ -612,6 +612,8 @@ __device__ __forceinline__ bool emi_shift_invert(const Src& coh_at, int n, int l
     rd = fmaf(rd, fmaf(-d, rd, 1.f), rd);
     const float fr = (lane == k) ? 1.f - rd : cr * rd;
     const float fi = (lane == k) ? 0.f : ci * rd;
+    int nu_ = n;
+    __asm__ volatile("" : "+s"(nu_));  // column groups at or beyond n hold zeros: skipped by a scalar test
     {
       float2 cur[4], nxt[4];
 #pragma unroll
@@ -663,6 +665,8 @@ __device__ __forceinline__ bool emi_shift_invert(const Src& coh_at, int n, int l
     xb[lane] = make_float2(xr, xi);
     __builtin_amdgcn_wave_barrier();
     float ar = 0.f, ai = 0.f;
+    int nu_ = n;
+    __asm__ volatile("" : "+s"(nu_));
     {
       float2 cur[4], nxt[4];
 #pragma unroll
@@ -673,6 +677,7 @@ __device__ __forceinline__ bool emi_shift_invert(const Src& coh_at, int n, int l
 #pragma unroll
           for (int t = 0; t < 4; ++t) nxt[t] = xb[(g + 1) * 4 + t];
         }
+        if (g * 4 < nu_)
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
           const int j = g * 4 + t;

@@ -20,6 +20,9 @@ last_refined = None
 #: ... and an int32 device tensor whose first ``last_refined`` entries are those
 #: points' indices (in no particular order)
 last_refined_points = None
+#: after a plain EMI call with 32 < N <= 64: a 0-d int32 device tensor, the
+#: number of points the shift-invert pass left to the Householder pass
+last_householder = None
 
 
 def _check_refine(refine, n: int):
@@ -65,7 +68,14 @@ def _emi_dev(c: torch.Tensor, n: int, ref: int, tol=None):
     ws = _dev.workspace(wsb, c.device)
     _lib.call("mr_emi", _dev.ptr(c), npts, n, int(ref), _dev.ptr(ph), _dev.ptr(q), _dev.ptr(ws), ws.numel(),
               _dev.stream())
+    _note_householder(ws, n)
     return ph, q
+
+
+def _note_householder(ws: torch.Tensor, n: int):
+    global last_householder
+    if 32 < n <= 64:
+        last_householder = ws[:4].view(torch.int32)[0]
 
 
 def emi(coh, ref: int = 0, *, refine=None):
@@ -133,6 +143,7 @@ def _emi_tcoh_dev(c: torch.Tensor, n: int, ref: int, ph: torch.Tensor, q: torch.
     ws = _dev.workspace(wsb, c.device)
     _lib.call("mr_emi_ds_temp_coh", _dev.ptr(c), npts, n, int(ref), _dev.ptr(pairs), pairs.shape[0], _dev.ptr(ph),
               _dev.ptr(q), _dev.ptr(t), _dev.ptr(ws), ws.numel(), _dev.stream())
+    _note_householder(ws, n)
 
 
 def emi_ds_temp_coh(coh, ref: int = 0, *, refine=None):
