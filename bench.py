@@ -38,10 +38,6 @@ def emi_algorithmic_flops(n: int) -> float:
     return n ** 3 + (4.0 / 3.0) * n ** 3 + 2 * 8.0 * n ** 2
 
 
-def emi_executed_flops(n: int) -> float:
-    """Flops the built EMI kernel executes per point: 2N^3 (FP64 sweep inverse of
-    |Gamma|) + (16/3) N^3 (FP32 Householder tridiagonalisation of the Hermitian M)."""
-    return (2.0 + 16.0 / 3.0) * n ** 3
 
 
 def parse():
@@ -74,17 +70,17 @@ def progress(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def measured_traffic(kernel, args):
-    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE/WRITE_SIZE counter
-    summary (copy-kernel calibrated), valid only for the workload it was measured on."""
+def measured_traffic(kernels, args):
+    """HBM bytes per launch of the given kernels (summed) from the committed FETCH_SIZE/WRITE_SIZE
+    counter summary (copy-kernel calibrated), valid only for the workload it was measured on."""
     if (args.size, args.n, args.az, args.r, args.batch) != (2500, 60, 7, 10, 1 << 18):
         return None
     try:
         with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
-            t = json.load(f)["per_dispatch"][kernel]
+            per = json.load(f)["per_dispatch"]
+        return sum(per[k]["read_bytes"] + per[k]["write_bytes"] for k in kernels)
     except (OSError, KeyError, ValueError):
         return None
-    return t["read_bytes"] + t["write_bytes"]
 
 
 def ensure_built():
@@ -623,23 +619,26 @@ def main():
     dom = max(["ks_test+select_shp", "emperical_co_pc", "emi+ds_temp_coh"], key=lambda k: stages[k]["s"])
     if dom == "emi+ds_temp_coh":
         ach = emi_flops * (npx / nb) / (t_emi / nb) / 1e12
-        roof = {"kernel": "emi_fast_kernel<64, true, false>", "bound": "valu", "achieved": ach,
+        ek = ["emi_fast_kernel<64, true, false, 1>", "emi_fast_kernel<64, true, false, 2>"]
+        roof = {"kernel": " + ".join(ek) if 32 < N <= 64 else "emi_wide_kernel", "bound": "valu", "achieved": ach,
                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
-                "traffic": measured_traffic("emi_fast_kernel<64,tc>", args),
-                "executed_TFLOP/s": emi_executed_flops(N) * (npx / nb) / (t_emi / nb) / 1e12,
-                "note": "EMI with the DS temporal coherence fused; VALU-bound (no MFMA); achieved = algorithmic "
-                        "flops N^3 + (4/3)N^3 + 16N^2 per point over the kernel's HIP-event time; "
-                        "executed_TFLOP/s counts what the kernel runs (2N^3 FP64 sweep + (16/3)N^3 FP32 "
-                        "Householder); peak = MI355X FP64 = FP32 non-packed vector rate"}
+                "traffic": measured_traffic(ek, args),
+                "note": "EMI with the DS temporal coherence fused, VALU-bound (no MFMA): the shift-invert pass over "
+                        "every point and the Householder pass over the points it leaves, timed together with HIP "
+                        "events per batch; achieved = algorithmic flops N^3 + (4/3)N^3 + 16N^2 per point; the "
+                        "passes execute more (an FP64 sweep, 2N^3, and a complex FP32 sweep, 8N^3, in place of "
+                        "Cholesky factors); peak = MI355X FP64 = FP32 non-packed vector rate"}
     elif dom == "ks_test+select_shp":
         ach = ks_bytes * npx / t_ks / 1e9
-        roof = {"kernel": "ks_pairs_kernel<float, 4>", "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic("ks_pairs_kernel<float,4>", args)}
+        ks = ["ks_sort_kernel<float, 1, false>", "ks_pairs_kernel<float, 4, unsigned char>",
+              "ks_emit_kernel<float, unsigned char>"]
+        roof = {"kernel": " + ".join(ks), "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic(ks, args)}
     else:
         ach = co_flops * (npx / nb) / (t_co / nb) / 1e12
         roof = {"kernel": "co_mfma_kernel<false, true>", "bound": "mfma", "achieved": ach, "peak": 2500.0,
                 "unit": "TFLOP/s", "frac": ach / 2500.0,
-                "traffic": measured_traffic("co_mfma_kernel<pc,n<=64>", args)}
+                "traffic": measured_traffic(["co_mfma_kernel<false, true>"], args)}
 
     # phase error of the last batch vs an FP64 solve (torch/rocSOLVER, complex128) on a sample
     coh = state["coh_last"]
