@@ -571,6 +571,7 @@ def main():
         state["p"], state["shp_num"], state["coh_last"] = p, shp_num, coh
         if mr.pl.last_householder is not None:
             state["householder_last"] = mr.pl.last_householder
+            state["resolved_last"] = mr.pl.last_resolved
 
     for _ in range(args.warmup):
         step()
@@ -619,13 +620,14 @@ def main():
     dom = max(["ks_test+select_shp", "emperical_co_pc", "emi+ds_temp_coh"], key=lambda k: stages[k]["s"])
     if dom == "emi+ds_temp_coh":
         ach = emi_flops * (npx / nb) / (t_emi / nb) / 1e12
-        ek = ["emi_fast_kernel<64, true, false, 1>", "emi_fast_kernel<64, true, false, 2>"]
+        ek = ["emi_fast_kernel<64, true, false, 1>", "emi_fast_kernel<64, true, true, 2>",
+              "emi_general_kernel<true, HIP_vector_type<float, 2u> >"]
         roof = {"kernel": " + ".join(ek) if 32 < N <= 64 else "emi_wide_kernel", "bound": "valu", "achieved": ach,
                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
                 "traffic": measured_traffic(ek, args),
                 "note": "EMI with the DS temporal coherence fused, VALU-bound (no MFMA): the shift-invert pass over "
-                        "every point and the Householder pass over the points it leaves, timed together with HIP "
-                        "events per batch; achieved = algorithmic flops N^3 + (4/3)N^3 + 16N^2 per point; the "
+                        "every point, the Householder pass over the points it leaves and the FP64 re-solve of those "
+                        "the Householder pass flags, timed together with HIP events per batch; achieved = algorithmic flops N^3 + (4/3)N^3 + 16N^2 per point; the "
                         "passes execute more (an FP64 sweep, 2N^3, and a complex FP32 sweep, 8N^3, in place of "
                         "Cholesky factors); peak = MI355X FP64 = FP32 non-packed vector rate"}
     elif dom == "ks_test+select_shp":
@@ -676,6 +678,7 @@ def main():
     phase["k_below_n_frac"] = float((state["shp_num"].reshape(-1)[s0:s0 + m] < N).float().mean())
     if "householder_last" in state:  # points of the last batch the shift-invert EMI left to Householder
         phase["householder_pass_frac"] = int(state["householder_last"]) / coh.shape[0]
+        phase["fp64_resolve_frac"] = int(state["resolved_last"]) / coh.shape[0]
     del Gs
     phase["gpu_over_reference"] = {
         "p99": phase["rms_p99_rad"] / phase["reference_fp32_lapack"]["rms_p99_rad"],

@@ -5,10 +5,13 @@
 // ds_temp_coh per batch) over ragged per-pixel inputs, grouped by N by the
 // caller (one call per image count).
 //
-// 32 < n <= 64: one fused launch.  One wave per pixel accumulates the coherence on
+// 32 < n <= 64: fused launches.  One wave per pixel accumulates the coherence on
 // MFMA into LDS tiles (co_mfma.hpp), runs EMI reading the tiles (emi_fast.hpp),
 // and forms the temporal coherence from the same tiles (tcoh.hpp); the
-// coherence matrix never reaches HBM.  Every stage shares its code with the
+// coherence matrix never reaches HBM.  As in the plain EMI, a shift-invert
+// pass, a Householder pass over the pixels it declines (with the conditioning
+// test) and an FP64 re-solve of the pixels that test flags, each rebuilding
+// the pixel's tiles.  Every stage shares its code with the
 // drop-in kernels, so for a sample matrix made of a pixel's SHP stacks in
 // window order the outputs equal emperical_co_pc -> emi -> ds_temp_coh exactly.
 // n <= 32, n > 64 (or any n with blocks < 0, the per-stage mode): per chunk of pixels,
@@ -21,6 +24,7 @@
 // launch is built on, so those groups always take the per-stage path.)
 #include "co_mfma.hpp"
 #include "emi_fast.hpp"
+#include "emi_general.hpp"
 #include "tcoh.hpp"
 
 namespace mr {
@@ -37,7 +41,11 @@ int co_mfma_wide_samples_launch(const float2* samples, const long long* row_off,
 constexpr int BATCH_DEFAULT_BLOCKS = 1 << 15;
 constexpr int BATCH_FALLBACK_BLOCKS = 8192;
 
-__global__ void batch_zero_count_kernel(unsigned* c) { *c = 0u; }
+__global__ void batch_zero_count_kernel(unsigned* c) {
+  c[0] = 0u;
+  c[1] = 0u;
+}
+constexpr double BATCH_HH_TOL = 1e-3;  // the plain EMI's Householder-pass tolerance
 
 __host__ __device__ inline size_t batch_fused_lds(bool two) {
   return (size_t)com_tile_floats(two) * 4 + 128 * 4 /* scales */ + EMI_FAST_LDS_BYTES;
@@ -45,8 +53,9 @@ __host__ __device__ inline size_t batch_fused_lds(bool two) {
 
 // MODE 1: EMI by the shift-invert formulation; a pixel it declines is
 // appended to list[*count] and left unwritten.  MODE 2: the Householder
-// formulation for the pixels of that list (the plain kernels' two passes, so
-// the outputs equal the staged path's bit for bit).
+// formulation for the pixels of that list, appending those its conditioning
+// test flags to res_list[*res_count] (the plain kernels' passes, so the
+// outputs equal the staged path's bit for bit).
 template <int NM, bool TWO, int MODE>
 __global__ __launch_bounds__(64) void ds_batch_fused_kernel(const float2* __restrict__ samples,
                                                             const long long* __restrict__ row_off,
@@ -56,7 +65,8 @@ __global__ __launch_bounds__(64) void ds_batch_fused_kernel(const float2* __rest
                                                             int ref, const int32_t* __restrict__ pairs, int npairs,
                                                             float2* __restrict__ ph, float* __restrict__ quality,
                                                             float* __restrict__ tcoh, unsigned* __restrict__ count,
-                                                            int* __restrict__ list) {
+                                                            int* __restrict__ list, unsigned* __restrict__ res_count,
+                                                            int* __restrict__ res_list) {
   extern __shared__ __align__(16) unsigned char smem[];
   float2* tiles = reinterpret_cast<float2*>(smem);
   float* sc = reinterpret_cast<float*>(smem + (size_t)com_tile_floats(TWO) * 4);  // scales, 1/sqrt(diag)
@@ -79,7 +89,9 @@ __global__ __launch_bounds__(64) void ds_batch_fused_kernel(const float2* __rest
         continue;
       }
     } else {
-      emi_fast_point<NM>(ct, n, lane, work, vr, vi, rq);
+      bool flag = false;
+      emi_fast_point<NM, true>(ct, n, lane, work, vr, vi, rq, BATCH_HH_TOL, &flag);
+      if (flag && lane == 0) res_list[atomicAdd(res_count, 1u)] = (int)t;
     }
     const float2 pv = emi_link_phase(vr, vi, ref);
     __builtin_amdgcn_wave_barrier();
@@ -92,6 +104,50 @@ __global__ __launch_bounds__(64) void ds_batch_fused_kernel(const float2* __rest
     if (lane == 0) tcoh[id] = tc;
     __builtin_amdgcn_wave_barrier();
   }
+}
+
+// Coherence tiles of one pixel as the FP64 solve's source.
+struct BatchTileSrc {
+  CohTiles<true> ct;
+  __device__ __forceinline__ float2 operator()(int a, int b) const { return ct(a, b); }
+  __device__ __forceinline__ float2 pair(int, int a, int b) const { return ct(a, b); }
+};
+
+// The FP64 re-solve of the pixels res_list[0 .. *res_count): wave 0 rebuilds
+// the pixel's coherence tiles, then the block runs emi_general_solve (slice in
+// LDS) with the temporal coherence: the same values the plain EMI's re-solve
+// computes from the stored coherence row.
+__global__ __launch_bounds__(EMI_GEN_THREADS) void ds_batch_resolve_kernel(
+    const float2* __restrict__ samples, const long long* __restrict__ row_off, const int32_t* __restrict__ n_samp,
+    const long long* __restrict__ ph_off, const int32_t* __restrict__ pix, int n, int ref,
+    const int32_t* __restrict__ pairs, float2* __restrict__ ph, float* __restrict__ quality,
+    float* __restrict__ tcoh, const unsigned* __restrict__ res_count, const int* __restrict__ res_list) {
+  __shared__ double red[EMI_GEN_THREADS / 64];
+  __shared__ double sh_key[EMI_GEN_THREADS];
+  __shared__ int sh_idx[EMI_GEN_THREADS];
+  __shared__ double2 tri_de[64];
+  __shared__ double tri_e[64];
+  extern __shared__ __align__(16) unsigned char smem[];
+  float2* tiles = reinterpret_cast<float2*>(smem);
+  float* sc = reinterpret_cast<float*>(smem + (size_t)com_tile_floats(true) * 4);
+  unsigned char* slice = reinterpret_cast<unsigned char*>(sc + 128);
+  const GenShared sh{red, sh_key, sh_idx, tri_de, tri_e};
+  const long long nit = *res_count;
+  for (long long it = blockIdx.x; it < nit; it += gridDim.x) {
+    const int id = pix[res_list[it]];
+    if (threadIdx.x < 64) {
+      int lane = threadIdx.x;
+      __asm__ volatile("" : "+v"(lane));
+      co_mfma_point<true>(samples + row_off[id], RowIdentity{}, n_samp[id], n, lane, tiles, sc);
+    }
+    __syncthreads();
+    const BatchTileSrc src{CohTiles<true>{tiles, sc + 64}};
+    emi_general_solve<true, float2>(src, n, ref, true, slice, sh, ph + ph_off[id], quality + id, pairs, tcoh + id);
+  }
+}
+
+__host__ __device__ inline size_t batch_resolve_lds(int n) {
+  return (size_t)com_tile_floats(true) * 4 + 128 * 4 + emi_gen_slice_bytes(n);
 }
 
 __global__ void ds_batch_scatter_kernel(const float2* __restrict__ ph_t, const float* __restrict__ q_t,
@@ -119,7 +175,7 @@ static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 extern "C" {
 
 size_t mr_ds_batch_fused_workspace_bytes(long long n_pix) {
-  return 256 + (((size_t)(n_pix > 0 ? n_pix : 0) * 4 + 255) & ~(size_t)255);
+  return 256 + 2 * ((((size_t)(n_pix > 0 ? n_pix : 0) * 4 + 255) & ~(size_t)255));
 }
 
 size_t mr_ds_batch_workspace_bytes(long long chunk, int n) {
@@ -149,19 +205,27 @@ int mr_ds_batch(const float* samples, const long long* row_off, const int32_t* n
     MR_CHECK_ARG(workspace && workspace_bytes >= need, "ds_batch: fused launch workspace %zu < %zu bytes",
                  workspace_bytes, need);
     unsigned* count = reinterpret_cast<unsigned*>(workspace);
+    unsigned* res_count = count + 1;
+    const size_t span = ((size_t)n_pix * 4 + 255) & ~(size_t)255;
     int* list = reinterpret_cast<int*>(reinterpret_cast<unsigned char*>(workspace) + 256);
+    int* res_list = reinterpret_cast<int*>(reinterpret_cast<unsigned char*>(workspace) + 256 + span);
     hipLaunchKernelGGL(mr::batch_zero_count_kernel, dim3(1), dim3(1), 0, st, count);
     if (int rc = mr::launch_status("ds_batch count init")) return rc;
     const int nb = (int)std::min<long long>(n_pix, blocks > 0 ? blocks : mr::BATCH_DEFAULT_BLOCKS);
     hipLaunchKernelGGL((mr::ds_batch_fused_kernel<64, true, 1>), dim3(nb), dim3(64), mr::batch_fused_lds(true), st,
                        smp, row_off, n_samples, ph_off, pix, n_pix, n, ref, pairs, n_pairs, phc, quality, t_coh, count,
-                       list);
+                       list, res_count, res_list);
     if (int rc = mr::launch_status("ds_batch shift-invert")) return rc;
     const int nb2 = (int)std::min<long long>(n_pix, mr::BATCH_FALLBACK_BLOCKS);
     hipLaunchKernelGGL((mr::ds_batch_fused_kernel<64, true, 2>), dim3(nb2), dim3(64), mr::batch_fused_lds(true), st,
                        smp, row_off, n_samples, ph_off, pix, n_pix, n, ref, pairs, n_pairs, phc, quality, t_coh, count,
-                       list);
-    return mr::launch_status("ds_batch householder");
+                       list, res_count, res_list);
+    if (int rc = mr::launch_status("ds_batch householder")) return rc;
+    const int nb3 = (int)std::min<long long>(n_pix, mr::EMI_GEN_BLOCKS);
+    hipLaunchKernelGGL(mr::ds_batch_resolve_kernel, dim3(nb3), dim3(mr::EMI_GEN_THREADS), mr::batch_resolve_lds(n),
+                       st, smp, row_off, n_samples, ph_off, pix, n, ref, pairs, phc, quality, t_coh, res_count,
+                       res_list);
+    return mr::launch_status("ds_batch FP64 re-solve");
   }
   MR_CHECK_ARG(chunk > 0, "ds_batch: chunk must be positive for n <= 32, n > 64 or the per-stage mode");
   const size_t need = mr_ds_batch_workspace_bytes(chunk, n);

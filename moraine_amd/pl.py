@@ -23,6 +23,9 @@ last_refined_points = None
 #: after a plain EMI call with 32 < N <= 64: a 0-d int32 device tensor, the
 #: number of points the shift-invert pass left to the Householder pass
 last_householder = None
+#: ... and a 0-d int32 device tensor, the number of those the Householder
+#: pass's conditioning test (phase tolerance 1e-3 rad) sent to the FP64 re-solve
+last_resolved = None
 
 
 def _check_refine(refine, n: int):
@@ -73,9 +76,10 @@ def _emi_dev(c: torch.Tensor, n: int, ref: int, tol=None):
 
 
 def _note_householder(ws: torch.Tensor, n: int):
-    global last_householder
+    global last_householder, last_resolved
     if 32 < n <= 64:
         last_householder = ws[:4].view(torch.int32)[0]
+        last_resolved = ws[4:8].view(torch.int32)[0]
 
 
 def emi(coh, ref: int = 0, *, refine=None):

@@ -191,3 +191,27 @@ def test_batched_edges():
     assert ph1[0].shape == (60,) and np.allclose(np.abs(ph1[0]), 1, atol=1e-5)
     with pytest.raises(ValueError):
         DSBatchRunner(DSBatch.from_matrices([np.zeros((5, 1), np.complex64)]))
+
+
+def test_fused_resolve_equals_per_stage():
+    """N = 60 pixels with fewer SHPs than images (K in [20, 59]: |Gamma| near
+    singular): the fused launch re-solves the pixels its Householder pass flags
+    from tiles rebuilt on chip, the per-stage path from the stored coherence
+    rows; the outputs are identical, and some pixels take that path."""
+    import torch
+
+    from moraine_amd.batched import DSBatch, DSBatchRunner
+
+    s, ro, ni, ns = synth.ds_samples_torch(6000, n_choices=(60,), k_range=(20, 59), seed=11, device="cuda")
+    b = DSBatch.from_tensors(s, ro, ni, ns)
+    _case("ds_batch fused re-solve vs per-stage, N=60, 6000 pixels, K in [20, 59]")
+    r = DSBatchRunner(b)
+    a = [x.clone() for x in r.run()]
+    n_hh, n_res = (int(v) for v in r.ws[:8].view(torch.int32).cpu())
+    print(f"  Householder pass {n_hh}, FP64 re-solve {n_res} of 6000")
+    assert 0 < n_res <= n_hh
+    c = DSBatchRunner(b, per_stage=True).run()
+    torch.cuda.synchronize()
+    assert torch.equal(a[0].view(torch.int64), c[0].view(torch.int64))
+    assert torch.equal(a[2].view(torch.int32), c[2].view(torch.int32))
+    assert torch.equal(a[3].view(torch.int32), c[3].view(torch.int32))
