@@ -97,9 +97,10 @@ constexpr int EMI_GEN_BLOCKS = 1024;
 constexpr int EMI_REF_BLOCKS = 1024;  // FP64 re-solve blocks for n > 64, four per CU (each with a global slice)
 
 __host__ __device__ inline size_t emi_gen_slice_bytes(int n) {
-  // A (n*n f64) + M (n*n c128) + vectors (col, u, q, p: n c128 each) + 12 n-double arrays
-  // (dg, td, tae, te2, ty, dphr, dphi, 5n LU work) + n ints (swept)
-  return ((size_t)n * n * 8 + (size_t)n * n * 16 + (size_t)n * 16 * 4 + (size_t)n * 8 * 13 + 255) & ~(size_t)255;
+  // M (n*n c128; A, n*n f64, in its first half until M is formed) + vectors (col, u, q, p:
+  // n c128 each) + 12 n-double arrays (dg, td, tae, te2, ty, dphr, dphi, 5n LU work) + n ints
+  // (swept).  n = 64: 76 288 bytes, two blocks per CU.
+  return ((size_t)n * n * 16 + (size_t)n * 16 * 4 + (size_t)n * 8 * 13 + 255) & ~(size_t)255;
 }
 
 __device__ __forceinline__ double block_sum(double v, double* red) {
@@ -111,6 +112,26 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   double s = 0.0;
   for (int i = 0; i < (int)(blockDim.x >> 6); ++i) s += red[i];
   return s;
+}
+
+// Two block sums in one exchange (each the same sum, in the same order, as block_sum's).
+__device__ __forceinline__ void block_sum2(double& a, double& b, double* red2) {
+  a = wave_sum(a);
+  b = wave_sum(b);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) {
+    red2[2 * w] = a;
+    red2[2 * w + 1] = b;
+  }
+  __syncthreads();
+  double sa = 0.0, sb = 0.0;
+  for (int i = 0; i < (int)(blockDim.x >> 6); ++i) {
+    sa += red2[2 * i];
+    sb += red2[2 * i + 1];
+  }
+  a = sa;
+  b = sb;
 }
 
 // Static LDS of a block running emi_general_solve.
@@ -138,7 +159,9 @@ struct GenRow {
 // eigenpair of T; otherwise `base` is a global slice and the block reduces
 // the pivot search, one thread bisects.  Writes ph_row[0..n) and *q_out; TC
 // (n <= 64, Z = float2): also the DS temporal coherence of the new phases
-// over the n(n-1)/2 pairs into *t_out.
+// over the n(n-1)/2 pairs into *t_out.  src is read twice (|Gamma| for the
+// sweep, then Gamma to form M in A's place).  Element loops run rows over the
+// block's waves and columns over lanes (no index division).
 template <bool TC, typename Z, class Src>
 __device__ void emi_general_solve(const Src& src, int n, int ref, bool small, unsigned char* base,
                                   const GenShared& sh, Z* ph_row, float* q_out, const int32_t* pairs, float* t_out) {
@@ -150,8 +173,10 @@ __device__ void emi_general_solve(const Src& src, int n, int ref, bool small, un
   double* tri_e = sh.tri_e;
   const int P = n * (n - 1) / 2;
   const int tid = threadIdx.x;
-  double* A = reinterpret_cast<double*>(base);
-  c128* M = reinterpret_cast<c128*>(A + (size_t)n * n);
+  const int wv = tid >> 6, ln = tid & 63;
+  constexpr int NW = EMI_GEN_THREADS / 64;
+  double* A = reinterpret_cast<double*>(base);  // the first half of M's space
+  c128* M = reinterpret_cast<c128*>(base);
   c128* col = M + (size_t)n * n;  // n
   c128* u = col + n;
   c128* q = u + n;
@@ -165,21 +190,16 @@ __device__ void emi_general_solve(const Src& src, int n, int ref, bool small, un
   double* dphi = dphr + n;
   double* wk = dphi + n;  // 5n
   int* swept = reinterpret_cast<int*>(wk + 5 * n);
-  for (int t = tid; t < n * n; t += EMI_GEN_THREADS) {
-    const int i = t / n, j = t - (t / n) * n;
-    c128 g;
-    if (i == j) {
-      g = {1.0, 0.0};
-    } else if (j > i) {
-      const Z c = src(i, j);
-      g = {c.x, c.y};
-    } else {
-      const Z c = src(j, i);
-      g = {c.x, -c.y};
+  for (int i = wv; i < n; i += NW)
+    for (int j = ln; j < n; j += 64) {
+      double a = 1.0;
+      if (i != j) {
+        const Z c = i < j ? src(i, j) : src(j, i);
+        const double gr = c.x, gi = c.y;
+        a = sqrt(gr * gr + gi * gi);
+      }
+      A[(size_t)i * n + j] = a;
     }
-    M[t] = g;
-    A[t] = sqrt(g.re * g.re + g.im * g.im);
-  }
   for (int i = tid; i < n; i += EMI_GEN_THREADS) {
     dg[i] = 1.0;
     swept[i] = 0;
@@ -222,10 +242,9 @@ __device__ void emi_general_solve(const Src& src, int n, int ref, bool small, un
     for (int i = tid; i < n; i += EMI_GEN_THREADS) col[i].re = A[(size_t)i * n + k];
     __syncthreads();
     const double rd = 1.0 / col[k].re;
-    for (int t = tid; t < n * n; t += EMI_GEN_THREADS) {
-      const int i = t / n, j = t - (t / n) * n;
+    for (int i = wv; i < n; i += NW) {
       const double f = (i == k) ? (1.0 - rd) : col[i].re * rd;
-      A[t] = fma(-f, col[j].re, A[t]);
+      for (int j = ln; j < n; j += 64) A[(size_t)i * n + j] = fma(-f, col[j].re, A[(size_t)i * n + j]);
     }
     __syncthreads();
     for (int i = tid; i < n; i += EMI_GEN_THREADS) {
@@ -236,10 +255,32 @@ __device__ void emi_general_solve(const Src& src, int n, int ref, bool small, un
     }
     __syncthreads();
   }
-  // M = inv(A) o Gamma  (A holds -inv)
-  for (int t = tid; t < n * n; t += EMI_GEN_THREADS) {
-    const double ai = -A[t];
-    M[t] = {ai * M[t].re, ai * M[t].im};
+  // M = inv(A) o Gamma (A holds -inv), written over A: chunks of EMI_GEN_THREADS
+  // entries from the last down; chunk c's entries of M cover A's entries 2t, 2t + 1,
+  // in chunks above c, so every A entry is read before it is overwritten
+  {
+    const int nn = n * n;
+    for (int c = (nn - 1) / EMI_GEN_THREADS; c >= 0; --c) {
+      const int t = c * EMI_GEN_THREADS + tid;
+      double ai = 0.0, gr = 0.0, gi = 0.0;
+      if (t < nn) {
+        const int i = t / n, j = t - (t / n) * n;
+        ai = -A[t];
+        if (i == j) {
+          gr = 1.0;
+        } else if (j > i) {
+          const Z g = src(i, j);
+          gr = g.x;
+          gi = g.y;
+        } else {
+          const Z g = src(j, i);
+          gr = g.x;
+          gi = -g.y;
+        }
+      }
+      __syncthreads();
+      if (t < nn) M[t] = {ai * gr, ai * gi};
+    }
   }
   __syncthreads();
   // Householder tridiagonalisation (FP64); reflector k kept in column k below the diagonal
@@ -273,15 +314,20 @@ __device__ void emi_general_solve(const Src& src, int n, int ref, bool small, un
       u[i] = v;
     }
     __syncthreads();
-    for (int i = tid; i < n; i += EMI_GEN_THREADS) {
+    // p = M u: four threads per row (columns k+1+r, k+5+r, ...), summed across the four
+    for (int i = tid >> 2; i < n; i += EMI_GEN_THREADS / 4) {
       c128 p = {0.0, 0.0};
       if (i > k)
-        for (int j = k + 1; j < n; ++j) {
+        for (int j = k + 1 + (tid & 3); j < n; j += 4) {
           const c128 m = M[(size_t)i * n + j];
           p.re += m.re * u[j].re - m.im * u[j].im;
           p.im += m.re * u[j].im + m.im * u[j].re;
         }
-      pv[i] = p;
+      p.re += __shfl_xor(p.re, 1);
+      p.im += __shfl_xor(p.im, 1);
+      p.re += __shfl_xor(p.re, 2);
+      p.im += __shfl_xor(p.im, 2);
+      if ((tid & 3) == 0) pv[i] = p;
     }
     __syncthreads();
     double cp = 0.0;
@@ -290,14 +336,13 @@ __device__ void emi_general_solve(const Src& src, int n, int ref, bool small, un
     for (int i = tid; i < n; i += EMI_GEN_THREADS)
       q[i] = (i > k) ? c128{2.0 * (pv[i].re - cval * u[i].re), 2.0 * (pv[i].im - cval * u[i].im)} : c128{0.0, 0.0};
     __syncthreads();
-    const int m = n - k - 1;
-    for (int t = tid; t < m * m; t += EMI_GEN_THREADS) {
-      const int i = k + 1 + t / m, j = k + 1 + (t - (t / m) * m);
-      c128 v = M[(size_t)i * n + j];
-      // -= u_i conj(q_j) + q_i conj(u_j)
-      v.re -= u[i].re * q[j].re + u[i].im * q[j].im + q[i].re * u[j].re + q[i].im * u[j].im;
-      v.im -= u[i].im * q[j].re - u[i].re * q[j].im + q[i].im * u[j].re - q[i].re * u[j].im;
-      M[(size_t)i * n + j] = v;
+    for (int i = k + 1 + wv; i < n; i += NW)
+      for (int j = k + 1 + ln; j < n; j += 64) {
+        c128 v = M[(size_t)i * n + j];
+        // -= u_i conj(q_j) + q_i conj(u_j)
+        v.re -= u[i].re * q[j].re + u[i].im * q[j].im + q[i].re * u[j].re + q[i].im * u[j].im;
+        v.im -= u[i].im * q[j].re - u[i].re * q[j].im + q[i].im * u[j].re - q[i].re * u[j].im;
+        M[(size_t)i * n + j] = v;
     }
     for (int i = tid; i < n; i += EMI_GEN_THREADS)
       if (i > k) M[(size_t)i * n + k] = u[i];
@@ -382,8 +427,7 @@ __device__ void emi_general_solve(const Src& src, int n, int ref, bool small, un
       sr += uu.re * pv[i].re + uu.im * pv[i].im;
       si += uu.re * pv[i].im - uu.im * pv[i].re;
     }
-    sr = block_sum(sr, red);
-    si = block_sum(si, red);
+    block_sum2(sr, si, sh_key);  // sh_key: free after the sweep
     for (int i = k + 1 + tid; i < n; i += EMI_GEN_THREADS) {
       const c128 uu = M[(size_t)i * n + k];
       pv[i].re -= 2.0 * (uu.re * sr - uu.im * si);
