@@ -39,12 +39,14 @@ constexpr int EMI_FAST_THREADS = 64;
 
 // TC: also the DS temporal coherence of each point from the staged row
 // (moraine/pl.py:126-145), as mr_ds_temp_coh computes it from (coh, ph, pairs).
-// CHK: points whose conditioning test fails (tau) are appended to
-// flag_list[*flag_count] (MODE 2: res_list[*res_count]) for the FP64 re-solve.
-// MODE 0: the Householder formulation for every point (with CHK: the precision
-// form's flags).  MODE 1: the shift-invert formulation; a point it declines is
-// appended to flag_list[*flag_count] and left unwritten.  MODE 2: the
-// Householder formulation for the points of that list.
+// CHK: points whose check fails (tau) are appended to flag_list[*flag_count]
+// (MODE 1, 2: res_list[*res_count]) for the FP64 re-solve; MODE 0 and 2 test
+// the conditioning of the Householder solve, MODE 1 the FP64 eigen-residual
+// (each block keeps inv(|Gamma|) in its slot of slot_doubles doubles at xslots).
+// MODE 0: the Householder formulation for every point.  MODE 1: the shift-invert
+// formulation; a point it declines is appended to flag_list[*flag_count] and
+// left unwritten.  MODE 2: the Householder formulation for the points of that
+// list.
 template <int NM, bool TC, bool CHK = false, int MODE = 0>
 __global__ __launch_bounds__(EMI_FAST_THREADS) void emi_fast_kernel(const float2* __restrict__ coh, long long npts,
                                                                     int n, int ref, float2* __restrict__ ph,
@@ -54,7 +56,9 @@ __global__ __launch_bounds__(EMI_FAST_THREADS) void emi_fast_kernel(const float2
                                                                     unsigned* __restrict__ flag_count = nullptr,
                                                                     int* __restrict__ flag_list = nullptr,
                                                                     unsigned* __restrict__ res_count = nullptr,
-                                                                    int* __restrict__ res_list = nullptr) {
+                                                                    int* __restrict__ res_list = nullptr,
+                                                                    double* __restrict__ xslots = nullptr,
+                                                                    int slot_doubles = 0) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int P = n * (n - 1) / 2;
   const int lane0 = threadIdx.x;
@@ -90,7 +94,13 @@ __global__ __launch_bounds__(EMI_FAST_THREADS) void emi_fast_kernel(const float2
     double rq;
     bool flag = false;
     if constexpr (MODE == 1) {
-      if (!emi_inv_point<NM>(CohPacked{cs, n}, n, lane, work, vr, vi, rq)) {
+      bool ok;
+      if constexpr (CHK)
+        ok = emi_inv_point_checked<NM>(CohPacked{cs, n}, n, lane, work, vr, vi, rq,
+                                       xslots + (size_t)blockIdx.x * slot_doubles, tau, flag);
+      else
+        ok = emi_inv_point<NM>(CohPacked{cs, n}, n, lane, work, vr, vi, rq);
+      if (!ok) {
         if (lane == 0) flag_list[atomicAdd(flag_count, 1u)] = (int)pt;
         __builtin_amdgcn_wave_barrier();
         continue;
@@ -103,7 +113,7 @@ __global__ __launch_bounds__(EMI_FAST_THREADS) void emi_fast_kernel(const float2
     if (lane < n) ph[pt * n + lane] = pv;
     if (lane == 0) quality[pt] = (float)rq;
     if (CHK && flag && lane == 0) {
-      if constexpr (MODE == 2)
+      if constexpr (MODE != 0)
         res_list[atomicAdd(res_count, 1u)] = (int)pt;
       else
         flag_list[atomicAdd(flag_count, 1u)] = (int)pt;
@@ -185,34 +195,56 @@ static size_t emi_fast_lds(int n, int nm) {
 
 // 32 < n <= 64: the shift-invert pass over every point, then the Householder
 // pass over the points it left (|Gamma| indefinite, sigma not below lambda_min,
-// or no convergence: the poorly conditioned points).  The Householder pass
-// runs the precision form's conditioning test at EMI_HH_TOL and the points it
-// flags are solved again in FP64 (emi_general_kernel<true>, slice in LDS).
+// or no convergence: the poorly conditioned points), then the FP64 re-solve
+// (emi_general_kernel<true>, slice in LDS) of the points the checks flag.  The
+// Householder pass tests its conditioning at tau2 (EMI_HH_TOL in the plain
+// EMI, the caller's tolerance in the precision form); with xslots (precision
+// form) the shift-invert pass also checks its FP64 eigen-residual at tau1.
 constexpr int EMI_FALLBACK_BLOCKS = 8192;
 constexpr double EMI_HH_TOL = 1e-3;
+constexpr int EMI_SI_SLOTS = 8192;  // blocks of the checked shift-invert pass, one slot each
+inline int emi_slot_doubles(int n) { return ((n * (n + 1) / 2) + 31) & ~31; }
+inline size_t emi_slots_bytes(long long npts, int n) {
+  return (size_t)std::min<long long>(npts > 0 ? npts : 1, EMI_SI_SLOTS) * emi_slot_doubles(n) * 8;
+}
 template <bool TC>
 static int emi_two_pass(const float2* c, long long n_pts, int n, int ref, float2* p, float* quality,
-                        const int32_t* pairs, float* t_coh, unsigned char* ws, hipStream_t st) {
-  unsigned* count = reinterpret_cast<unsigned*>(ws);
-  unsigned* res_count = count + 1;
-  int* list = reinterpret_cast<int*>(ws + 256);
-  int* res_list = reinterpret_cast<int*>(ws + 256 + emi_list_span(n_pts));
+                        const int32_t* pairs, float* t_coh, unsigned* count, int* list, unsigned* res_count,
+                        int* res_list, double* xslots, double tau1, double tau2, hipStream_t st) {
   const size_t lds = emi_fast_lds(n, 64);
-  hipLaunchKernelGGL(emi_zero_count_kernel, dim3(1), dim3(1), 0, st, count);
+  hipLaunchKernelGGL(emi_zero_count_kernel, dim3(1), dim3(1), 0, st, count < res_count ? count : res_count);
   if (int rc = launch_status("emi count init")) return rc;
-  const int nb = (int)std::min<long long>(n_pts, 1 << 20);
-  hipLaunchKernelGGL((emi_fast_kernel<64, TC, false, 1>), dim3(nb), dim3(EMI_FAST_THREADS), lds, st, c, n_pts, n,
-                     ref, p, quality, pairs, t_coh, 0.0, count, list, nullptr, nullptr);
+  if (xslots != nullptr) {
+    const int nb = (int)std::min<long long>(n_pts, EMI_SI_SLOTS);
+    hipLaunchKernelGGL((emi_fast_kernel<64, TC, true, 1>), dim3(nb), dim3(EMI_FAST_THREADS), lds, st, c, n_pts, n,
+                       ref, p, quality, pairs, t_coh, tau1, count, list, res_count, res_list, xslots,
+                       emi_slot_doubles(n));
+  } else {
+    const int nb = (int)std::min<long long>(n_pts, 1 << 20);
+    hipLaunchKernelGGL((emi_fast_kernel<64, TC, false, 1>), dim3(nb), dim3(EMI_FAST_THREADS), lds, st, c, n_pts, n,
+                       ref, p, quality, pairs, t_coh, 0.0, count, list, nullptr, nullptr, nullptr, 0);
+  }
   if (int rc = launch_status("emi_fast shift-invert")) return rc;
   const int nb2 = (int)std::min<long long>(n_pts, EMI_FALLBACK_BLOCKS);
   hipLaunchKernelGGL((emi_fast_kernel<64, TC, true, 2>), dim3(nb2), dim3(EMI_FAST_THREADS), lds, st, c, n_pts, n,
-                     ref, p, quality, pairs, t_coh, EMI_HH_TOL, count, list, res_count, res_list);
+                     ref, p, quality, pairs, t_coh, tau2, count, list, res_count, res_list, nullptr, 0);
   if (int rc = launch_status("emi_fast householder")) return rc;
   const int nb3 = (int)std::min<long long>(n_pts, EMI_GEN_BLOCKS);
   hipLaunchKernelGGL(emi_general_kernel<true>, dim3(nb3), dim3(EMI_GEN_THREADS), emi_gen_slice_bytes(n), st, c,
                      n_pts, n, ref, p, quality, nullptr, res_list, res_count, TC ? pairs : nullptr,
                      TC ? t_coh : nullptr);
   return launch_status("emi FP64 re-solve");
+}
+
+// The plain two-pass path's workspace: the Householder list's count at ws[0],
+// the re-solve count at ws[4], the lists after the first 256 bytes.
+template <bool TC>
+static int emi_two_pass_plain(const float2* c, long long n_pts, int n, int ref, float2* p, float* quality,
+                              const int32_t* pairs, float* t_coh, unsigned char* ws, hipStream_t st) {
+  unsigned* count = reinterpret_cast<unsigned*>(ws);
+  return emi_two_pass<TC>(c, n_pts, n, ref, p, quality, pairs, t_coh, count, reinterpret_cast<int*>(ws + 256),
+                          count + 1, reinterpret_cast<int*>(ws + 256 + emi_list_span(n_pts)), nullptr, 0.0, EMI_HH_TOL,
+                          st);
 }
 
 }  // namespace mr
@@ -240,8 +272,8 @@ int mr_emi(const float* coh, long long n_pts, int n, int ref, float* ph, float* 
     const size_t need = mr_emi_workspace_bytes(n_pts, n);
     MR_CHECK_ARG(workspace != nullptr && workspace_bytes >= need, "emi: workspace %zu < %zu bytes", workspace_bytes,
                  need);
-    return mr::emi_two_pass<false>(c, n_pts, n, ref, p, quality, nullptr, nullptr,
-                                   reinterpret_cast<unsigned char*>(workspace), st);
+    return mr::emi_two_pass_plain<false>(c, n_pts, n, ref, p, quality, nullptr, nullptr,
+                                         reinterpret_cast<unsigned char*>(workspace), st);
   }
   if (n <= mr::EMI_WIDE_MAX_N) return mr::emi_wide_launch(c, n_pts, n, ref, p, quality, st);
   const size_t need = mr_emi_workspace_bytes(n_pts, n);
@@ -291,9 +323,9 @@ int mr_emi_ds_temp_coh(const float* coh, long long n_pts, int n, int ref, const 
     const size_t need = mr_emi_workspace_bytes(n_pts, n);
     MR_CHECK_ARG(workspace != nullptr && workspace_bytes >= need, "emi_ds_temp_coh: workspace %zu < %zu bytes",
                  workspace_bytes, need);
-    return mr::emi_two_pass<true>(reinterpret_cast<const float2*>(coh), n_pts, n, ref, reinterpret_cast<float2*>(ph),
-                                  quality, pairs, t_coh, reinterpret_cast<unsigned char*>(workspace),
-                                  mr::as_stream(stream));
+    return mr::emi_two_pass_plain<true>(reinterpret_cast<const float2*>(coh), n_pts, n, ref,
+                                        reinterpret_cast<float2*>(ph), quality, pairs, t_coh,
+                                        reinterpret_cast<unsigned char*>(workspace), mr::as_stream(stream));
   }
   const int rc = mr_emi(coh, n_pts, n, ref, ph, quality, workspace, workspace_bytes, stream);
   if (rc != MR_OK) return rc;
@@ -302,6 +334,8 @@ int mr_emi_ds_temp_coh(const float* coh, long long n_pts, int n, int ref, const 
 
 size_t mr_emi_refined_workspace_bytes(long long n_pts, int n) {
   const size_t list = 256 + (((size_t)(n_pts > 0 ? n_pts : 0) * 4 + 255) & ~(size_t)255);
+  // 32 < n <= 64: the Householder-pass list and the shift-invert pass's FP64 slots
+  if (n > 32 && n <= 64) return list + mr::emi_list_span(n_pts) + mr::emi_slots_bytes(n_pts, n);
   if (n <= 64) return list;
   // n > 64: the FP64 re-solve's per-block slices in global memory
   const long long nb = n_pts < mr::EMI_REF_BLOCKS ? n_pts : mr::EMI_REF_BLOCKS;
@@ -325,33 +359,37 @@ int mr_emi_refined(const float* coh, long long n_pts, int n, int ref, double tol
                workspace_bytes, need);
   hipStream_t st = mr::as_stream(stream);
   unsigned char* w = reinterpret_cast<unsigned char*>(workspace);
+  // the re-solve count at w[0] and its list after the first 256 bytes (every n)
   unsigned* count = reinterpret_cast<unsigned*>(w);
   int* list = reinterpret_cast<int*>(w + 256);
-  hipLaunchKernelGGL(mr::emi_zero_count_kernel, dim3(1), dim3(1), 0, st, count);
-  if (int rc = mr::launch_status("emi_refined")) return rc;
   const float2* c = reinterpret_cast<const float2*>(coh);
   float2* p = reinterpret_cast<float2*>(ph);
-  const bool fused = t_coh != nullptr && n > 32 && n <= 64;
   if (n > 32 && n <= 64) {
-    const size_t lds = mr::emi_fast_lds(n, 64);
-    const int nb = (int)std::min<long long>(n_pts, 1 << 20);
-    if (fused)
-      hipLaunchKernelGGL((mr::emi_fast_kernel<64, true, true>), dim3(nb), dim3(mr::EMI_FAST_THREADS), lds, st, c,
-                         n_pts, n, ref, p, quality, pairs, t_coh, tol, count, list);
-    else
-      hipLaunchKernelGGL((mr::emi_fast_kernel<64, false, true>), dim3(nb), dim3(mr::EMI_FAST_THREADS), lds, st, c,
-                         n_pts, n, ref, p, quality, nullptr, nullptr, tol, count, list);
-    if (int rc = mr::launch_status("emi_refined")) return rc;
-  } else {
-    if (int rc = mr::emi_wide_launch(c, n_pts, n, ref, p, quality, st, tol, count, list)) return rc;
+    // the two-pass path with both checks at tol (the shift-invert pass's FP64
+    // eigen-residual, the Householder pass's conditioning test)
+    const size_t span = mr::emi_list_span(n_pts);
+    unsigned* hh_count = count + 1;
+    int* hh_list = reinterpret_cast<int*>(w + 256 + span);
+    double* xslots = reinterpret_cast<double*>(w + 256 + 2 * span);
+    // (the Householder pass at min(tol, EMI_HH_TOL): never looser than the plain EMI, so
+    // the points this leaves unflagged keep the plain call's outputs)
+    const double tol2 = std::min(tol, mr::EMI_HH_TOL);
+    if (t_coh != nullptr)
+      return mr::emi_two_pass<true>(c, n_pts, n, ref, p, quality, pairs, t_coh, hh_count, hh_list, count, list,
+                                    xslots, tol, tol2, st);
+    return mr::emi_two_pass<false>(c, n_pts, n, ref, p, quality, nullptr, nullptr, hh_count, hh_list, count, list,
+                                   xslots, tol, tol2, st);
   }
+  hipLaunchKernelGGL(mr::emi_zero_count_kernel, dim3(1), dim3(1), 0, st, count);
+  if (int rc = mr::launch_status("emi_refined")) return rc;
+  if (int rc = mr::emi_wide_launch(c, n_pts, n, ref, p, quality, st, tol, count, list)) return rc;
   // FP64 re-solve of the flagged points, one block per point (slice in LDS for
-  // n <= 64, in the workspace after the list above)
+  // n <= 32, in the workspace after the list above for n > 64)
   if (n <= 64) {
     const int nb2 = (int)std::min<long long>(n_pts, mr::EMI_GEN_BLOCKS);
     hipLaunchKernelGGL(mr::emi_general_kernel<true>, dim3(nb2), dim3(mr::EMI_GEN_THREADS),
-                       mr::emi_gen_slice_bytes(n), st, c, n_pts, n, ref, p, quality, nullptr, list, count, pairs,
-                       fused ? t_coh : nullptr);
+                       mr::emi_gen_slice_bytes(n), st, c, n_pts, n, ref, p, quality, nullptr, list, count, nullptr,
+                       nullptr);
   } else {
     const int nb2 = (int)std::min<long long>(n_pts, mr::EMI_REF_BLOCKS);
     unsigned char* slices = w + 256 + (((size_t)n_pts * 4 + 255) & ~(size_t)255);
@@ -359,7 +397,7 @@ int mr_emi_refined(const float* coh, long long n_pts, int n, int ref, double tol
                        p, quality, slices, list, count, nullptr, nullptr);
   }
   if (int rc = mr::launch_status("emi_refined")) return rc;
-  if (t_coh && !fused) return mr_ds_temp_coh(coh, ph, n_pts, n, pairs, n_pairs, t_coh, stream);
+  if (t_coh) return mr_ds_temp_coh(coh, ph, n_pts, n, pairs, n_pairs, t_coh, stream);
   return MR_OK;
 }
 

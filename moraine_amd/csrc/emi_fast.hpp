@@ -470,7 +470,8 @@ __device__ __forceinline__ void emi_fast_point(const Src& coh_at, int n, int lan
 // |Gamma|'s inertia (Sylvester); all positive <=> PD.
 template <int NM, class Src>
 __device__ __forceinline__ bool emi_fast_form_m(const Src& coh_at, int n, int lane, unsigned char* lds,
-                                                RegArr<float, NM>& mr_, RegArr<float, NM>& mi_) {
+                                                RegArr<float, NM>& mr_, RegArr<float, NM>& mi_,
+                                                double* __restrict__ xs = nullptr) {
   double* colb = reinterpret_cast<double*>(lds);        // 64: sweep column broadcast
   // ---- step 1: A = |Gamma| (row per lane, FP64), sweep inverse ----------
   RegArr<double, NM> a;
@@ -528,6 +529,13 @@ __device__ __forceinline__ bool emi_fast_form_m(const Src& coh_at, int n, int la
     __builtin_amdgcn_wave_barrier();
   }
   // a now holds -inv(A) row `lane` (diagonal entry offset by +2)
+  if (xs != nullptr) {
+    // X = inv(A) in FP64 by packed columns for the eigen-residual check: entry
+    // (i, j), i <= j, at j(j+1)/2 + i, so each store covers consecutive lanes
+#pragma unroll
+    for (int j = 0; j < NM; ++j)
+      if (lane <= j && j < n) xs[j * (j + 1) / 2 + lane] = (j == lane) ? 2.0 - a.get(j) : -a.get(j);
+  }
 
   // ---- step 2: M = inv(A) o Gamma (FP32 complex row per lane) ------------
 #pragma unroll
@@ -577,10 +585,64 @@ __device__ __forceinline__ float wmax_f(float v) {
   return fmaxf(fmaxf(rl_f(v, 0), rl_f(v, 16)), fmaxf(rl_f(v, 32), rl_f(v, 48)));
 }
 
-template <int NM, class Src>
+// y = (X o Gamma) x in FP64, lane i: y_i.  xs: X = inv(|Gamma|) in FP64 by
+// packed columns (emi_fast_form_m); xb: x broadcast (FP64); (x_r, x_i): this
+// lane's entry.  Column j of the packed X covers rows i <= j; entry (i, j),
+// i < j, also enters row j as X_ij conj(Gamma_ij) x_i (one wave sum per
+// column).  One column at a time, the next column's load in flight (the
+// caller holds the FP32 inverse in registers meanwhile).
+template <class Src>
+__device__ __forceinline__ void emi_m64_product(const Src& coh_at, int n, int lane, const double* __restrict__ xs,
+                                                const double2* xb, double x_r, double x_i, double& yr, double& yi) {
+  yr = 0.0;
+  yi = 0.0;
+  double a_next = lane == 0 ? xs[0] : 0.0;
+#pragma unroll 1
+  for (int j = 0; j < n; ++j) {
+    const double a = a_next;
+    const int jn = j + 1;
+    a_next = (jn < n && lane <= jn) ? xs[jn * (jn + 1) / 2 + lane] : 0.0;
+    const double2 xj = xb[j];
+    double gr = 0.0, gi = 0.0;
+    if (lane == j) {
+      gr = 1.0;
+    } else if (lane < j) {
+      const float2 g = coh_at(lane, j);
+      gr = g.x;
+      gi = g.y;
+    }
+    yr = fma(a, gr * xj.x - gi * xj.y, yr);
+    yi = fma(a, gr * xj.y + gi * xj.x, yi);
+    const bool lo = lane < j;
+    const double sr = wsum_d(lo ? a * (gr * x_r + gi * x_i) : 0.0);
+    const double si = wsum_d(lo ? a * (gr * x_i - gi * x_r) : 0.0);
+    if (lane == j) {
+      yr += sr;
+      yi += si;
+    }
+  }
+}
+
+// REF (the precision form): after the iteration converges, FP64 refinement
+// steps through the same product.  Each forms the FP64 residual
+// r = M x - lambda x (M = X o Gamma with X = inv(|Gamma|) in FP64 from xs,
+// emi_fast_form_m; lambda = x^H M x) and the correction d = P X' r, X' the
+// FP32 inverse of B and P the projector off x.  To first order the error of x
+// is inv(M - lambda_1 I) r on the complement of x; X' shrinks each of its
+// components by at most 1 - rho (rho: the iteration's contraction per sweep,
+// estimated from its last changes), so the linked phase error is estimated as
+// 2 max_i |d_i| / |x_i| / (1 - rho).  At or below tau on the first step the
+// point keeps the plain outputs; above, x <- x - d until the estimate falls
+// below tau / 16 (at most EMI_REF_STEPS corrections; lambda is then the FP64
+// Rayleigh quotient).  *flag: no contraction estimate or no convergence, the
+// point goes to the FP64 re-solve.
+constexpr int EMI_REF_STEPS = 3;
+
+template <int NM, bool REF = false, class Src>
 __device__ __forceinline__ bool emi_shift_invert(const Src& coh_at, int n, int lane, unsigned char* lds,
                                                  RegArr<float, NM>& br, RegArr<float, NM>& bi, float& vr, float& vi,
-                                                 double& rq) {
+                                                 double& rq, const double* __restrict__ xs = nullptr,
+                                                 double tau = 0.0, bool* flag = nullptr) {
   float2* colb = reinterpret_cast<float2*>(lds);        // 64: sweep column broadcast
   float2* xb = reinterpret_cast<float2*>(lds + 512);    // 64: iterate broadcast
   // B = M - sigma I, and this lane's diagonal entry (real) for the pivot search
@@ -661,8 +723,28 @@ __device__ __forceinline__ bool emi_shift_invert(const Src& coh_at, int n, int l
   }
   float mu = 0.f;
   bool conv = false;
-  for (int it = 0; it < EMI_INV_MAX; ++it) {
-    xb[lane] = make_float2(xr, xi);
+  float d1 = 0.f, d2 = 0.f;  // REF: the two squared changes before the last
+  int step = -1;             // REF: refinement step, -1 while iterating
+  double x_r = 0.0, x_i = 0.0, amp = 0.0, lam = 0.0;
+  bool refined = false;
+  constexpr int kSweeps = EMI_INV_MAX + (REF ? EMI_REF_STEPS + 1 : 0);
+  for (int it = 0; it < kSweeps; ++it) {
+    float vxr = xr, vxi = xi;  // the vector multiplied: the iterate, or (refining) the FP64 residual
+    if constexpr (REF) {
+      if (step >= 0) {
+        double2* xd = reinterpret_cast<double2*>(lds);  // 64 x 16 bytes over colb and xb
+        __builtin_amdgcn_wave_barrier();
+        xd[lane] = make_double2(x_r, x_i);
+        __builtin_amdgcn_wave_barrier();
+        double yr64, yi64;
+        emi_m64_product(coh_at, n, lane, xs, xd, x_r, x_i, yr64, yi64);
+        lam = wsum_d(lane < n ? x_r * yr64 + x_i * yi64 : 0.0);  // x^H M x (real: M Hermitian)
+        vxr = lane < n ? (float)(yr64 - lam * x_r) : 0.f;
+        vxi = lane < n ? (float)(yi64 - lam * x_i) : 0.f;
+        __builtin_amdgcn_wave_barrier();
+      }
+    }
+    xb[lane] = make_float2(vxr, vxi);
     __builtin_amdgcn_wave_barrier();
     float ar = 0.f, ai = 0.f;
     int nu_ = n;
@@ -691,6 +773,34 @@ __device__ __forceinline__ bool emi_shift_invert(const Src& coh_at, int n, int l
         for (int t = 0; t < 4; ++t) cur[t] = nxt[t];
       }
     }
+    if constexpr (REF) {
+      if (step >= 0) {
+        // d = X' r, projected off x; the estimated linked-phase error
+        double d_r = lane < n ? (double)(2.f * vxr - ar) : 0.0, d_i = lane < n ? (double)(2.f * vxi - ai) : 0.0;
+        const double cr = wsum_d(x_r * d_r + x_i * d_i), ci = wsum_d(x_r * d_i - x_i * d_r);  // x^H d
+        d_r -= cr * x_r - ci * x_i;
+        d_i -= cr * x_i + ci * x_r;
+        const double ax = sqrt(x_r * x_r + x_i * x_i);
+        const double est = amp * wmax_d(lane < n ? sqrt(d_r * d_r + d_i * d_i) / ax : 0.0);
+        if (step == 0 && est <= tau) break;  // within tolerance: the plain outputs
+        const bool done = est <= tau * (1.0 / 16.0);
+        if (!done && step == EMI_REF_STEPS) {
+          *flag = true;
+          break;
+        }
+        x_r -= d_r;
+        x_i -= d_i;
+        const double s = 1.0 / sqrt(wsum_d(x_r * x_r + x_i * x_i));
+        x_r *= s;
+        x_i *= s;
+        if (done) {
+          refined = true;
+          break;
+        }
+        ++step;
+        continue;
+      }
+    }
     const float yr = lane < n ? 2.f * xr - ar : 0.f, yi = lane < n ? 2.f * xi - ai : 0.f;
     mu = wsum_f(xr * yr + xi * yi);  // x^H X x (x unit, X Hermitian: real)
     const float nrm = sqrtf(wsum_f(yr * yr + yi * yi));
@@ -703,10 +813,41 @@ __device__ __forceinline__ bool emi_shift_invert(const Src& coh_at, int n, int l
     __builtin_amdgcn_wave_barrier();
     if (it + 1 >= EMI_INV_MIN && dmax < EMI_INV_TOL * EMI_INV_TOL) {
       conv = true;
+      if constexpr (REF) {
+        // rho^2 from the larger of the last two ratios of the squared changes,
+        // each floored at FP32 resolution (a change at rounding level gives a
+        // larger rho)
+        constexpr float fl = 1.4210855e-14f;  // (2^-23)^2
+        const float r2 = fmaxf(fmaxf(dmax, fl) / fmaxf(d2, fl), fmaxf(d2, fl) / fmaxf(d1, fl));
+        const double rho = sqrt((double)r2);
+        if (!(rho < 1.0)) {
+          *flag = true;
+          break;
+        }
+        amp = 2.0 / (1.0 - rho);
+        x_r = lane < n ? (double)xr : 0.0;
+        x_i = lane < n ? (double)xi : 0.0;
+        const double s = 1.0 / sqrt(wsum_d(x_r * x_r + x_i * x_i));
+        x_r *= s;
+        x_i *= s;
+        step = 0;
+        continue;
+      }
       break;
     }
+    if (it + 1 >= EMI_INV_MAX) break;
+    d1 = d2;
+    d2 = dmax;
   }
   if (!conv || !(mu > 0.f) || !isfinite(mu)) return false;
+  if constexpr (REF) {
+    if (refined) {
+      vr = lane < n ? (float)x_r : 0.f;
+      vi = lane < n ? (float)x_i : 0.f;
+      rq = lam;
+      return true;
+    }
+  }
   vr = lane < n ? xr : 0.f;
   vi = lane < n ? xi : 0.f;
   rq = (double)EMI_INV_SIGMA + 1.0 / (double)mu;
@@ -725,6 +866,18 @@ __device__ __forceinline__ bool emi_inv_point(const Src& coh_at, int n, int lane
   RegArr<float, NM> mr_, mi_;
   if (!emi_fast_form_m<NM>(coh_at, n, lane, lds, mr_, mi_)) return false;
   return emi_shift_invert<NM>(coh_at, n, lane, lds, mr_, mi_, vr, vi, rq);
+}
+
+// The precision form's shift-invert point (REF above; xs: this wave's slot of
+// n(n+1)/2 doubles for inv(|Gamma|)); flag is set when the point needs the
+// FP64 re-solve.
+template <int NM, class Src>
+__device__ __forceinline__ bool emi_inv_point_checked(const Src& coh_at, int n, int lane, unsigned char* lds,
+                                                      float& vr, float& vi, double& rq, double* xs, double tau,
+                                                      bool& flag) {
+  RegArr<float, NM> mr_, mi_;
+  if (!emi_fast_form_m<NM>(coh_at, n, lane, lds, mr_, mi_, xs)) return false;
+  return emi_shift_invert<NM, true>(coh_at, n, lane, lds, mr_, mi_, vr, vi, rq, xs, tau, &flag);
 }
 
 // ph_i = v_i conj(v_ref) / |v_i conj(v_ref)|

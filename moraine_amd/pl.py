@@ -88,9 +88,11 @@ def emi(coh, ref: int = 0, *, refine=None):
     complex128 coherence runs every stage in FP64.
 
     refine (not in moraine): a phase tolerance in radians, e.g. 1e-3.  Points
-    whose estimated phase error from the FP32 stages exceeds it are solved
-    again in FP64 throughout; the others are unchanged.  (Above 200 images
-    EMI is FP64 throughout and refine has nothing to do.)"""
+    whose phase error from the FP32 stages may exceed it are solved again in
+    FP64 throughout; the others keep the plain call's outputs.  The test is an
+    FP64 eigen-residual bound for the shift-invert points (32 < N <= 64) and a
+    conditioning estimate from the tridiagonal form for the Householder ones.
+    (Above 200 images EMI is FP64 throughout and refine has nothing to do.)"""
     as_torch = _dev.is_torch(coh)
     c = _dev.to_dev_complex(coh)
     n = nimage_from_npair(c.shape[-1])
