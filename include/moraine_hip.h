@@ -152,7 +152,11 @@ int mr_emperical_co_pc_c128(const double* rslc, int nl, int w, int n, const int3
  * smallest eigenvalue of M; ph = v*conj(v[ref]) / |v*conj(v[ref])|.
  * inv(|Gamma|) is formed in FP64; the eigen-solve precision is stated in the
  * implementation notes of emi.hip.  Singular |Gamma| gives NaN for that point.
- * workspace >= mr_emi_workspace_bytes(n_pts, n).
+ * 32 < n <= 64: a shift-invert pass, a Householder pass over the points it
+ * declines, and an FP64 re-solve of those whose estimated phase error exceeds
+ * 1e-3 rad; once the stream has passed the call, the workspace's first two
+ * 32-bit words hold the number of points of the Householder pass and of the
+ * re-solve.  workspace >= mr_emi_workspace_bytes(n_pts, n).
  * ------------------------------------------------------------------------- */
 size_t mr_emi_workspace_bytes(long long n_pts, int n);
 int mr_emi(const float* coh, long long n_pts, int n, int ref, float* ph, float* quality,
@@ -255,13 +259,20 @@ int mr_emi_ds_temp_coh(const float* coh, long long n_pts, int n, int ref, const 
 
 /* Precision form of emi / emi + ds_temp_coh for 2 <= n <= 200 (above, mr_emi
  * is FP64 throughout), no reference counterpart (moraine/pl.py:18-77 computes
- * in the array library's precision): the same solve as mr_emi, then every
- * point whose estimated phase error eps32 ||T|| n / (gap min_i |v_i| 60 sqrt 60)
- * exceeds tol (radians; T the tridiagonal form, gap its two smallest
- * eigenvalues' distance, v the unit eigenvector) is solved again in FP64
- * throughout.  With t_coh non-null the DS temporal coherence is written as
- * mr_emi_ds_temp_coh does (pairs: the full triu network).  Points not flagged
- * get exactly mr_emi's outputs.
+ * in the array library's precision): the same solve as mr_emi with the checks
+ * at tol (radians).  Householder solves (every point for n <= 32 and
+ * n > 64; those the shift-invert pass declines for 32 < n <= 64) are flagged
+ * when their estimated phase error eps32 ||T|| n / (gap min_i |v_i| 60 sqrt 60)
+ * exceeds tol (for 32 < n <= 64: min(tol, 1e-3); T the tridiagonal form, gap
+ * its two smallest eigenvalues' distance, v the unit eigenvector).
+ * Shift-invert solves (32 < n <= 64) form the FP64 residual of the iterate
+ * against inv(|Gamma|) o Gamma (inv in FP64) and estimate the phase error from
+ * the FP32 inverse applied to it; above tol the iterate is corrected in FP64
+ * (iterative refinement), and points that do not converge are flagged.
+ * Flagged points are solved again in FP64 throughout.  With t_coh non-null
+ * the DS temporal coherence is written as mr_emi_ds_temp_coh does (pairs: the
+ * full triu network).  Points neither corrected nor flagged get exactly
+ * mr_emi's outputs.
  * workspace >= mr_emi_refined_workspace_bytes(n_pts, n); once the stream has
  * passed the call, its first 4 bytes hold the number of re-solved points. */
 size_t mr_emi_refined_workspace_bytes(long long n_pts, int n);
@@ -280,10 +291,11 @@ int mr_emi_refined(const float* coh, long long n_pts, int n, int ref, double tol
  *   pairs      the full triu network (n_pairs = n(n-1)/2);
  *   ph         complex64, pixel id's n phases at complex element ph_off[id];
  *   quality, t_coh  float32 indexed by pixel id.
- * 32 < n <= 64 runs the fused launch (coherence stays on chip; `blocks` sets
- * its grid, 0 = default; workspace >= mr_ds_batch_fused_workspace_bytes(n_pix)
- * for the list of pixels the shift-invert EMI leaves to the Householder
- * pass); n <= 32, n > 64, or blocks < 0 (per-stage mode:
+ * 32 < n <= 64 runs the fused launches (coherence stays on chip; `blocks`
+ * sets the first one's grid, 0 = default; workspace >=
+ * mr_ds_batch_fused_workspace_bytes(n_pix) for the lists of pixels the
+ * shift-invert EMI leaves to the Householder pass and of those the FP64
+ * re-solve takes, as in mr_emi); n <= 32, n > 64, or blocks < 0 (per-stage mode:
  * coherence, EMI and temporal coherence as separate launches through HBM, the
  * drop-in sequence), runs per chunk of `chunk` pixels with
  * workspace >= mr_ds_batch_workspace_bytes(chunk, n).  For a sample matrix
