@@ -51,7 +51,7 @@ def parse():
     ap.add_argument("--r", type=int, default=10)
     ap.add_argument("--ps", type=float, default=0.10)
     ap.add_argument("--batch", type=int, default=1 << 18, help="pixels per coherence/EMI batch")
-    ap.add_argument("--cpu-sample", type=int, default=160, help="CPU baseline: interior sub-tile edge")
+    ap.add_argument("--cpu-sample", type=int, default=256, help="CPU baseline: interior sub-tile edge")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--err-sample", type=int, default=2048)
     ap.add_argument("--forms", default="plain,batched,serving,rows,streamed,precision",
@@ -92,29 +92,27 @@ def ensure_built():
 
 
 # ---------------------------------------------------------------------------- CPU baseline
-def cpu_baseline(args):
-    """Reference CPU path restated in C/OpenMP (KS, coherence, temporal coherence) plus
-    LAPACK EMI (float32 inv + complex64 eigh, as moraine/pl.py:45-65) on a bounded
-    interior sub-tile of the same workload model, all host cores."""
+def _cpu_pipeline(z, az, r, rows, cols, cores):
+    """The reference CPU pipeline on the pixels rows x cols of stack z: KS with the p
+    series per pixel pair and select_shp (C/OpenMP port), emperical_co_pc over the
+    full pair network (port), EMI as moraine/pl.py:45-65 computes it (float32 inv,
+    complex64 eigh, LAPACK, one thread per chunk), ds_temp_coh (port).  Returns the
+    seconds taken and the SHP masks of those pixels."""
     import concurrent.futures as cf
 
     import numpy as np
 
-    from moraine_amd import synth
     from oracle import moraine_ref as R
     from oracle import port
 
-    port.build()
-    cores = len(os.sched_getaffinity(0))
-    s, az, r, n = args.cpu_sample, args.az, args.r, args.n
-    z = synth.ds_stack(s + 2 * az, s + 2 * r, n, seed=123, ps_frac=args.ps)
+    n = z.shape[-1]
     rmli = (np.abs(z) ** 2).astype(np.float32)
     t0 = time.perf_counter()
-    p = port.ks_test(rmli, az, r, nthreads=cores)
+    p = port.ks_test(rmli, az, r, nthreads=cores, per_pair=True)
     is_shp, _ = R.select_shp(p, 0.05)
-    ii, jj = np.meshgrid(np.arange(az, az + s), np.arange(r, r + s), indexing="ij")
+    ii, jj = np.meshgrid(rows, cols, indexing="ij")
     idx = np.stack([ii.ravel(), jj.ravel()], -1).astype(np.int32)
-    pc_shp = is_shp[az:az + s, r:r + s].reshape(-1, 2 * az + 1, 2 * r + 1)
+    pc_shp = is_shp[ii.ravel(), jj.ravel()]
     pairs = R.image_pairs_from_bandwidth(n)
     coh = port.emperical_co_pc(z, idx, pc_shp, pairs, nthreads=cores)
     chunks = np.array_split(np.arange(idx.shape[0]), cores * 4)
@@ -122,12 +120,73 @@ def cpu_baseline(args):
         res = list(ex.map(lambda c: R.emi_ref32(coh[c]), chunks))
     ph = np.concatenate([a for a, _ in res])
     port.ds_temp_coh(coh, ph, pairs, nthreads=cores)
-    dt = time.perf_counter() - t0
-    npx = idx.shape[0]
+    return time.perf_counter() - t0, pc_shp
+
+
+def _gpu_pipeline_config1(z, az, r, torch):
+    """The drop-in pipeline on the GPU for a whole small stack (KS + select_shp on the
+    CPU path's amplitudes |z|^2, emperical_co_pc, EMI + ds_temp_coh): best of five
+    timed runs (amplitudes uploaded once, outside them) and the SHP masks."""
+    import numpy as np
+
+    import moraine_amd as mr
+
+    zt = torch.from_numpy(z).cuda()
+    rmli = torch.from_numpy((np.abs(z) ** 2).astype(np.float32)).cuda()
+    nl, w, n = z.shape
+    ii, jj = torch.meshgrid(torch.arange(nl, device="cuda", dtype=torch.int32),
+                            torch.arange(w, device="cuda", dtype=torch.int32), indexing="ij")
+    idx = torch.stack([ii.reshape(-1), jj.reshape(-1)], -1).contiguous()
+
+    def run():
+        _, shp, _ = mr.ks_test_select_shp(rmli, az, r, 0.05, return_p=False)
+        coh = mr.emperical_co_pc(zt, idx, shp.view(-1, 2 * az + 1, 2 * r + 1))
+        mr.pl.emi_ds_temp_coh(coh)
+        return shp
+
+    run()
+    torch.cuda.synchronize()
+    best = float("inf")
+    for _ in range(5):
+        t0 = time.perf_counter()
+        shp = run()
+        torch.cuda.synchronize()
+        best = min(best, time.perf_counter() - t0)
+    return best, shp.view(nl * w, 2 * az + 1, 2 * r + 1).cpu().numpy()
+
+
+def cpu_baseline(args, torch):
+    """The reference CPU path restated (oracle/port.c, C/OpenMP, the p series per
+    pixel pair as the reference evaluates it; LAPACK EMI as moraine/pl.py:45-65) on
+    all host cores: on a bounded interior sub-tile of the headline workload (the
+    value), and on the whole small CPU workload (200x200, 17 images, 11x15 windows)
+    with the GPU drop-in timed on the same stack beside it."""
+    import numpy as np
+
+    from moraine_amd import synth
+    from oracle import port
+
+    port.build()
+    cores = len(os.sched_getaffinity(0))
+    s, az, r, n = args.cpu_sample, args.az, args.r, args.n
+    z = synth.ds_stack(s + 2 * az, s + 2 * r, n, seed=123, ps_frac=args.ps)
+    dt, _ = _cpu_pipeline(z, az, r, np.arange(az, az + s), np.arange(r, r + s), cores)
+    npx = s * s
+    # the whole small workload: 200 x 200 pixels, 17 images, 11 x 15 windows, no PS pixels
+    z1 = synth.ds_stack(200, 200, 17, seed=1)
+    dt1, shp_cpu = _cpu_pipeline(z1, 5, 7, np.arange(200), np.arange(200), cores)
+    gt1, shp_gpu = _gpu_pipeline_config1(z1, 5, 7, torch)
     return {
         "value": npx / dt, "unit": "DS pixels/s", "cores": cores, "kind": "port",
         "sample": f"{s}x{s} interior pixels (+{az}/{r} halo) of the N={n}, {2*az+1}x{2*r+1}, {int(args.ps*100)}% PS "
-                  f"workload: C/OpenMP KS+select+coherence+tcoh, numpy LAPACK EMI (f32 inv, c64 eigh); {dt:.1f} s",
+                  f"workload: C/OpenMP KS (p series per pixel pair) + select + coherence + tcoh, numpy LAPACK "
+                  f"EMI (f32 inv, c64 eigh); {dt:.1f} s",
+        "small_workload": {
+            "workload": "200x200 pixels, 17 images, 11x15 windows, p_max=0.05, no PS pixels, every pixel",
+            "cpu_s": dt1, "cpu_px_per_s": 40000 / dt1, "gpu_s": gt1, "gpu_px_per_s": 40000 / gt1,
+            "gpu_over_cpu": dt1 / gt1, "shp_masks_equal": bool(np.array_equal(shp_cpu, shp_gpu)),
+            "note": "same pipeline and stack; GPU: best of 5 after a warm-up, launch-bound at this size",
+        },
     }
 
 
@@ -704,7 +763,7 @@ def main():
                             "note": "EMI + temporal coherence of the last batch with refine=tol, HIP events"}
 
     progress("plain pass timed; CPU baseline")
-    cpu = None if args.no_cpu_baseline else cpu_baseline(args)
+    cpu = None if args.no_cpu_baseline else cpu_baseline(args, torch)
     forms = {}
     wanted = {f.strip() for f in args.forms.split(",") if f.strip()}
     del state, rslc, idx, ph, q, tc, coh

@@ -55,9 +55,13 @@ static int ks_k(const float* ref, const float* sec, int n) {
 }
 
 /* _sort_numba + _ks_test_numba (moraine/shp.py:80-125).  dist may be NULL.
- * k_out (int32, may be NULL) receives k or -1 where NaN is written. */
+ * k_out (int32, may be NULL) receives k or -1 where NaN is written.
+ * per_pair = 0: p depends only on k, so the series is evaluated once per k;
+ * per_pair = 1: once per pixel pair, as the reference does (moraine/shp.py:118,
+ * up to 100 exp per pair) -- the same values at the reference's cost, for the
+ * timed CPU baseline. */
 void port_ks_test_f32(const float* rmli, int nl, int w, int n, int az, int r,
-                      float* p, float* dist, int32_t* k_out, int nthreads) {
+                      float* p, float* dist, int32_t* k_out, int nthreads, int per_pair) {
   size_t npx = (size_t)nl * w;
   float* s = (float*)malloc(npx * n * sizeof(float));
   const int aw = 2 * az + 1, rw = 2 * r + 1, win = aw * rw;
@@ -88,7 +92,7 @@ void port_ks_test_f32(const float* rmli, int nl, int w, int n, int az, int r,
             continue;
           }
           int k = ks_k(ref, s + ((size_t)si * w + sj) * n, n);
-          p[o] = (float)pk[k];
+          p[o] = per_pair ? (float)ks_p((en + 0.12 + 0.11 / en) * fabs((double)k / n)) : (float)pk[k];
           if (dist) dist[o] = (float)fabs((double)k / n);
           if (k_out) k_out[o] = k;
         }

@@ -26,8 +26,7 @@ def build(force: bool = False) -> str:
 def _load():
     global _lib
     if _lib is None:
-        if not os.path.exists(_SO):
-            build()
+        build()  # compiles when libport.so is missing or older than port.c
         _lib = ctypes.CDLL(_SO)
     return _lib
 
@@ -40,8 +39,11 @@ def nthreads_default() -> int:
     return len(os.sched_getaffinity(0))
 
 
-def ks_test(rmli: np.ndarray, az: int, r: int, return_dist=False, return_k=False, nthreads=None):
-    """Port of _sort_numba + _ks_test_numba for float32 stacks."""
+def ks_test(rmli: np.ndarray, az: int, r: int, return_dist=False, return_k=False, nthreads=None,
+            per_pair=False):
+    """Port of _sort_numba + _ks_test_numba for float32 stacks.  per_pair: evaluate
+    the p series for every pixel pair as the reference does (same values, the
+    reference's cost) instead of once per step count k."""
     lib = _load()
     rmli = np.ascontiguousarray(rmli, dtype=np.float32)
     nl, w, n = rmli.shape
@@ -50,7 +52,7 @@ def ks_test(rmli: np.ndarray, az: int, r: int, return_dist=False, return_k=False
     d = np.empty(shape, np.float32) if return_dist else None
     k = np.empty(shape, np.int32) if return_k else None
     lib.port_ks_test_f32(_p(rmli), nl, w, n, az, r, _p(p), _p(d) if d is not None else None,
-                         _p(k) if k is not None else None, nthreads or nthreads_default())
+                         _p(k) if k is not None else None, nthreads or nthreads_default(), int(per_pair))
     out = [p]
     if return_dist:
         out.insert(0, d)

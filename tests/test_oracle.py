@@ -146,6 +146,16 @@ def test_port_matches_numpy_restatement(port):
     assert np.all(np.isnan(p[~ok]))
 
 
+def test_port_per_pair_series_equals_per_k_table(port):
+    """The timed baseline evaluates the p series for every pixel pair, as the
+    reference does; the values equal the per-k table's exactly."""
+    g = np.load(os.path.join(GOLD, "ks_cfg1.npz"))
+    rmli = g["rmli"]
+    p_k = port.ks_test(rmli, int(g["az"]), int(g["r"]))
+    p_pair = port.ks_test(rmli, int(g["az"]), int(g["r"]), per_pair=True)
+    np.testing.assert_array_equal(p_pair.view(np.uint32), p_k.view(np.uint32))
+
+
 def test_golden_ks_regression():
     g = np.load(os.path.join(GOLD, "ks_cfg1.npz"))
     k = R.ks_k_field(g["rmli"], int(g["az"]), int(g["r"]))
