@@ -298,7 +298,11 @@ int mr_emi_refined(const float* coh, long long n_pts, int n, int ref, double tol
  * re-solve takes, as in mr_emi); n <= 32, n > 64, or blocks < 0 (per-stage mode:
  * coherence, EMI and temporal coherence as separate launches through HBM, the
  * drop-in sequence), runs per chunk of `chunk` pixels with
- * workspace >= mr_ds_batch_workspace_bytes(chunk, n).  For a sample matrix
+ * workspace >= mr_ds_batch_workspace_bytes(chunk, n).  coh_waves: the tile
+ * layout of the per-stage coherence kernel for 64 < n <= 224 (its grid of
+ * 32 x 32 tiles spread over that many waves per workgroup: 0 = the drop-in's
+ * layout, 4 waves up to n = 128 and 8 above; 2 for n <= 96, 4 or 8); the
+ * outputs do not depend on it.  For a sample matrix
  * made of a pixel's SHP stacks in window order the outputs equal the drop-in
  * composition exactly.
  * ------------------------------------------------------------------------- */
@@ -307,7 +311,7 @@ size_t mr_ds_batch_fused_workspace_bytes(long long n_pix);
 int mr_ds_batch(const float* samples, const long long* row_off, const int32_t* n_samples,
                 const long long* ph_off, const int32_t* pix, long long n_pix, int n, int ref,
                 const int32_t* pairs, int n_pairs, float* ph, float* quality, float* t_coh, int blocks,
-                long long chunk, void* workspace, size_t workspace_bytes, mr_stream_t stream);
+                long long chunk, int coh_waves, void* workspace, size_t workspace_bytes, mr_stream_t stream);
 
 #ifdef __cplusplus
 }

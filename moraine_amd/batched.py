@@ -106,6 +106,14 @@ class AutoTuner:
     FUSED_BLOCKS = (0, 4096, 16384, 65536)  # 0 = one block per pixel (capped)
     CHUNKS = (4096, 8192, 16384, 32768, 65536)
 
+    @staticmethod
+    def coh_waves(n: int) -> tuple:
+        """Tile layouts of the per-stage coherence kernel for n images: waves per
+        workgroup over its grid of 32 x 32 tiles (() where it has one layout)."""
+        if not 64 < n <= 224:
+            return ()
+        return (2, 4, 8) if n <= 96 else (4, 8)
+
     def __init__(self, path: str | None = None, reps: int = 2, timer=None):
         self.path = path or os.environ.get("MORAINE_AMD_TUNE_CACHE")
         self.reps = reps
@@ -157,6 +165,7 @@ class _Group:
     blocks: int = 0
     chunk: int = 0
     staged: bool = True
+    coh_waves: int = 0  # 0: the drop-in's tile layout
 
 
 class DSBatchRunner:
@@ -193,13 +202,13 @@ class DSBatchRunner:
         self._tuned = False
 
     # ---------------------------------------------------------------- launches
-    def _call(self, g: _Group, blocks: int, chunk: int):
+    def _call(self, g: _Group, blocks: int, chunk: int, coh_waves: int | None = None):
         ws = self.ws
         _lib.call("mr_ds_batch", _dev.ptr(self.batch.samples), _dev.ptr(self.batch.row_off),
                   _dev.ptr(self.batch.n_samples), _dev.ptr(self.ph_off), _dev.ptr(g.pix), g.count, g.n, self.ref,
                   _dev.ptr(g.pairs), g.pairs.shape[0], _dev.ptr(self.ph), _dev.ptr(self.quality),
-                  _dev.ptr(self.t_coh), int(blocks), int(chunk), _dev.ptr(ws),
-                  0 if ws is None else ws.numel(), _dev.stream())
+                  _dev.ptr(self.t_coh), int(blocks), int(chunk), int(g.coh_waves if coh_waves is None else coh_waves),
+                  _dev.ptr(ws), 0 if ws is None else ws.numel(), _dev.stream())
 
     def _staged(self, g: _Group) -> bool:
         return self.per_stage or g.n > 64 or g.n <= 32
@@ -214,7 +223,9 @@ class DSBatchRunner:
         the staged one (coherence chunk through HBM, then the EMI + temporal
         coherence kernel; one of CHUNKS), encoded as -chunk: both give the same
         outputs bit for bit, and which is faster depends on N and the SHP
-        counts."""
+        counts.  For staged groups with 64 < N <= 224 it then chooses the tile
+        layout of the coherence kernel (waves per workgroup over its 32 x 32
+        tile grid, coh_waves), which does not change the outputs either."""
         dev = self.batch.samples.device
         need = 0
         for g in self.groups:
@@ -243,6 +254,10 @@ class DSBatchRunner:
                 else:
                     g.chunk = self.tuner.choose(AutoTuner.key("chunk", g.n, g.count), chunks,
                                                 lambda c, g=g: self._call(g, -1, c))
+                waves = AutoTuner.coh_waves(g.n) if g.staged else ()
+                if waves:  # the coherence kernel's tile layout, at the chosen chunk
+                    g.coh_waves = self.tuner.choose(AutoTuner.key("coh_waves", g.n, g.count), waves,
+                                                    lambda v, g=g: self._call(g, -1, g.chunk, v))
         self._tuned = True
 
     def launch(self):

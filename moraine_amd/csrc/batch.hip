@@ -37,7 +37,7 @@ int co_mfma_samples_launch(const float2* samples, const long long* row_off, cons
                            int npairs, float2* coh, hipStream_t st);
 int co_mfma_wide_samples_launch(const float2* samples, const long long* row_off, const int32_t* n_samp,
                                 const int32_t* pix, long long pix0, long long cn, int n, const int32_t* pairs,
-                                int npairs, float2* coh, hipStream_t st);
+                                int npairs, float2* coh, hipStream_t st, int waves);
 constexpr int BATCH_DEFAULT_BLOCKS = 1 << 15;
 constexpr int BATCH_FALLBACK_BLOCKS = 8192;
 
@@ -187,8 +187,8 @@ size_t mr_ds_batch_workspace_bytes(long long chunk, int n) {
 
 int mr_ds_batch(const float* samples, const long long* row_off, const int32_t* n_samples, const long long* ph_off,
                 const int32_t* pix, long long n_pix, int n, int ref, const int32_t* pairs, int n_pairs, float* ph,
-                float* quality, float* t_coh, int blocks, long long chunk, void* workspace, size_t workspace_bytes,
-                mr_stream_t stream) {
+                float* quality, float* t_coh, int blocks, long long chunk, int coh_waves, void* workspace,
+                size_t workspace_bytes, mr_stream_t stream) {
   MR_CHECK_ARG(n >= 2 && n <= 1024, "ds_batch: n=%d outside [2, 1024]", n);
   MR_CHECK_ARG(ref >= 0 && ref < n, "ds_batch: ref=%d outside [0,%d)", ref, n);
   MR_CHECK_ARG(n_pix >= 0, "ds_batch: negative pixel count");
@@ -248,7 +248,8 @@ int mr_ds_batch(const float* samples, const long long* row_off, const int32_t* n
     const bool wide = n > 64 && n <= 224;
     const int rc0 =
         n <= 64 ? mr::co_mfma_samples_launch(smp, row_off, n_samples, pix, c0, cn, n, pairs, n_pairs, coh, st)
-        : wide  ? mr::co_mfma_wide_samples_launch(smp, row_off, n_samples, pix, c0, cn, n, pairs, n_pairs, coh, st)
+        : wide  ? mr::co_mfma_wide_samples_launch(smp, row_off, n_samples, pix, c0, cn, n, pairs, n_pairs, coh, st,
+                                                  coh_waves)
                 : mr::co_samples_launch(smp, row_off, n_samples, pix, c0, cn, n, pairs, n_pairs, coh, st);
     if (rc0) return rc0;
     // EMI and the temporal coherence of the chunk: one kernel for 32 < n <= 64 (equal, bit for bit, to

@@ -355,15 +355,61 @@ int co_mfma_wide_launch(const float2* rslc, int nl, int w, int n, const int32_t*
   return launch_status("co_mfma_wide");
 }
 
+#undef MR_COW_DISPATCH
+
+// Sample-matrix form with an explicit tile layout: the NB x NB tile grid's
+// upper triangle spread over W waves (tile slots per wave ceil(NT / W)).
+template <int NB, int W>
+static void cow_samples(const float2* samples, const long long* row_off, const int32_t* n_samp, const int32_t* pix,
+                        long long pix0, long long cn, int n, const int32_t* pairs, int npairs, float2* coh,
+                        hipStream_t st) {
+  const int nb = (int)std::min<long long>(cn, 1 << 20);
+  hipLaunchKernelGGL((co_mfma_wide_kernel<true, NB, W, false>), dim3(nb), dim3(64 * W), cow_lds_bytes(1, NB, W), st,
+                     samples, 0, 0, n, nullptr, nullptr, cn, 1, 1, pairs, npairs, coh, row_off, n_samp, pix, pix0);
+}
+
+template <int NB>
+static int cow_samples_waves(int waves, const float2* samples, const long long* row_off, const int32_t* n_samp,
+                             const int32_t* pix, long long pix0, long long cn, int n, const int32_t* pairs,
+                             int npairs, float2* coh, hipStream_t st) {
+  constexpr int kDefault = NB <= 4 ? 4 : 8;  // the drop-in's layout for this grid
+  if (waves == 0) waves = kDefault;
+  if constexpr (NB == 3) {
+    if (waves == 2) {
+      cow_samples<NB, 2>(samples, row_off, n_samp, pix, pix0, cn, n, pairs, npairs, coh, st);
+      return MR_OK;
+    }
+  }
+  if (waves == 4) {
+    cow_samples<NB, 4>(samples, row_off, n_samp, pix, pix0, cn, n, pairs, npairs, coh, st);
+  } else if (waves == 8) {
+    cow_samples<NB, 8>(samples, row_off, n_samp, pix, pix0, cn, n, pairs, npairs, coh, st);
+  } else {
+    set_error("ds_batch: %d waves per coherence tile grid unsupported for n=%d (0, %s4 or 8)", waves, n,
+              NB == 3 ? "2, " : "");
+    return MR_ERR_UNSUPPORTED;
+  }
+  return MR_OK;
+}
+
+// waves: 0 = the drop-in's layout (4 waves for n <= 128, 8 above); 2 (n <= 96), 4 or 8.
 int co_mfma_wide_samples_launch(const float2* samples, const long long* row_off, const int32_t* n_samp,
                                 const int32_t* pix, long long pix0, long long cn, int n, const int32_t* pairs,
-                                int npairs, float2* coh, hipStream_t st) {
-  const int nb = (int)std::min<long long>(cn, 1 << 20);
-  MR_COW_DISPATCH(true, 1, samples, 0, 0, n, nullptr, nullptr, cn, 1, 1, pairs, npairs, coh, row_off, n_samp, pix,
-                  pix0);
-  return launch_status("ds_batch coherence (MFMA, 4-8 waves)");
+                                int npairs, float2* coh, hipStream_t st, int waves) {
+  int rc;
+  if (n <= 96)
+    rc = cow_samples_waves<3>(waves, samples, row_off, n_samp, pix, pix0, cn, n, pairs, npairs, coh, st);
+  else if (n <= 128)
+    rc = cow_samples_waves<4>(waves, samples, row_off, n_samp, pix, pix0, cn, n, pairs, npairs, coh, st);
+  else if (n <= 160)
+    rc = cow_samples_waves<5>(waves, samples, row_off, n_samp, pix, pix0, cn, n, pairs, npairs, coh, st);
+  else if (n <= 192)
+    rc = cow_samples_waves<6>(waves, samples, row_off, n_samp, pix, pix0, cn, n, pairs, npairs, coh, st);
+  else
+    rc = cow_samples_waves<7>(waves, samples, row_off, n_samp, pix, pix0, cn, n, pairs, npairs, coh, st);
+  if (rc) return rc;
+  return launch_status("ds_batch coherence (MFMA tile grid)");
 }
-#undef MR_COW_DISPATCH
 
 // Sample-matrix form (batched path, per-stage mode): point pt is pixel
 // pix[pix0 + pt] with K = n_samp[id] rows at samples + row_off[id]; writes the

@@ -110,7 +110,8 @@ def test_batched_launch_shapes_do_not_change_results():
     r = DSBatchRunner(b, tuner=tuner)
     out = r.run()
     torch.cuda.synchronize()
-    assert len(tuner.picks) == 3
+    assert len(tuner.picks) == 4  # the N = 90 group also picks its coherence tile layout
+    assert tuner.picks[AutoTuner.key("coh_waves", 90, r.groups[-1].count)] in (2, 4, 8)
     for g in r.groups:
         if not g.staged:
             assert g.blocks in AutoTuner.FUSED_BLOCKS and 32 < g.n <= 64
@@ -215,3 +216,34 @@ def test_fused_resolve_equals_per_stage():
     assert torch.equal(a[0].view(torch.int64), c[0].view(torch.int64))
     assert torch.equal(a[2].view(torch.int32), c[2].view(torch.int32))
     assert torch.equal(a[3].view(torch.int32), c[3].view(torch.int32))
+
+
+def test_batched_coherence_tile_layouts_equal():
+    """Every tile layout of the per-stage coherence kernel (waves per workgroup
+    over its 32 x 32 tile grid) gives the default layout's outputs bit for bit;
+    a layout the grid does not have is rejected."""
+    import torch
+
+    from moraine_amd.batched import AutoTuner, DSBatch, DSBatchRunner
+
+    s, ro, ni, ns = synth.ds_samples_torch(900, n_choices=(70, 150, 200), k_range=(20, 240), seed=8, device="cuda")
+    b = DSBatch.from_tensors(s, ro, ni, ns)
+    r = DSBatchRunner(b)
+    ref = [x.clone() for x in r.run()]
+    for g in r.groups:
+        for v in AutoTuner.coh_waves(g.n):
+            _case(f"ds_batch coherence layout N={g.n} waves={v}")
+            r.ph.zero_()
+            r.quality.zero_()
+            r.t_coh.zero_()
+            r._call(g, -1, g.chunk, v)
+            torch.cuda.synchronize()
+            sel = g.pix.long()
+            assert torch.equal(r.quality[sel].view(torch.int32), ref[2][sel].view(torch.int32))
+            assert torch.equal(r.t_coh[sel].view(torch.int32), ref[3][sel].view(torch.int32))
+            offs = ref[1][sel]
+            for o in offs[:64].tolist():
+                assert torch.equal(r.ph[o:o + g.n].view(torch.int64), ref[0][o:o + g.n].view(torch.int64))
+    assert r.groups[1].n == 150
+    with pytest.raises(NotImplementedError):
+        r._call(r.groups[1], -1, r.groups[1].chunk, 2)  # no two-wave layout for a 5 x 5 tile grid
