@@ -680,7 +680,7 @@ def main():
     if dom == "emi+ds_temp_coh":
         ach = emi_flops * (npx / nb) / (t_emi / nb) / 1e12
         ek = ["emi_fast_kernel<64, true, false, 1>", "emi_fast_kernel<64, true, true, 2>",
-              "emi_general_kernel<true, HIP_vector_type<float, 2u> >"]
+              "emi_general_kernel<true, float2 >"]
         roof = {"kernel": " + ".join(ek) if 32 < N <= 64 else "emi_wide_kernel", "bound": "valu", "achieved": ach,
                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
                 "traffic": measured_traffic(ek, args),
