@@ -252,7 +252,9 @@ def batched_leg(args, torch):
                     f"tau~U[12,96] d",
         "value": args.batched_px / t_graph, "unit": "DS pixels/s", "s_per_batch": t_graph,
         "per_stage_value": args.batched_px / t_stage, "fused_vs_per_stage": t_stage / t_graph,
-        "tuner_picks": tuner.picks, "groups": groups,
+        "tuner_picks": tuner.picks,
+        "tuner_trials_ms": {k: {str(c): round(v * 1e3, 3) for c, v in t.items()} for k, t in tuner.trials.items()},
+        "groups": groups,
         "phase_rms_vs_fp64": {"median": float(np.median(rms_all)), "p99": float(np.quantile(rms_all, 0.99)),
                               "max": float(rms_all.max()), "pixels": int(rms_all.size)},
     }

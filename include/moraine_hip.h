@@ -291,23 +291,30 @@ int mr_emi_refined(const float* coh, long long n_pts, int n, int ref, double tol
  *   pairs      the full triu network (n_pairs = n(n-1)/2);
  *   ph         complex64, pixel id's n phases at complex element ph_off[id];
  *   quality, t_coh  float32 indexed by pixel id.
- * 32 < n <= 64 runs the fused launches (coherence stays on chip; `blocks`
- * sets the first one's grid, 0 = default; workspace >=
- * mr_ds_batch_fused_workspace_bytes(n_pix) for the lists of pixels the
- * shift-invert EMI leaves to the Householder pass and of those the FP64
- * re-solve takes, as in mr_emi); n <= 32, n > 64, or blocks < 0 (per-stage mode:
- * coherence, EMI and temporal coherence as separate launches through HBM, the
- * drop-in sequence), runs per chunk of `chunk` pixels with
- * workspace >= mr_ds_batch_workspace_bytes(chunk, n).  coh_waves: the tile
- * layout of the per-stage coherence kernel for 64 < n <= 224 (its grid of
- * 32 x 32 tiles spread over that many waves per workgroup: 0 = the drop-in's
- * layout, 4 waves up to n = 128 and 8 above; 2 for n <= 96, 4 or 8); the
- * outputs do not depend on it.  For a sample matrix
- * made of a pixel's SHP stacks in window order the outputs equal the drop-in
- * composition exactly.
+ * blocks >= 0 runs the fused launches for n <= 200 (coherence never goes
+ * through an HBM buffer of the chunk; `blocks` sets the grid, 0 = default):
+ * 32 < n <= 64 one wave per pixel with the coherence in LDS tiles, a
+ * shift-invert pass, a Householder pass over the pixels it declines and an
+ * FP64 re-solve of those the conditioning test flags, as in mr_emi; n <= 32
+ * one wave per pixel, coherence in an LDS tile; 64 < n <= 200 one workgroup
+ * per pixel whose coherence row goes to that workgroup's slot of the
+ * workspace (re-used for each of its pixels).  Workspace >=
+ * mr_ds_batch_fused_workspace_bytes(n_pix, n, blocks): the pixel lists for
+ * 32 < n <= 64 (256 + 8 bytes per pixel), min(n_pix, blocks or 2048) slots of
+ * n(n-1)/2 complex64 for 64 < n <= 200, nothing for n <= 32.
+ * n > 200, or blocks < 0 (per-stage mode: coherence, EMI and temporal
+ * coherence as separate launches through HBM, the drop-in sequence), runs per
+ * chunk of `chunk` pixels with workspace >= mr_ds_batch_workspace_bytes(chunk, n).
+ * coh_waves: the tile layout of the per-stage coherence kernel for
+ * 64 < n <= 224 (its grid of 32 x 32 tiles spread over that many waves per
+ * workgroup: 0 = the drop-in's layout, 3 waves up to n = 96, 4 up to n = 128 and
+ * 8 above; 2 or 3 for n <= 96, 4 or 8); the outputs do not depend on it, nor on
+ * the mode or grid.
+ * For a sample matrix made of a pixel's SHP stacks in window order the outputs
+ * equal the drop-in composition exactly.
  * ------------------------------------------------------------------------- */
 size_t mr_ds_batch_workspace_bytes(long long chunk, int n);
-size_t mr_ds_batch_fused_workspace_bytes(long long n_pix);
+size_t mr_ds_batch_fused_workspace_bytes(long long n_pix, int n, int blocks);
 int mr_ds_batch(const float* samples, const long long* row_off, const int32_t* n_samples,
                 const long long* ph_off, const int32_t* pix, long long n_pix, int n, int ref,
                 const int32_t* pairs, int n_pairs, float* ph, float* quality, float* t_coh, int blocks,

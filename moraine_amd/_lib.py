@@ -56,7 +56,7 @@ SIGNATURES = {
     "mr_nearest_pd_f32": (_i, [_vp, _ll, _i, _vp, _vp, _sz, _vp]),
     "mr_nearest_pd_f64": (_i, [_vp, _ll, _i, _vp, _vp, _sz, _vp]),
     "mr_ds_batch_workspace_bytes": (_sz, [_ll, _i]),
-    "mr_ds_batch_fused_workspace_bytes": (_sz, [_ll]),
+    "mr_ds_batch_fused_workspace_bytes": (_sz, [_ll, _i, _i]),
     "mr_ds_batch": (_i, [_vp, _vp, _vp, _vp, _vp, _ll, _i, _i, _vp, _i, _vp, _vp, _vp, _i, _ll, _i, _vp, _sz, _vp]),
 }
 

@@ -4,10 +4,12 @@ Each pixel carries its own complex64 sample matrix (K_i SHP stacks of N_i
 images, row-major K_i x N_i) and goes through coherence -> EMI -> temporal
 coherence, as moraine's batch driver does for a point cloud
 (moraine/pl.py:148-168).  Pixels are grouped by image count; each group is one
-call of ``mr_ds_batch`` (csrc/batch.hip): a fused one-wave-per-pixel launch for
-32 < N <= 64 (the coherence matrix stays in LDS), chunked coherence / EMI /
-temporal-coherence launches otherwise.  Launch shapes (grid size of the fused
-kernel, chunk size of the large-N path) come from :class:`AutoTuner`, and a
+call of ``mr_ds_batch`` (csrc/batch.hip): fused launches for N <= 200 (one wave
+per pixel with the coherence in LDS for N <= 64, one workgroup per pixel with
+the coherence row in a per-workgroup slot above), chunked coherence / EMI /
+temporal-coherence launches for N > 200 or in the per-stage mode.  Launch
+shapes (fused or staged, grid size of the fused kernel, chunk size, tile layout
+of the staged coherence kernel) come from :class:`AutoTuner`, and a
 :class:`DSBatchRunner` can capture all group launches in one HIP graph and
 replay it on new data written into the same buffers.
 
@@ -103,8 +105,14 @@ class AutoTuner:
     set), in a JSON file, so a later process starts from them.
     """
 
-    FUSED_BLOCKS = (0, 4096, 16384, 65536)  # 0 = one block per pixel (capped)
+    FUSED_BLOCKS = (0, 4096, 16384, 65536)  # one-wave fused launches, N <= 64 (0: one per pixel, capped)
+    WIDE_BLOCKS = (0, 1024, 4096)  # workgroup-per-pixel fused launch, 64 < N <= 200 (0: 2048)
     CHUNKS = (4096, 8192, 16384, 32768, 65536)
+
+    @staticmethod
+    def fused_blocks(n: int) -> tuple:
+        """Grid sizes tried for the fused launch of an N-image group."""
+        return AutoTuner.WIDE_BLOCKS if n > 64 else AutoTuner.FUSED_BLOCKS
 
     @staticmethod
     def coh_waves(n: int) -> tuple:
@@ -112,7 +120,7 @@ class AutoTuner:
         workgroup over its grid of 32 x 32 tiles (() where it has one layout)."""
         if not 64 < n <= 224:
             return ()
-        return (2, 4, 8) if n <= 96 else (4, 8)
+        return (2, 3, 4, 8) if n <= 96 else (4, 8)
 
     def __init__(self, path: str | None = None, reps: int = 2, timer=None):
         self.path = path or os.environ.get("MORAINE_AMD_TUNE_CACHE")
@@ -179,7 +187,7 @@ class DSBatchRunner:
         self.ref = int(ref)
         # per_stage: coherence, EMI and temporal coherence as separate launches
         # through HBM for every group (the drop-in sequence), instead of the
-        # fused launch for N <= 64
+        # fused launches for N <= 200
         self.per_stage = bool(per_stage)
         self.tuner = tuner
         dev = batch.samples.device
@@ -210,18 +218,20 @@ class DSBatchRunner:
                   _dev.ptr(self.t_coh), int(blocks), int(chunk), int(g.coh_waves if coh_waves is None else coh_waves),
                   _dev.ptr(ws), 0 if ws is None else ws.numel(), _dev.stream())
 
+    FUSED_MAX_N = 200  # largest N with a fused launch
+
     def _staged(self, g: _Group) -> bool:
-        return self.per_stage or g.n > 64 or g.n <= 32
+        return self.per_stage or g.n > self.FUSED_MAX_N
 
     def _chunk_fits(self, n: int, chunk: int) -> bool:
         return _lib.query("mr_ds_batch_workspace_bytes", chunk, n) <= self.max_ws
 
     def tune(self):
         """Choose each group's launch shape (autotuner if given, else defaults)
-        and allocate the shared workspace.  For 32 < N <= 64 the tuner also
-        chooses between the fused launch (one of FUSED_BLOCKS grid sizes) and
-        the staged one (coherence chunk through HBM, then the EMI + temporal
-        coherence kernel; one of CHUNKS), encoded as -chunk: both give the same
+        and allocate the shared workspace.  For N <= 200 the tuner also
+        chooses between the fused launch (one of fused_blocks(N) grid sizes)
+        and the staged one (coherence chunk through HBM, then EMI and temporal
+        coherence; one of CHUNKS), encoded as -chunk: both give the same
         outputs bit for bit, and which is faster depends on N and the SHP
         counts.  For staged groups with 64 < N <= 224 it then chooses the tile
         layout of the coherence kernel (waves per workgroup over its 32 x 32
@@ -230,8 +240,10 @@ class DSBatchRunner:
         need = 0
         for g in self.groups:
             g.staged = self._staged(g)
-            if not g.staged:  # the fused launch's fallback list
-                need = max(need, _lib.query("mr_ds_batch_fused_workspace_bytes", g.count))
+            if not g.staged:  # the fused launch's lists or slots, for every grid the tuner may try
+                blocks = AutoTuner.fused_blocks(g.n) if self.tuner is not None else (0,)
+                need = max([need] + [_lib.query("mr_ds_batch_fused_workspace_bytes", g.count, g.n, b)
+                                     for b in blocks])
             if g.staged or self.tuner is not None:
                 cands = [c for c in AutoTuner.CHUNKS if self._chunk_fits(g.n, c)] or [1024]
                 g.chunk = min(cands[-1], max(g.count, 1))
@@ -243,7 +255,7 @@ class DSBatchRunner:
                     continue
                 chunks = [c for c in AutoTuner.CHUNKS if self._chunk_fits(g.n, c) and c <= g.chunk] or [g.chunk]
                 if not g.staged:
-                    cands = list(AutoTuner.FUSED_BLOCKS) + [-c for c in chunks]
+                    cands = list(AutoTuner.fused_blocks(g.n)) + [-c for c in chunks]
                     pick = self.tuner.choose(
                         AutoTuner.key("fused_or_staged", g.n, g.count), cands,
                         lambda v, g=g: self._call(g, v, 0) if v >= 0 else self._call(g, -1, -v))

@@ -5,26 +5,36 @@
 // ds_temp_coh per batch) over ragged per-pixel inputs, grouped by N by the
 // caller (one call per image count).
 //
-// 32 < n <= 64: fused launches.  One wave per pixel accumulates the coherence on
-// MFMA into LDS tiles (co_mfma.hpp), runs EMI reading the tiles (emi_fast.hpp),
-// and forms the temporal coherence from the same tiles (tcoh.hpp); the
-// coherence matrix never reaches HBM.  As in the plain EMI, a shift-invert
-// pass, a Householder pass over the pixels it declines (with the conditioning
-// test) and an FP64 re-solve of the pixels that test flags, each rebuilding
-// the pixel's tiles.  Every stage shares its code with the
-// drop-in kernels, so for a sample matrix made of a pixel's SHP stacks in
-// window order the outputs equal emperical_co_pc -> emi -> ds_temp_coh exactly.
-// n <= 32, n > 64 (or any n with blocks < 0, the per-stage mode): per chunk of pixels,
-// the coherence kernel in sample-matrix mode (the drop-in's choice for n),
-// EMI and the temporal-coherence kernel as separate launches through an HBM
-// coherence buffer, then a scatter into the ragged outputs.  The per-stage
-// mode is what the drop-in functions do for one group; it gives the same
-// results as the fused launch.  (For n <= 32 EMI runs the single-wave
-// two-lanes-per-row kernel, faster than the one-lane-per-row body the fused
-// launch is built on, so those groups always take the per-stage path.)
+// Fused launches (blocks >= 0), one per stage chain, for every n <= 200:
+//  * 32 < n <= 64: one wave per pixel accumulates the coherence on MFMA into
+//    LDS tiles (co_mfma.hpp), runs EMI reading the tiles (emi_fast.hpp), and
+//    forms the temporal coherence from the same tiles (tcoh.hpp).  As in the
+//    plain EMI, a shift-invert pass, a Householder pass over the pixels it
+//    declines (with the conditioning test) and an FP64 re-solve of the pixels
+//    that test flags, each rebuilding the pixel's tiles.
+//  * n <= 32: one wave per pixel, coherence into one LDS tile, EMI by the
+//    two-lanes-per-row body (emi_wide.hpp, the drop-in's EMI for this n) reading
+//    the tile, temporal coherence from the tile.
+//  * 64 < n <= 200: one workgroup of ceil(n / 32) waves per pixel.  The waves
+//    accumulate the coherence tiles (the per-stage coherence kernel's body,
+//    co_mfma.hpp) and write the row to the workgroup's own slot of the
+//    workspace; the EMI body (emi_wide.hpp) and the temporal coherence read it
+//    there.  The slot is re-used for every pixel of the workgroup, so the row
+//    stays in the L2 / Infinity Cache instead of going through an HBM buffer
+//    of the whole chunk.
+// Every stage shares its code with the drop-in kernels, so for a sample matrix
+// made of a pixel's SHP stacks in window order the outputs equal
+// emperical_co_pc -> emi -> ds_temp_coh exactly.
+// n > 200, or any n with blocks < 0 (the per-stage mode): per chunk of pixels,
+// the coherence kernel in sample-matrix mode (the drop-in's choice for n), EMI
+// and the temporal-coherence kernel as separate launches through an HBM
+// coherence buffer, then a scatter into the ragged outputs.  The per-stage mode
+// is what the drop-in functions do for one group; it gives the same results as
+// the fused launches.
 #include "co_mfma.hpp"
 #include "emi_fast.hpp"
 #include "emi_general.hpp"
+#include "emi_wide.hpp"
 #include "tcoh.hpp"
 
 namespace mr {
@@ -150,6 +160,111 @@ __host__ __device__ inline size_t batch_resolve_lds(int n) {
   return (size_t)com_tile_floats(true) * 4 + 128 * 4 + emi_gen_slice_bytes(n);
 }
 
+// Outputs of pixel id: phases ph[off + i], quality[id] (ragged batch).
+struct BatchOut {
+  float2* phs;
+  float* qs;
+  long long off;
+  int pid;
+  __device__ __forceinline__ void ph(int i, float2 v) const { phs[off + i] = v; }
+  __device__ __forceinline__ void quality(float v) const { qs[pid] = v; }
+  __device__ __forceinline__ int id() const { return pid; }
+};
+
+// n <= 32: pixel t's coherence into one LDS tile, EMI by the drop-in's body
+// for this n reading the tile, temporal coherence from the tile and the linked
+// phases (kept in LDS by the EMI body).
+__host__ __device__ inline size_t batch_small_lds() {
+  return ((ew_lds_bytes(32) + 15) & ~(size_t)15) + (size_t)com_tile_floats(false) * 4 + 128 * 4 + 32 * 8;
+}
+
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void ds_batch_fused_small_kernel(
+    const float2* __restrict__ samples, const long long* __restrict__ row_off, const int32_t* __restrict__ n_samp,
+    const long long* __restrict__ ph_off, const int32_t* __restrict__ pix, long long npix, int n, int ref,
+    const int32_t* __restrict__ pairs, int npairs, float2* __restrict__ ph, float* __restrict__ quality,
+    float* __restrict__ tcoh) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  ew_lds_init<32>(smem);
+  unsigned char* rest = smem + ((ew_lds_bytes(32) + 15) & ~(size_t)15);
+  float2* tiles = reinterpret_cast<float2*>(rest);
+  float* sc = reinterpret_cast<float*>(rest + (size_t)com_tile_floats(false) * 4);  // scales, 1/sqrt(diag)
+  float2* phl = reinterpret_cast<float2*>(sc + 128);                                 // linked phases
+  for (long long t = blockIdx.x; t < npix; t += gridDim.x) {
+    int lane = threadIdx.x;
+    __asm__ volatile("" : "+v"(lane));  // per-pixel opaque lane (see emi.hip)
+    const int id = pix[t];
+    co_mfma_point<false>(samples + row_off[id], RowIdentity{}, n_samp[id], n, lane, tiles, sc);
+    const CohTiles<false> ct{tiles, sc + 64};
+    ew_point<2, 16, 8, 4, false>(ct, n, ref, smem, BatchOut{ph, quality, ph_off[id], id}, 0.0, nullptr, nullptr, phl);
+    const float tc = tcoh_point<true>([&](int, int a, int b) { return ct(a, b); }, [&](int i) { return phl[i]; },
+                                      pairs, npairs, lane);
+    if (lane == 0) tcoh[id] = tc;
+    __syncthreads();  // the next pixel overwrites the tile
+  }
+}
+
+// 64 < n <= 200: NB = ceil(n / 32) waves per workgroup, as the EMI body's two
+// lanes per row need.  Per pixel: coherence row into the workgroup's slot
+// (slots + blockIdx.x * npairs), EMI from the slot, DS temporal coherence from
+// the slot on wave 0.
+__host__ __device__ inline size_t batch_wide_lds(int e, int nb) {
+  return ((ew_lds_bytes(2 * e) + 15) & ~(size_t)15) + (size_t)2 * 32 * nb * 4 + (size_t)2 * e * 8 +
+         (size_t)nb * 32 * 32 * 8;
+}
+
+template <int E, int GS, int GH, int NB>
+__global__ __launch_bounds__(64 * NB) __attribute__((amdgpu_waves_per_eu(E <= 64 ? 3 : 1))) void
+ds_batch_fused_wide_kernel(const float2* __restrict__ samples, const long long* __restrict__ row_off,
+                           const int32_t* __restrict__ n_samp, const long long* __restrict__ ph_off,
+                           const int32_t* __restrict__ pix, long long npix, int n, int ref,
+                           const int32_t* __restrict__ pairs, int npairs, float2* __restrict__ ph,
+                           float* __restrict__ quality, float* __restrict__ tcoh, float2* __restrict__ slots) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  ew_lds_init<2 * E>(smem);
+  unsigned char* rest = smem + ((ew_lds_bytes(2 * E) + 15) & ~(size_t)15);
+  float* sc = reinterpret_cast<float*>(rest);                 // 32 NB scales
+  float* rs = sc + 32 * NB;                                   // 32 NB 1/sqrt(diag)
+  float2* phl = reinterpret_cast<float2*>(rs + 32 * NB);      // 2E linked phases
+  float2* stage = phl + 2 * E;                                // NB 32x32 tiles (other pair orders)
+  const bool canon = co_pairs_canonical(pairs, npairs, n);
+  float2* slot = slots + (size_t)blockIdx.x * npairs;
+  for (long long t = blockIdx.x; t < npix; t += gridDim.x) {
+    const int id = pix[t];
+    cow_point<true, NB, NB, true>(samples + row_off[id], nullptr, n_samp[id], n, pairs, npairs, canon, sc, rs, stage,
+                            slot);
+    __syncthreads();  // the row is complete in the slot
+    ew_point<2, E, GS, GH, false>(EwPairRow{slot, n}, n, ref, smem, BatchOut{ph, quality, ph_off[id], id}, 0.0,
+                                  nullptr, nullptr, phl);
+    if (threadIdx.x < 64) {
+      int lane = threadIdx.x;
+      __asm__ volatile("" : "+v"(lane));
+      const float tc = tcoh_point<true>([&](int p, int, int) { return slot[p]; }, [&](int i) { return phl[i]; },
+                                        pairs, npairs, lane);
+      if (lane == 0) tcoh[id] = tc;
+    }
+    __syncthreads();  // the next pixel overwrites the slot
+  }
+}
+
+// Grid of the 64 < n <= 200 fused launch: `blocks` if positive, else 2048
+// workgroups (several rounds of the resident count at every n, for balance),
+// never more than the pixels.
+constexpr int BATCH_WIDE_DEFAULT_BLOCKS = 2048;
+inline long long batch_wide_grid(long long n_pix, int blocks) {
+  const long long nb = blocks > 0 ? blocks : BATCH_WIDE_DEFAULT_BLOCKS;
+  return n_pix < nb ? n_pix : nb;
+}
+
+template <int E, int GS, int GH, int NB>
+static int batch_wide_launch(const float2* smp, const long long* row_off, const int32_t* n_samples,
+                             const long long* ph_off, const int32_t* pix, long long n_pix, int n, int ref,
+                             const int32_t* pairs, int n_pairs, float2* ph, float* quality, float* t_coh, int nb,
+                             float2* slots, hipStream_t st) {
+  hipLaunchKernelGGL((ds_batch_fused_wide_kernel<E, GS, GH, NB>), dim3(nb), dim3(64 * NB), batch_wide_lds(E, NB), st,
+                     smp, row_off, n_samples, ph_off, pix, n_pix, n, ref, pairs, n_pairs, ph, quality, t_coh, slots);
+  return launch_status("ds_batch fused (64 < n <= 200)");
+}
+
 __global__ void ds_batch_scatter_kernel(const float2* __restrict__ ph_t, const float* __restrict__ q_t,
                                         const float* __restrict__ t_t, const int32_t* __restrict__ pix, long long c0,
                                         long long cn, int n, const long long* __restrict__ ph_off,
@@ -174,8 +289,12 @@ static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 extern "C" {
 
-size_t mr_ds_batch_fused_workspace_bytes(long long n_pix) {
-  return 256 + 2 * ((((size_t)(n_pix > 0 ? n_pix : 0) * 4 + 255) & ~(size_t)255));
+size_t mr_ds_batch_fused_workspace_bytes(long long n_pix, int n, int blocks) {
+  if (n_pix <= 0) return 0;
+  if (n > 32 && n <= 64) return 256 + 2 * ((((size_t)n_pix * 4 + 255) & ~(size_t)255));
+  if (n > 64 && n <= mr::EMI_WIDE_MAX_N)
+    return (size_t)mr::batch_wide_grid(n_pix, blocks) * ((size_t)n * (n - 1) / 2) * sizeof(float2);
+  return 0;  // n <= 32: on chip; n > 200: no fused launch
 }
 
 size_t mr_ds_batch_workspace_bytes(long long chunk, int n) {
@@ -200,8 +319,30 @@ int mr_ds_batch(const float* samples, const long long* row_off, const int32_t* n
   hipStream_t st = mr::as_stream(stream);
   const float2* smp = reinterpret_cast<const float2*>(samples);
   float2* phc = reinterpret_cast<float2*>(ph);
+  if (n <= 32 && blocks >= 0) {
+    const int nb = (int)std::min<long long>(n_pix, blocks > 0 ? blocks : mr::BATCH_DEFAULT_BLOCKS);
+    hipLaunchKernelGGL(mr::ds_batch_fused_small_kernel, dim3(nb), dim3(64), mr::batch_small_lds(), st, smp, row_off,
+                       n_samples, ph_off, pix, n_pix, n, ref, pairs, n_pairs, phc, quality, t_coh);
+    return mr::launch_status("ds_batch fused (n <= 32)");
+  }
+  if (n > 64 && n <= mr::EMI_WIDE_MAX_N && blocks >= 0) {
+    const size_t need = mr_ds_batch_fused_workspace_bytes(n_pix, n, blocks);
+    MR_CHECK_ARG(workspace && workspace_bytes >= need, "ds_batch: fused launch workspace %zu < %zu bytes",
+                 workspace_bytes, need);
+    const int nb = (int)mr::batch_wide_grid(n_pix, blocks);
+    float2* slots = reinterpret_cast<float2*>(workspace);
+#define MR_BW(E, GS, GH, NB)                                                                                       \
+  mr::batch_wide_launch<E, GS, GH, NB>(smp, row_off, n_samples, ph_off, pix, n_pix, n, ref, pairs, n_pairs, phc,  \
+                                       quality, t_coh, nb, slots, st)
+    if (n <= 96) return MR_BW(48, 4, 2, 3);
+    if (n <= 128) return MR_BW(64, 4, 2, 4);
+    if (n <= 160) return MR_BW(80, 8, 4, 5);
+    if (n <= 192) return MR_BW(100, 4, 2, 6);
+    return MR_BW(100, 4, 2, 7);
+#undef MR_BW
+  }
   if (n > 32 && n <= 64 && blocks >= 0) {
-    const size_t need = mr_ds_batch_fused_workspace_bytes(n_pix);
+    const size_t need = mr_ds_batch_fused_workspace_bytes(n_pix, n, blocks);
     MR_CHECK_ARG(workspace && workspace_bytes >= need, "ds_batch: fused launch workspace %zu < %zu bytes",
                  workspace_bytes, need);
     unsigned* count = reinterpret_cast<unsigned*>(workspace);
@@ -227,7 +368,7 @@ int mr_ds_batch(const float* samples, const long long* row_off, const int32_t* n
                        res_list);
     return mr::launch_status("ds_batch FP64 re-solve");
   }
-  MR_CHECK_ARG(chunk > 0, "ds_batch: chunk must be positive for n <= 32, n > 64 or the per-stage mode");
+  MR_CHECK_ARG(chunk > 0, "ds_batch: chunk must be positive for n > 200 or the per-stage mode");
   const size_t need = mr_ds_batch_workspace_bytes(chunk, n);
   MR_CHECK_ARG(workspace && workspace_bytes >= need, "ds_batch: workspace %zu < %zu bytes", workspace_bytes, need);
   const long long P = (long long)n * (n - 1) / 2;

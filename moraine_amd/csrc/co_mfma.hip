@@ -130,27 +130,12 @@ __global__ __launch_bounds__(64 * COM_WAVES) void co_mfma_kernel(
 // (21x41 windows) / n = 90 (15x21).  PF prefetches one chunk of operands ahead
 // (off: the registers cost more than it hides).
 // ---------------------------------------------------------------------------
-// upper tiles of the NB x NB block grid in row-major order: t -> (a, b), a <= b
-template <int NB>
-__device__ __forceinline__ void cow_tile_ab(int t, int& a, int& b) {
-  a = 0;
-#pragma unroll
-  for (int i = 0; i + 1 < NB; ++i) {
-    const bool past = a == i && t >= NB - i;  // beyond row i (NB - i tiles)
-    t = past ? t - (NB - i) : t;
-    a += past ? 1 : 0;
-  }
-  b = a + t;
-}
-
-// list | scales (32 NB) | 1/sqrt(diag) (32 NB) | one staged tile per wave | K
-__host__ __device__ inline size_t cow_lds_bytes(int win, int nb, int waves) {
-  return (((size_t)win * 4 + 15) & ~(size_t)15) + (size_t)2 * 32 * nb * 4 + (size_t)waves * 32 * 32 * 8 + 16;
-}
-
 // SAMP: point pt is pixel pix[pix0 + pt], its K = n_samp[id] rows are the
 // sample matrix at rslc + row_off[id] (batched form), no window.
-template <bool SAMP, int NB, int W, bool PF>
+// XCD: workgroups are dispatched round-robin over the 8 XCDs (block b runs on
+// XCD b % 8); renumbered, every XCD takes a contiguous range of points, whose
+// overlapping windows then share that XCD's L2 (the grid is a multiple of 8).
+template <bool SAMP, int NB, int W, bool XCD = false>
 __global__ __launch_bounds__(64 * W) void co_mfma_wide_kernel(
     const float2* __restrict__ rslc, int nl, int w, int n, const int32_t* __restrict__ idx,
     const uint8_t* __restrict__ shp, long long npts, int aw, int rw, const int32_t* __restrict__ pairs, int npairs,
@@ -167,10 +152,10 @@ __global__ __launch_bounds__(64 * W) void co_mfma_wide_kernel(
   const int wv = threadIdx.x >> 6;
   const int lane0 = threadIdx.x & 63;
   const bool canon = co_pairs_canonical(pairs, npairs, n);
-  for (long long pt = blockIdx.x; pt < npts; pt += gridDim.x) {
+  const unsigned bx = XCD ? (blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8 : blockIdx.x;
+  for (long long pt = bx; pt < npts; pt += gridDim.x) {
     int lane = lane0;
     __asm__ volatile("" : "+v"(lane));
-    const int hf = lane >> 5, c32 = lane & 31;
     // ---- in-bounds SHPs in window order (wave 0) -----------------------------
     const float2* base = rslc;
     if (SAMP) {
@@ -184,172 +169,42 @@ __global__ __launch_bounds__(64 * W) void co_mfma_wide_kernel(
       if (lane == 0) sK[0] = K;
     }
     __syncthreads();
-    const int K = sK[0];
-    // ---- per-image power-of-two scale: wave w, image blocks w, w + W, ... -------
-    for (int ib = wv; ib < NB; ib += W) {
-      const int img = 32 * ib + c32;
-      float mx = 0.f;
-      if (img < n) {
-        // eight independent loads in flight per lane (the two half-waves take
-        // alternate SHPs); a short last trip re-reads SHP K-1, which leaves
-        // the maximum unchanged
-        for (int k0 = 0; k0 < K; k0 += 16) {
-          float2 z[8];
-#pragma unroll
-          for (int u = 0; u < 8; ++u) {
-            const int k = k0 + 2 * u + hf < K ? k0 + 2 * u + hf : K - 1;
-            z[u] = base[(size_t)(SAMP ? k : list[k]) * n + img];
-          }
-#pragma unroll
-          for (int u = 0; u < 8; ++u) mx = fmaxf(mx, fmaxf(fabsf(z[u].x), fabsf(z[u].y)));
-        }
-      }
-      mx = fmaxf(mx, __shfl_xor(mx, 32, 64));
-      float s = 1.f;
-      if (mx > 0.f && mx < INFINITY) {
-        int e;
-        frexpf(mx, &e);
-        s = ldexpf(1.f, 6 - e);  // scaled maximum in [32, 64)
-      }
-      if (hf == 0) sc[img] = s;
-    }
-    __syncthreads();
-    // ---- accumulate this wave's tiles ------------------------------------------
-    constexpr int NT = NB * (NB + 1) / 2;
-    constexpr int NS = (NT + W - 1) / W;  // tile slots per wave
-    int ta[NS], tb[NS];
-    bool on[NS];
-    float sa[NS], sb[NS];
-#pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      const int t = wv + W * s;
-      on[s] = t < NT;
-      cow_tile_ab<NB>(on[s] ? t : 0, ta[s], tb[s]);
-      sa[s] = sc[32 * ta[s] + c32];
-      sb[s] = sc[32 * tb[s] + c32];
-    }
-    f16x_t ar[NS] = {}, ai[NS] = {};
-    const int nch = (K + 7) >> 3;
-    // this wave's operand rows for chunk c (one chunk is prefetched ahead)
-    auto load = [&](int c, float2(&xa)[NS][4], float2(&xb)[NS][4]) {
-#pragma unroll
-      for (int s = 0; s < NS; ++s) {
-        if (!on[s]) continue;
-        const int ra = 32 * ta[s] + c32, rb = 32 * tb[s] + c32;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int k = c * 8 + hf * 4 + q;
-          const bool okk = k < K;
-          const size_t off = okk ? (size_t)(SAMP ? k : list[k]) * n : 0;
-          xa[s][q] = (okk && ra < n) ? base[off + ra] : make_float2(0.f, 0.f);
-          xb[s][q] = (okk && rb < n) ? base[off + rb] : make_float2(0.f, 0.f);
-        }
-      }
-    };
-    float2 xa[NS][4], xb[NS][4];
-    if (PF && nch > 0) load(0, xa, xb);
-    for (int c = 0; c < nch; ++c) {
-      float2 na[NS][4], nb[NS][4];
-      if (!PF) load(c, xa, xb);
-      if (PF && c + 1 < nch) load(c + 1, na, nb);
-#pragma unroll
-      for (int s = 0; s < NS; ++s) {
-        if (!on[s]) continue;
-        float v[8];
-        half8_t x0h, x0l, y0h, y0l, x1h, x1l, y1h, y1l;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          v[2 * q] = xa[s][q].x * sa[s];
-          v[2 * q + 1] = xa[s][q].y * sa[s];
-        }
-        split4c(v, x0h, x0l, y0h, y0l);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          v[2 * q] = xb[s][q].x * sb[s];
-          v[2 * q + 1] = xb[s][q].y * sb[s];
-        }
-        split4c(v, x1h, x1l, y1h, y1l);
-        ar[s] = mfma3(x0h, x0l, x1h, x1l, ar[s]);
-        ai[s] = mfma3(y0h, y0l, x1h, x1l, ai[s]);
-      }
-      if (PF && c + 1 < nch) {
-#pragma unroll
-        for (int s = 0; s < NS; ++s)
-#pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            xa[s][q] = na[s][q];
-            xb[s][q] = nb[s][q];
-          }
-      }
-    }
-    // ---- 1/sqrt(S'_mm) from the diagonal tiles -----------------------------------
-#pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      if (!on[s] || ta[s] != tb[s]) continue;
-      // select the diagonal entry this lane owns, then one reciprocal square root
-      bool own = false;
-      float d = 0.f;
-#pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        const bool hit = 8 * (q >> 2) + 4 * hf + (q & 3) == c32;
-        own = own || hit;
-        d = hit ? ar[s][q] : d;
-      }
-      const int m = 32 * ta[s] + c32;
-      if (own) rs[m] = 1.f / sqrtf(m < n ? d : 0.f);
-    }
-    __syncthreads();
-    // ---- output ------------------------------------------------------------------
-    float2* out = coh + (size_t)pt * npairs;
-    float2* tile = stage + wv * 1024;
-#pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      if (!on[s]) continue;
-      if (canon) {
-        co_write_tile_direct(ar[s], ai[s], ta[s], tb[s], n, lane, rs, out);
-      } else {
-        co_store_tile(tile, ar[s], ai[s], lane);
-        __builtin_amdgcn_wave_barrier();
-        for (int p = lane; p < npairs; p += 64) {
-          const int a = pairs[2 * p], b = pairs[2 * p + 1];
-          const int lo = min(a, b), hi = max(a, b);
-          if ((lo >> 5) != ta[s] || (hi >> 5) != tb[s]) continue;
-          float2 v = tile[(lo & 31) * 32 + (hi & 31)];
-          if (a > b) v.y = -v.y;
-          const float f = rs[a] * rs[b];
-          out[p] = make_float2(v.x * f, v.y * f);
-        }
-        __builtin_amdgcn_wave_barrier();
-      }
-    }
+    cow_point<SAMP, NB, W>(base, list, sK[0], n, pairs, npairs, canon, sc, rs, stage, coh + (size_t)pt * npairs);
     __syncthreads();
   }
 }
 
-// Tile grid and waves per workgroup for n: 3x3 / 4x4 grids on 4 waves, 5x5 to
-// 7x7 on 8 waves (at most four tiles per wave).
+// Tile grid and waves per workgroup for n: one wave per tile row of the 3x3 and
+// 4x4 grids (two tiles per wave), 5x5 to 7x7 on 8 waves (at most four tiles per
+// wave); the point-cloud form in XCD-contiguous point order for the 3x3 and
+// 4x4 grids.  Measured on 15x21 (n = 90, 150) and 21x41 (n = 120) windows with
+// every layout bit-identical: n = 90 4 waves 6.14 ms -> XCD order 5.72 ->
+// 3 waves 4.97 ms per 131 072 points; n = 120 13.39 -> 12.09 ms per 65 536
+// points with the XCD order; n = 150 8.53 ms on 8 waves, 8.77 with the XCD
+// order, 11.66 on 5 waves.
 #define MR_COW_DISPATCH(SAMP, LDS_WIN, ...)                                                                        \
   do {                                                                                                          \
     if (n <= 96)                                                                                                \
-      hipLaunchKernelGGL((co_mfma_wide_kernel<SAMP, 3, 4, false>), dim3(nb), dim3(64 * 4),                      \
-                         cow_lds_bytes(LDS_WIN, 3, 4), st, __VA_ARGS__);                                        \
+      hipLaunchKernelGGL((co_mfma_wide_kernel<SAMP, 3, 3, !SAMP>), dim3(nb), dim3(64 * 3),                      \
+                         cow_lds_bytes(LDS_WIN, 3, 3), st, __VA_ARGS__);                                        \
     else if (n <= 128)                                                                                          \
-      hipLaunchKernelGGL((co_mfma_wide_kernel<SAMP, 4, 4, false>), dim3(nb), dim3(64 * 4),                      \
+      hipLaunchKernelGGL((co_mfma_wide_kernel<SAMP, 4, 4, !SAMP>), dim3(nb), dim3(64 * 4),                      \
                          cow_lds_bytes(LDS_WIN, 4, 4), st, __VA_ARGS__);                                        \
     else if (n <= 160)                                                                                          \
-      hipLaunchKernelGGL((co_mfma_wide_kernel<SAMP, 5, 8, false>), dim3(nb), dim3(64 * 8),                      \
+      hipLaunchKernelGGL((co_mfma_wide_kernel<SAMP, 5, 8>), dim3(nb), dim3(64 * 8),                             \
                          cow_lds_bytes(LDS_WIN, 5, 8), st, __VA_ARGS__);                                        \
     else if (n <= 192)                                                                                          \
-      hipLaunchKernelGGL((co_mfma_wide_kernel<SAMP, 6, 8, false>), dim3(nb), dim3(64 * 8),                      \
+      hipLaunchKernelGGL((co_mfma_wide_kernel<SAMP, 6, 8>), dim3(nb), dim3(64 * 8),                             \
                          cow_lds_bytes(LDS_WIN, 6, 8), st, __VA_ARGS__);                                        \
     else                                                                                                        \
-      hipLaunchKernelGGL((co_mfma_wide_kernel<SAMP, 7, 8, false>), dim3(nb), dim3(64 * 8),                      \
+      hipLaunchKernelGGL((co_mfma_wide_kernel<SAMP, 7, 8>), dim3(nb), dim3(64 * 8),                             \
                          cow_lds_bytes(LDS_WIN, 7, 8), st, __VA_ARGS__);                                        \
   } while (0)
 
 int co_mfma_wide_launch(const float2* rslc, int nl, int w, int n, const int32_t* idx, const uint8_t* shp,
                         long long npts, int aw, int rw, const int32_t* pairs, int npairs, float2* coh, hipStream_t st) {
-  const int nb = (int)std::min<long long>(npts, 1 << 20);
+  // a multiple of 8 workgroups keeps the XCD renumbering a bijection
+  const int nb = (int)((std::min<long long>(npts, 1 << 20) + 7) / 8 * 8);
   MR_COW_DISPATCH(false, aw * rw, rslc, nl, w, n, idx, shp, npts, aw, rw, pairs, npairs, coh, nullptr, nullptr,
                   nullptr, 0ll);
   return launch_status("co_mfma_wide");
@@ -364,7 +219,7 @@ static void cow_samples(const float2* samples, const long long* row_off, const i
                         long long pix0, long long cn, int n, const int32_t* pairs, int npairs, float2* coh,
                         hipStream_t st) {
   const int nb = (int)std::min<long long>(cn, 1 << 20);
-  hipLaunchKernelGGL((co_mfma_wide_kernel<true, NB, W, false>), dim3(nb), dim3(64 * W), cow_lds_bytes(1, NB, W), st,
+  hipLaunchKernelGGL((co_mfma_wide_kernel<true, NB, W>), dim3(nb), dim3(64 * W), cow_lds_bytes(1, NB, W), st,
                      samples, 0, 0, n, nullptr, nullptr, cn, 1, 1, pairs, npairs, coh, row_off, n_samp, pix, pix0);
 }
 
@@ -372,11 +227,14 @@ template <int NB>
 static int cow_samples_waves(int waves, const float2* samples, const long long* row_off, const int32_t* n_samp,
                              const int32_t* pix, long long pix0, long long cn, int n, const int32_t* pairs,
                              int npairs, float2* coh, hipStream_t st) {
-  constexpr int kDefault = NB <= 4 ? 4 : 8;  // the drop-in's layout for this grid
+  constexpr int kDefault = NB <= 4 ? NB : 8;  // the drop-in's layout for this grid
   if (waves == 0) waves = kDefault;
   if constexpr (NB == 3) {
-    if (waves == 2) {
-      cow_samples<NB, 2>(samples, row_off, n_samp, pix, pix0, cn, n, pairs, npairs, coh, st);
+    if (waves == 2 || waves == 3) {
+      if (waves == 2)
+        cow_samples<NB, 2>(samples, row_off, n_samp, pix, pix0, cn, n, pairs, npairs, coh, st);
+      else
+        cow_samples<NB, 3>(samples, row_off, n_samp, pix, pix0, cn, n, pairs, npairs, coh, st);
       return MR_OK;
     }
   }
@@ -386,13 +244,13 @@ static int cow_samples_waves(int waves, const float2* samples, const long long* 
     cow_samples<NB, 8>(samples, row_off, n_samp, pix, pix0, cn, n, pairs, npairs, coh, st);
   } else {
     set_error("ds_batch: %d waves per coherence tile grid unsupported for n=%d (0, %s4 or 8)", waves, n,
-              NB == 3 ? "2, " : "");
+              NB == 3 ? "2, 3, " : "");
     return MR_ERR_UNSUPPORTED;
   }
   return MR_OK;
 }
 
-// waves: 0 = the drop-in's layout (4 waves for n <= 128, 8 above); 2 (n <= 96), 4 or 8.
+// waves: 0 = the drop-in's layout (3 waves for n <= 96, 4 for n <= 128, 8 above); 2 or 3 (n <= 96), 4 or 8.
 int co_mfma_wide_samples_launch(const float2* samples, const long long* row_off, const int32_t* n_samp,
                                 const int32_t* pix, long long pix0, long long cn, int n, const int32_t* pairs,
                                 int npairs, float2* coh, hipStream_t st, int waves) {

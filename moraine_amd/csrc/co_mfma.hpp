@@ -108,16 +108,15 @@ struct CoAcc {
   float s;
 };
 
-template <bool TWO, class Row>
-__device__ __forceinline__ CoAcc co_mfma_accumulate(const float2* __restrict__ base, const Row& row, int K, int n,
-                                                    int lane) {
-  const int hf = lane >> 5;  // which 8-slot half of a k-chunk this lane feeds
-  const int r0 = lane & 31;  // row of block 0 (= column of block 0 as B operand)
-  const int r1 = 32 + (lane & 31);
-  // ---- per-image power-of-two scale (lane = image) --------------------------
-  // sixteen SHPs per trip: the row lookups and gathers of a trip are all in
-  // flight before the first maximum needs them; a short last trip re-reads
-  // SHP K-1 (a repeated sample leaves the maximum unchanged)
+// Per-image power-of-two scale (lane = image): 2^(6 - e) with 2^(e-1) <= the
+// image's largest |component| over the K rows < 2^e, so the scaled maximum
+// lies in [32, 64); 1 when that maximum is 0 or not finite.  Sixteen SHPs per
+// trip: the row lookups and gathers of a trip are all in flight before the
+// first maximum needs them; a short last trip re-reads SHP K-1 (a repeated
+// sample leaves the maximum unchanged).
+template <class Row>
+__device__ __forceinline__ float co_image_scale(const float2* __restrict__ base, const Row& row, int K, int n,
+                                                int lane) {
   float mx = 0.f;
   if (lane < n) {
     for (int k = 0; k < K; k += 16) {
@@ -134,6 +133,18 @@ __device__ __forceinline__ CoAcc co_mfma_accumulate(const float2* __restrict__ b
     frexpf(mx, &e);
     s = ldexpf(1.f, 6 - e);  // scaled maximum in [32, 64)
   }
+  return s;
+}
+
+// PRE: the caller passes this lane's image scale (from co_image_scale) as s_pre;
+// otherwise it is computed here first.
+template <bool TWO, class Row, bool PRE = false>
+__device__ __forceinline__ CoAcc co_mfma_accumulate(const float2* __restrict__ base, const Row& row, int K, int n,
+                                                    int lane, float s_pre = 1.f) {
+  const int hf = lane >> 5;  // which 8-slot half of a k-chunk this lane feeds
+  const int r0 = lane & 31;  // row of block 0 (= column of block 0 as B operand)
+  const int r1 = 32 + (lane & 31);
+  const float s = PRE ? s_pre : co_image_scale(base, row, K, n, lane);
   const float s0 = __shfl(s, r0, 64);
   const float s1 = TWO ? __shfl(s, r1 < 64 ? r1 : 0, 64) : 1.f;
   // ---- main loop: 8 SHPs (16 k-slots) per chunk -----------------------------
@@ -294,5 +305,170 @@ struct CohTiles {
     return make_float2(v.x * f, v.y * f);
   }
 };
+
+// upper tiles of the NB x NB block grid in row-major order: t -> (a, b), a <= b
+template <int NB>
+__device__ __forceinline__ void cow_tile_ab(int t, int& a, int& b) {
+  a = 0;
+#pragma unroll
+  for (int i = 0; i + 1 < NB; ++i) {
+    const bool past = a == i && t >= NB - i;  // beyond row i (NB - i tiles)
+    t = past ? t - (NB - i) : t;
+    a += past ? 1 : 0;
+  }
+  b = a + t;
+}
+
+// list | scales (32 NB) | 1/sqrt(diag) (32 NB) | one staged tile per wave | K
+__host__ __device__ inline size_t cow_lds_bytes(int win, int nb, int waves) {
+  return (((size_t)win * 4 + 15) & ~(size_t)15) + (size_t)2 * 32 * nb * 4 + (size_t)waves * 32 * 32 * 8 + 16;
+}
+
+// One point on the block's W waves: the per-image scales over its K rows,
+// the upper tiles of S' accumulated on MFMA (tile t = wave + W s), 1/sqrt of
+// the diagonal, and the coherence row written to out (triu order; through the
+// wave's LDS tile for a pair list other than the canonical one).  Rows are
+// base[(SAMP ? k : list[k]) * n + image], k < K.  LDS: sc / rs 32 NB floats
+// each, stage W 32x32 complex tiles.  Every thread of the block calls it; it
+// leaves LDS reusable only after the caller's next barrier.  LOW: a wave loads
+// one tile's operands at a time instead of all its tiles' at once (fewer
+// registers live, for callers whose register budget is set by a later phase;
+// same arithmetic).  (Prefetching a chunk ahead was measured: it costs more
+// registers than it hides.)
+template <bool SAMP, int NB, int W, bool LOW = false>
+__device__ __forceinline__ void cow_point(const float2* __restrict__ base, const int* list, int K, int n,
+                                          const int32_t* __restrict__ pairs, int npairs, bool canon, float* sc,
+                                          float* rs, float2* stage, float2* __restrict__ out) {
+  const int wv = threadIdx.x >> 6;
+  int lane = threadIdx.x & 63;
+  __asm__ volatile("" : "+v"(lane));
+  const int hf = lane >> 5, c32 = lane & 31;
+  // ---- per-image power-of-two scale: wave w, image blocks w, w + W, ... -------
+  for (int ib = wv; ib < NB; ib += W) {
+    const int img = 32 * ib + c32;
+    float mx = 0.f;
+    if (img < n) {
+      // eight independent loads in flight per lane (the two half-waves take
+      // alternate SHPs); a short last trip re-reads SHP K-1, which leaves
+      // the maximum unchanged
+      for (int k0 = 0; k0 < K; k0 += 16) {
+        float2 z[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int k = k0 + 2 * u + hf < K ? k0 + 2 * u + hf : K - 1;
+          z[u] = base[(size_t)(SAMP ? k : list[k]) * n + img];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) mx = fmaxf(mx, fmaxf(fabsf(z[u].x), fabsf(z[u].y)));
+      }
+    }
+    mx = fmaxf(mx, __shfl_xor(mx, 32, 64));
+    float s = 1.f;
+    if (mx > 0.f && mx < INFINITY) {
+      int e;
+      frexpf(mx, &e);
+      s = ldexpf(1.f, 6 - e);  // scaled maximum in [32, 64)
+    }
+    if (hf == 0) sc[img] = s;
+  }
+  __syncthreads();
+  // ---- accumulate this wave's tiles ------------------------------------------
+  constexpr int NT = NB * (NB + 1) / 2;
+  constexpr int NS = (NT + W - 1) / W;  // tile slots per wave
+  int ta[NS], tb[NS];
+  bool on[NS];
+  float sa[NS], sb[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const int t = wv + W * s;
+    on[s] = t < NT;
+    cow_tile_ab<NB>(on[s] ? t : 0, ta[s], tb[s]);
+    sa[s] = sc[32 * ta[s] + c32];
+    sb[s] = sc[32 * tb[s] + c32];
+  }
+  f16x_t ar[NS] = {}, ai[NS] = {};
+  const int nch = (K + 7) >> 3;
+  // operand rows of tile slot s for chunk c
+  auto load1 = [&](int c, int s, float2(&xa)[4], float2(&xb)[4]) {
+    const int ra = 32 * ta[s] + c32, rb = 32 * tb[s] + c32;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int k = c * 8 + hf * 4 + q;
+      const bool okk = k < K;
+      const size_t off = okk ? (size_t)(SAMP ? k : list[k]) * n : 0;
+      xa[q] = (okk && ra < n) ? base[off + ra] : make_float2(0.f, 0.f);
+      xb[q] = (okk && rb < n) ? base[off + rb] : make_float2(0.f, 0.f);
+    }
+  };
+  float2 xa[NS][4], xb[NS][4];
+  for (int c = 0; c < nch; ++c) {
+    if (!LOW) {
+#pragma unroll
+      for (int s = 0; s < NS; ++s)
+        if (on[s]) load1(c, s, xa[s], xb[s]);
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      if (!on[s]) continue;
+      if (LOW) load1(c, s, xa[s], xb[s]);
+      float v[8];
+      half8_t x0h, x0l, y0h, y0l, x1h, x1l, y1h, y1l;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        v[2 * q] = xa[s][q].x * sa[s];
+        v[2 * q + 1] = xa[s][q].y * sa[s];
+      }
+      split4c(v, x0h, x0l, y0h, y0l);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        v[2 * q] = xb[s][q].x * sb[s];
+        v[2 * q + 1] = xb[s][q].y * sb[s];
+      }
+      split4c(v, x1h, x1l, y1h, y1l);
+      ar[s] = mfma3(x0h, x0l, x1h, x1l, ar[s]);
+      ai[s] = mfma3(y0h, y0l, x1h, x1l, ai[s]);
+      if (LOW) __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  // ---- 1/sqrt(S'_mm) from the diagonal tiles -----------------------------------
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    if (!on[s] || ta[s] != tb[s]) continue;
+    // select the diagonal entry this lane owns, then one reciprocal square root
+    bool own = false;
+    float d = 0.f;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const bool hit = 8 * (q >> 2) + 4 * hf + (q & 3) == c32;
+      own = own || hit;
+      d = hit ? ar[s][q] : d;
+    }
+    const int m = 32 * ta[s] + c32;
+    if (own) rs[m] = 1.f / sqrtf(m < n ? d : 0.f);
+  }
+  __syncthreads();
+  // ---- output ------------------------------------------------------------------
+  float2* tile = stage + wv * 1024;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    if (!on[s]) continue;
+    if (canon) {
+      co_write_tile_direct(ar[s], ai[s], ta[s], tb[s], n, lane, rs, out);
+    } else {
+      co_store_tile(tile, ar[s], ai[s], lane);
+      __builtin_amdgcn_wave_barrier();
+      for (int p = lane; p < npairs; p += 64) {
+        const int a = pairs[2 * p], b = pairs[2 * p + 1];
+        const int lo = min(a, b), hi = max(a, b);
+        if ((lo >> 5) != ta[s] || (hi >> 5) != tb[s]) continue;
+        float2 v = tile[(lo & 31) * 32 + (hi & 31)];
+        if (a > b) v.y = -v.y;
+        const float f = rs[a] * rs[b];
+        out[p] = make_float2(v.x * f, v.y * f);
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+}
 
 }  // namespace mr

@@ -80,10 +80,16 @@ class TempNet:
 
 
 def nimage_from_npair(npair: int) -> int:
-    """Number of images from the number of all pairs (raises ValueError otherwise)."""
-    guess = math.ceil(math.sqrt(2 * npair))
-    for d in (0, -1, 1, -2, 2):
-        guess = guess + d
-        if guess * (guess - 1) == 2 * npair:
-            return guess
-    raise ValueError("input is not correct value of number of all pairs")
+    """Number of images n with n(n-1)/2 == npair (raises ValueError otherwise).
+
+    n is the positive root of n^2 - n - 2*npair = 0, (1 + sqrt(1 + 8*npair)) / 2,
+    taken in exact integer arithmetic.  As in the reference, 0 pairs gives 0.
+    """
+    npair = int(npair)
+    if npair == 0:
+        return 0
+    root = math.isqrt(1 + 8 * npair) if npair > 0 else -1
+    n = (1 + root) // 2
+    if root < 0 or n * (n - 1) != 2 * npair:
+        raise ValueError("input is not correct value of number of all pairs")
+    return n

@@ -73,3 +73,24 @@ def test_sample_generators_shapes():
     assert s.numel() == int((ni.astype(np.int64) * ns).sum())
     np.testing.assert_array_equal(ro.numpy(), DSBatch.layout(ni, ns)[0])
     assert np.isfinite(s.numpy()).all()
+
+
+def test_fused_workspace_sizes():
+    """Workspace of the fused launches (a host-side query, no GPU needed): the
+    pixel lists for 32 < N <= 64, one coherence row per workgroup for
+    64 < N <= 200 (2048 workgroups by default, never more than the pixels),
+    nothing for N <= 32 and no fused launch above N = 200."""
+    from moraine_amd import _lib
+
+    def q(n_pix, n, blocks):
+        return _lib.query("mr_ds_batch_fused_workspace_bytes", n_pix, n, blocks)
+
+    assert q(1000, 17, 0) == 0
+    assert q(1000, 60, 0) == 256 + 2 * ((1000 * 4 + 255) // 256 * 256)
+    assert q(5000, 150, 0) == 2048 * (150 * 149 // 2) * 8
+    assert q(100, 150, 0) == 100 * (150 * 149 // 2) * 8
+    assert q(5000, 200, 7) == 7 * (200 * 199 // 2) * 8
+    assert q(5000, 201, 0) == 0
+    assert q(0, 150, 0) == 0
+    assert AutoTuner.fused_blocks(150) == AutoTuner.WIDE_BLOCKS
+    assert AutoTuner.fused_blocks(30) == AutoTuner.FUSED_BLOCKS

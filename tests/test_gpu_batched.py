@@ -110,16 +110,18 @@ def test_batched_launch_shapes_do_not_change_results():
     r = DSBatchRunner(b, tuner=tuner)
     out = r.run()
     torch.cuda.synchronize()
-    assert len(tuner.picks) == 4  # the N = 90 group also picks its coherence tile layout
-    assert tuner.picks[AutoTuner.key("coh_waves", 90, r.groups[-1].count)] in (2, 4, 8)
     for g in r.groups:
+        # every group (N = 30, 45, 90) was timed fused and staged
+        k = AutoTuner.key("fused_or_staged", g.n, g.count)
+        assert any(v < 0 for v in tuner.trials[k]) and any(v >= 0 for v in tuner.trials[k])
         if not g.staged:
-            assert g.blocks in AutoTuner.FUSED_BLOCKS and 32 < g.n <= 64
+            assert g.blocks in AutoTuner.fused_blocks(g.n)
         else:
             assert g.chunk in AutoTuner.CHUNKS or g.chunk == g.count
-    # the N = 45 group was timed fused and staged
-    k45 = [k for k in tuner.trials if k.startswith("fused_or_staged:n=45")]
-    assert k45 and any(v < 0 for v in tuner.trials[k45[0]]) and any(v >= 0 for v in tuner.trials[k45[0]])
+    # a staged N = 90 group also picks its coherence kernel's tile layout
+    g90 = r.groups[-1]
+    assert (AutoTuner.key("coh_waves", 90, g90.count) in tuner.picks) == g90.staged
+    assert len(tuner.picks) == 3 + int(g90.staged)
     assert torch.equal(out[0].view(torch.int64), ref[0].view(torch.int64))
     assert torch.equal(out[2].view(torch.int32), ref[2].view(torch.int32))
     assert torch.equal(out[3].view(torch.int32), ref[3].view(torch.int32))
@@ -142,6 +144,38 @@ def test_fused_equals_per_stage():
     assert torch.equal(a[0].view(torch.int64), c[0].view(torch.int64))
     assert torch.equal(a[2].view(torch.int32), c[2].view(torch.int32))
     assert torch.equal(a[3].view(torch.int32), c[3].view(torch.int32))
+
+
+def test_fused_small_and_wide_equal_per_stage():
+    """The fused launches for N <= 32 (one wave per pixel, coherence in an LDS
+    tile) and 64 < N <= 200 (one workgroup per pixel, coherence row in the
+    workgroup's slot) equal the per-stage sequence bit for bit, at every tile
+    grid and EMI register layout (N at each range's ends) and at grids small
+    enough that each workgroup takes many pixels (its slot re-used)."""
+    import torch
+
+    from moraine_amd.batched import DSBatch, DSBatchRunner
+
+    ns_ = (2, 17, 32, 65, 96, 97, 128, 129, 160, 161, 192, 193, 200)
+    s, ro, ni, ns = synth.ds_samples_torch(1300, n_choices=ns_, k_range=(20, 300), seed=21, device="cuda")
+    b = DSBatch.from_tensors(s, ro, ni, ns)
+    _case("ds_batch per-stage reference, N in " + ",".join(map(str, ns_)))
+    c = [x.clone() for x in DSBatchRunner(b, per_stage=True).run()]
+    r = DSBatchRunner(b)
+    r.tune()
+    assert not any(g.staged for g in r.groups)
+    for blocks in (0, 3):
+        _case(f"ds_batch fused N<=32 and 64<N<=200, blocks={blocks}")
+        r.ph.zero_()
+        r.quality.zero_()
+        r.t_coh.zero_()
+        for g in r.groups:
+            g.blocks = blocks
+        a = r.run()
+        torch.cuda.synchronize()
+        assert torch.equal(a[0].view(torch.int64), c[0].view(torch.int64)), blocks
+        assert torch.equal(a[2].view(torch.int32), c[2].view(torch.int32)), blocks
+        assert torch.equal(a[3].view(torch.int32), c[3].view(torch.int32)), blocks
 
 
 def _coh64(z):
@@ -220,7 +254,7 @@ def test_fused_resolve_equals_per_stage():
 
 def test_batched_coherence_tile_layouts_equal():
     """Every tile layout of the per-stage coherence kernel (waves per workgroup
-    over its 32 x 32 tile grid) gives the default layout's outputs bit for bit;
+    over its 32 x 32 tile grid) gives the fused launch's outputs bit for bit;
     a layout the grid does not have is rejected."""
     import torch
 
@@ -228,8 +262,9 @@ def test_batched_coherence_tile_layouts_equal():
 
     s, ro, ni, ns = synth.ds_samples_torch(900, n_choices=(70, 150, 200), k_range=(20, 240), seed=8, device="cuda")
     b = DSBatch.from_tensors(s, ro, ni, ns)
-    r = DSBatchRunner(b)
-    ref = [x.clone() for x in r.run()]
+    ref = [x.clone() for x in DSBatchRunner(b).run()]  # the fused launches
+    r = DSBatchRunner(b, per_stage=True)
+    r.tune()
     for g in r.groups:
         for v in AutoTuner.coh_waves(g.n):
             _case(f"ds_batch coherence layout N={g.n} waves={v}")

@@ -77,6 +77,16 @@ def test_tempnet_matches_printed_and_oracle():
         assert mr.nimage_from_npair(n * (n - 1) // 2) == n
     with pytest.raises(ValueError):
         mr.nimage_from_npair(7)
+    # every pair count up to n = 2000: triangular numbers map to their n, the rest raise
+    tri = {n * (n - 1) // 2: n for n in range(2, 2001)}
+    for npair in range(1, 1999 * 2000 // 2 + 1, 97):
+        if npair in tri:
+            assert mr.nimage_from_npair(npair) == tri[npair]
+        else:
+            with pytest.raises(ValueError):
+                mr.nimage_from_npair(npair)
+    assert all(mr.nimage_from_npair(p) == n for p, n in tri.items())
+    assert mr.nimage_from_npair(0) == 0  # the reference's guess loop accepts 0 images first
     with pytest.raises(ValueError):
         mr.TempNet(np.array([[0, 1], [0, 1], [2, 1]]))
     with pytest.raises(ValueError):
