@@ -33,13 +33,15 @@ __global__ __launch_bounds__(64 * COM_WAVES) void co_mfma_kernel(
   const float* rs = sc + 64;
 
   // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs, so
-  // block b runs on XCD b % 8.  Renumbering gives every XCD a contiguous range
-  // of points, whose overlapping SHP windows then share that XCD's L2.
-  const unsigned nbx = (gridDim.x + 7) / 8;
-  const unsigned bx = (blockIdx.x % 8) * nbx + blockIdx.x / 8;
+  // block b runs on XCD b % 8.  XCD x takes the x-th of eight contiguous ranges
+  // of points and its workgroups sweep that range in order, round after
+  // round, so overlapping SHP windows share that XCD's L2 (the grid is a
+  // multiple of 8).
+  const unsigned nbx = gridDim.x / 8, xcd = blockIdx.x % 8, jb = blockIdx.x / 8;
+  const long long per = (npts + 7) / 8;
+  const long long beg = (long long)xcd * per, end = beg + per < npts ? beg + per : npts;
   const bool canon = !RASTER && co_pairs_canonical(pairs, npairs, n);
-  if (bx >= gridDim.x) return;  // only when gridDim.x is not a multiple of 8
-  for (long long pt = (long long)bx * COM_WAVES + wv; pt < npts; pt += (long long)gridDim.x * COM_WAVES) {
+  for (long long pt = beg + (long long)jb * COM_WAVES + wv; pt < end; pt += (long long)nbx * COM_WAVES) {
     int ia, ir;
     const uint8_t* ms;
     if (RASTER) {
@@ -133,8 +135,9 @@ __global__ __launch_bounds__(64 * COM_WAVES) void co_mfma_kernel(
 // SAMP: point pt is pixel pix[pix0 + pt], its K = n_samp[id] rows are the
 // sample matrix at rslc + row_off[id] (batched form), no window.
 // XCD: workgroups are dispatched round-robin over the 8 XCDs (block b runs on
-// XCD b % 8); renumbered, every XCD takes a contiguous range of points, whose
-// overlapping windows then share that XCD's L2 (the grid is a multiple of 8).
+// XCD b % 8); XCD x takes the x-th of eight contiguous ranges of points and
+// sweeps it in order, so overlapping windows share that XCD's L2 (the grid is
+// a multiple of 8).
 template <bool SAMP, int NB, int W, bool XCD = false>
 __global__ __launch_bounds__(64 * W) void co_mfma_wide_kernel(
     const float2* __restrict__ rslc, int nl, int w, int n, const int32_t* __restrict__ idx,
@@ -152,8 +155,12 @@ __global__ __launch_bounds__(64 * W) void co_mfma_wide_kernel(
   const int wv = threadIdx.x >> 6;
   const int lane0 = threadIdx.x & 63;
   const bool canon = co_pairs_canonical(pairs, npairs, n);
-  const unsigned bx = XCD ? (blockIdx.x % 8) * (gridDim.x / 8) + blockIdx.x / 8 : blockIdx.x;
-  for (long long pt = bx; pt < npts; pt += gridDim.x) {
+  // XCD order: XCD x sweeps the x-th of eight contiguous ranges of points
+  const long long per = XCD ? (npts + 7) / 8 : npts;
+  const long long beg = XCD ? (long long)(blockIdx.x % 8) * per : 0;
+  const long long end = beg + per < npts ? beg + per : npts;
+  const unsigned jb = XCD ? blockIdx.x / 8 : blockIdx.x, nbx = XCD ? gridDim.x / 8 : gridDim.x;
+  for (long long pt = beg + jb; pt < end; pt += nbx) {
     int lane = lane0;
     __asm__ volatile("" : "+v"(lane));
     // ---- in-bounds SHPs in window order (wave 0) -----------------------------
@@ -200,6 +207,7 @@ __global__ __launch_bounds__(64 * W) void co_mfma_wide_kernel(
       hipLaunchKernelGGL((co_mfma_wide_kernel<SAMP, 7, 8>), dim3(nb), dim3(64 * 8),                             \
                          cow_lds_bytes(LDS_WIN, 7, 8), st, __VA_ARGS__);                                        \
   } while (0)
+
 
 int co_mfma_wide_launch(const float2* rslc, int nl, int w, int n, const int32_t* idx, const uint8_t* shp,
                         long long npts, int aw, int rw, const int32_t* pairs, int npairs, float2* coh, hipStream_t st) {
