@@ -489,10 +489,12 @@ __device__ __forceinline__ bool emi_fast_form_m(const Src& coh_at, int n, int la
     a.set(j, v);
     if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
   }
-  double dg = lane < n ? 1.0 : 0.0;  // own diagonal entry, tracked through the sweep
-  bool swept = lane >= n;
+  // Pivots in natural order: this pass only takes a PD |Gamma|, for which the
+  // symmetric elimination needs no pivoting, and the first non-positive pivot
+  // (in any order) shows that |Gamma| is not PD (Sylvester: the pivots of the
+  // natural order are ratios of leading principal minors).
   for (int s = 0; s < n; ++s) {
-    int k = 63 - (wmax_i(pivot_key<6>(fabs(dg), !swept, lane)) & 63);  // unswept row, largest |diagonal|
+    int k = s;
     __asm__ volatile("" : "+s"(k));  // an opaque index keeps the row in VGPRs (M0-relative read, no scratch copy)
     const double c = a.dyn(k);
     // Broadcasting d-1 in place of the pivot d lets the rank-1 update itself
@@ -524,8 +526,6 @@ __device__ __forceinline__ bool emi_fast_form_m(const Src& coh_at, int n, int la
         for (int t = 0; t < 8; ++t) cur[t] = nxt[t];
       }
     }
-    dg = fma(-f, c, dg);
-    swept = swept || (lane == k);
     __builtin_amdgcn_wave_barrier();
   }
   // a now holds -inv(A) row `lane` (diagonal entry offset by +2)
