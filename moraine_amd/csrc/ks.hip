@@ -293,6 +293,8 @@ __device__ __forceinline__ bool ks_tile_of(int ntx, int nl, long long chunk, int
   return true;
 }
 
+static_assert(MR_KS_MAX_N < (1 << 16), "the walk's |ib - ia| is one v_sad_u16 of indices below 2^16");
+
 template <typename T, int KS_ILP, typename KB>
 __global__ __launch_bounds__(KS_THREADS) void ks_pairs_kernel(const typename KeyOf<T>::type* __restrict__ keys,
                                                               const uint8_t* __restrict__ valid, int nl, int w,
@@ -370,8 +372,8 @@ __global__ __launch_bounds__(KS_THREADS) void ks_pairs_kernel(const typename Key
             const int ge = ka[q] >= kb_[q];
             ia[q] = min(ia[q] + le, n);
             ib[q] = min(ib[q] + ge, n);
-            const int d = ib[q] - ia[q];
-            km[q] = max(km[q], max(d, -d));
+            // |ib - ia| in one v_sad_u16 (both in [0, n], n < 2^16: the high halves are zero)
+            km[q] = max(km[q], (int)__builtin_amdgcn_sad_u16((unsigned)ib[q], (unsigned)ia[q], 0u));
             ka[q] = ref[ra[q] + ia[q]];
             kb_[q] = ref[rb[q] + ib[q]];
           }
