@@ -645,7 +645,7 @@ __device__ __forceinline__ bool emi_shift_invert(const Src& coh_at, int n, int l
                                                  double tau = 0.0, bool* flag = nullptr) {
   float2* colb = reinterpret_cast<float2*>(lds);        // 64: sweep column broadcast
   float2* xb = reinterpret_cast<float2*>(lds + 512);    // 64: iterate broadcast
-  // B = M - sigma I, and this lane's diagonal entry (real) for the pivot search
+  // B = M - sigma I, and this lane's diagonal entry (real) for the pivot order
   float dg = 0.f;
 #pragma unroll
   for (int j = 0; j < NM; ++j) {
@@ -654,17 +654,36 @@ __device__ __forceinline__ bool emi_shift_invert(const Src& coh_at, int n, int l
     br.set(j, b);
     if (dj) dg = b;
   }
-  // ---- X = inv(B): Hermitian sweep with largest-diagonal pivoting ---------
+  // ---- X = inv(B): Hermitian sweep --------------------------------------
   // Row i:  B_ij -= f_i conj(B_jk) (row k = conj(column k)), f_i = B_ik / d,
   // d = B_kk.  Broadcasting d - 1 in place of the pivot lets the update write
   // column k itself (B_ik - f_i (d - 1) = B_ik / d); the pivot lane uses
   // f = 1 - 1/d, so its row becomes B_kj / d and its diagonal 2 - 1/d (the
   // +2 is undone in the products below).  After all steps B holds -inv(B)
   // off the diagonal and 2 - inv(B)_ii on it.
-  bool swept = lane >= n;
+  // Static pivot order: the rows by decreasing diagonal of B (ties: smaller
+  // row first), from one bitonic sort across the wave; lane s then holds the
+  // row of the s-th pivot.  (B is PD here, so every diagonal entry is
+  // positive and its float bits order as the values do.)
+  unsigned kh = lane < n ? ~__float_as_uint(dg) : 0xffffffffu;  // ascending key: decreasing diagonal
+  int kl = lane;
+#pragma unroll
+  for (int size = 2; size <= 64; size <<= 1) {
+#pragma unroll
+    for (int j = size >> 1; j > 0; j >>= 1) {
+      const unsigned oh = (unsigned)__shfl_xor((int)kh, j, 64);
+      const int ol = __shfl_xor(kl, j, 64);
+      const bool other_less = oh < kh || (oh == kh && ol < kl);
+      const bool keep_min = ((lane & size) == 0) == ((lane & j) == 0);
+      if (keep_min == other_less) {
+        kh = oh;
+        kl = ol;
+      }
+    }
+  }
   for (int s = 0; s < n; ++s) {
-    int k = 63 - (wmax_i(pivot_key<6>(fabs((double)dg), !swept, lane)) & 63);
-    __asm__ volatile("" : "+s"(k));
+    int k = __builtin_amdgcn_readlane(kl, s);
+    __asm__ volatile("" : "+s"(k));  // an opaque index keeps the row in VGPRs (M0-relative read, no scratch copy)
     const float cr = br.dyn(k), ci = bi.dyn(k);  // B[lane][k]
     const float d = rl_f(cr, k);
     if (!(d > 0.f)) return false;  // sigma >= lambda_min (or B singular): Householder path
@@ -702,8 +721,6 @@ __device__ __forceinline__ bool emi_shift_invert(const Src& coh_at, int n, int l
         for (int t = 0; t < 4; ++t) cur[t] = nxt[t];
       }
     }
-    dg = fmaf(-fr, cr, fmaf(-fi, ci, dg));
-    swept = swept || (lane == k);
     __builtin_amdgcn_wave_barrier();
   }
   // ---- inverse iteration: y = X x = 2 x - (B x) ---------------------------
