@@ -353,9 +353,15 @@ int mr_ds_batch(const float* samples, const long long* row_off, const int32_t* n
     hipLaunchKernelGGL(mr::batch_zero_count_kernel, dim3(1), dim3(1), 0, st, count);
     if (int rc = mr::launch_status("ds_batch count init")) return rc;
     const int nb = (int)std::min<long long>(n_pix, blocks > 0 ? blocks : mr::BATCH_DEFAULT_BLOCKS);
-    hipLaunchKernelGGL((mr::ds_batch_fused_kernel<64, true, 1>), dim3(nb), dim3(64), mr::batch_fused_lds(true), st,
-                       smp, row_off, n_samples, ph_off, pix, n_pix, n, ref, pairs, n_pairs, phc, quality, t_coh, count,
-                       list, res_count, res_list);
+    // the shift-invert pass keeps 60 columns per lane up to n = 60 (as the plain EMI)
+    if (n <= 60)
+      hipLaunchKernelGGL((mr::ds_batch_fused_kernel<60, true, 1>), dim3(nb), dim3(64), mr::batch_fused_lds(true), st,
+                         smp, row_off, n_samples, ph_off, pix, n_pix, n, ref, pairs, n_pairs, phc, quality, t_coh,
+                         count, list, res_count, res_list);
+    else
+      hipLaunchKernelGGL((mr::ds_batch_fused_kernel<64, true, 1>), dim3(nb), dim3(64), mr::batch_fused_lds(true), st,
+                         smp, row_off, n_samples, ph_off, pix, n_pix, n, ref, pairs, n_pairs, phc, quality, t_coh,
+                         count, list, res_count, res_list);
     if (int rc = mr::launch_status("ds_batch shift-invert")) return rc;
     const int nb2 = (int)std::min<long long>(n_pix, mr::BATCH_FALLBACK_BLOCKS);
     hipLaunchKernelGGL((mr::ds_batch_fused_kernel<64, true, 2>), dim3(nb2), dim3(64), mr::batch_fused_lds(true), st,

@@ -214,16 +214,25 @@ static int emi_two_pass(const float2* c, long long n_pts, int n, int ref, float2
   const size_t lds = emi_fast_lds(n, 64);
   hipLaunchKernelGGL(emi_zero_count_kernel, dim3(1), dim3(1), 0, st, count < res_count ? count : res_count);
   if (int rc = launch_status("emi count init")) return rc;
-  if (xslots != nullptr) {
-    const int nb = (int)std::min<long long>(n_pts, EMI_SI_SLOTS);
-    hipLaunchKernelGGL((emi_fast_kernel<64, TC, true, 1>), dim3(nb), dim3(EMI_FAST_THREADS), lds, st, c, n_pts, n,
-                       ref, p, quality, pairs, t_coh, tau1, count, list, res_count, res_list, xslots,
-                       emi_slot_doubles(n));
-  } else {
-    const int nb = (int)std::min<long long>(n_pts, 1 << 20);
-    hipLaunchKernelGGL((emi_fast_kernel<64, TC, false, 1>), dim3(nb), dim3(EMI_FAST_THREADS), lds, st, c, n_pts, n,
-                       ref, p, quality, pairs, t_coh, 0.0, count, list, nullptr, nullptr, nullptr, 0);
+  // the shift-invert pass keeps NM columns per lane: 60 up to n = 60 (the
+  // padded columns would cost 1/16 of its FMAs), else 64
+#define MR_EMI_SI(NM)                                                                                               \
+  if (xslots != nullptr) {                                                                                          \
+    const int nb = (int)std::min<long long>(n_pts, EMI_SI_SLOTS);                                                   \
+    hipLaunchKernelGGL((emi_fast_kernel<NM, TC, true, 1>), dim3(nb), dim3(EMI_FAST_THREADS), lds, st, c, n_pts, n,  \
+                       ref, p, quality, pairs, t_coh, tau1, count, list, res_count, res_list, xslots,              \
+                       emi_slot_doubles(n));                                                                        \
+  } else {                                                                                                          \
+    const int nb = (int)std::min<long long>(n_pts, 1 << 20);                                                        \
+    hipLaunchKernelGGL((emi_fast_kernel<NM, TC, false, 1>), dim3(nb), dim3(EMI_FAST_THREADS), lds, st, c, n_pts, n, \
+                       ref, p, quality, pairs, t_coh, 0.0, count, list, nullptr, nullptr, nullptr, 0);             \
   }
+  if (n <= 60) {
+    MR_EMI_SI(60)
+  } else {
+    MR_EMI_SI(64)
+  }
+#undef MR_EMI_SI
   if (int rc = launch_status("emi_fast shift-invert")) return rc;
   const int nb2 = (int)std::min<long long>(n_pts, EMI_FALLBACK_BLOCKS);
   hipLaunchKernelGGL((emi_fast_kernel<64, TC, true, 2>), dim3(nb2), dim3(EMI_FAST_THREADS), lds, st, c, n_pts, n,

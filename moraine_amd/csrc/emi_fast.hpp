@@ -468,6 +468,7 @@ __device__ __forceinline__ void emi_fast_point(const Src& coh_at, int n, int lan
 // not positive, when |Gamma| is not positive definite: the sweep pivots are
 // those of a symmetric elimination with symmetric pivoting, so their signs are
 // |Gamma|'s inertia (Sylvester); all positive <=> PD.
+constexpr int EMI_SI_GD = 8;  // FP64 sweep: columns per software-pipelined LDS group
 template <int NM, class Src>
 __device__ __forceinline__ bool emi_fast_form_m(const Src& coh_at, int n, int lane, unsigned char* lds,
                                                 RegArr<float, NM>& mr_, RegArr<float, NM>& mi_,
@@ -510,20 +511,24 @@ __device__ __forceinline__ bool emi_fast_form_m(const Src& coh_at, int n, int la
     {
       // software-pipelined broadcast of colb: group g+1 is loaded before the
       // FMAs of group g, so LDS latency overlaps the FP64 FMAs
-      double cur[8], nxt[8];
+      // (groups of GD columns; a last partial group when NM is not a multiple)
+      constexpr int GD = EMI_SI_GD, NG = (NM + GD - 1) / GD;
+      double cur[GD], nxt[GD];
 #pragma unroll
-      for (int t = 0; t < 8; ++t) cur[t] = colb[t];
+      for (int t = 0; t < GD; ++t) cur[t] = colb[t];
 #pragma unroll
-      for (int g = 0; g < NM / 8; ++g) {
-        if (g + 1 < NM / 8) {
+      for (int g = 0; g < NG; ++g) {
+        if (g + 1 < NG) {
 #pragma unroll
-          for (int t = 0; t < 8; ++t) nxt[t] = colb[(g + 1) * 8 + t];
+          for (int t = 0; t < GD; ++t)
+            if ((g + 1) * GD + t < NM) nxt[t] = colb[(g + 1) * GD + t];
         }
 #pragma unroll
-        for (int t = 0; t < 8; ++t) a.set(g * 8 + t, fma(-f, cur[t], a.get(g * 8 + t)));
+        for (int t = 0; t < GD; ++t)
+          if (g * GD + t < NM) a.set(g * GD + t, fma(-f, cur[t], a.get(g * GD + t)));
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int t = 0; t < 8; ++t) cur[t] = nxt[t];
+        for (int t = 0; t < GD; ++t) cur[t] = nxt[t];
       }
     }
     __builtin_amdgcn_wave_barrier();

@@ -130,3 +130,30 @@ def test_co_pc_all_shp_largest_k_per_entry(n, az, r, k):
     print(f"  K={k}: per-entry max abs {ea:.2e}, max rel (|coh| >= 0.1) {er:.2e}")
     # per entry: 4e-6 absolute, 2e-5 relative where |coh| >= 0.1 (FP16 hi+lo products summed in FP32 over 2K slots)
     assert ea < 4e-6 and er < 2e-5
+
+
+@pytest.mark.parametrize("n,nl,w,first,last", [(60, 48, 37, 23, 5), (17, 40, 64, 0, 0), (90, 30, 41, 7, 11),
+                                               (33, 9, 300, 5, 0)])
+def test_co_pc_point_order_does_not_change_results(n, nl, w, first, last):
+    """The point-cloud kernels visit a run of consecutive raster pixels in
+    strips (a run of at least two raster rows per XCD range) and any other
+    point list in list order.  Each point's coherence is computed alone, so a
+    dense run starting and ending mid-row and the same points shuffled agree bit
+    for bit (the last case is shorter than two rows per range: list order)."""
+    mr = _mr()
+    z = synth.ds_stack(nl, w, n, seed=n + w, ps_frac=0.1)
+    rng = np.random.default_rng(w)
+    px = np.arange(first, nl * w - last)
+    idx = np.stack([px // w, px % w], -1).astype(np.int32)
+    shp = rng.random((px.size, 7, 9)) < 0.5
+    shp[:, 3, 4] = True
+    perm = rng.permutation(px.size)
+    _case(f"emperical_co_pc point order n={n} raster {nl}x{w} points={px.size}")
+    dense = np.asarray(mr.emperical_co_pc(z, idx, shp))
+    shuffled = np.asarray(mr.emperical_co_pc(z, idx[perm], shp[perm]))
+    np.testing.assert_array_equal(dense[perm].view(np.uint32), shuffled.view(np.uint32))
+    sub = rng.choice(px.size, 64, replace=False)
+    c64 = R.emperical_co_pc64(z, idx[sub], shp[sub])
+    ea, er = _coh_errors(dense[sub], c64)
+    # per entry: 4e-6 absolute, 2e-5 relative where |coh| >= 0.1
+    assert ea < 4e-6 and er < 2e-5

@@ -344,7 +344,8 @@ def test_uncompress_colliding_pairs_gpu(n):
 
 
 # ---------------------------------------------------------------------------- EMI
-@pytest.mark.parametrize("n", [2, 3, 17, 30, 32, 33, 60, 64, 65, 90, 96, 97, 120, 128, 129, 150, 160, 161, 200, 201])
+@pytest.mark.parametrize("n", [2, 3, 17, 30, 32, 33, 59, 60, 61, 64, 65, 90, 96, 97, 120, 128, 129, 150, 160, 161, 200,
+                               201])
 def test_emi_closure_free(n):
     mr = _mr()
     coh, exp = synth.closure_free_coh(24, n, seed=n)
@@ -357,7 +358,8 @@ def test_emi_closure_free(n):
     np.testing.assert_allclose(np.abs(ph), 1.0, atol=1e-5)
 
 
-@pytest.mark.parametrize("n,win", [(17, (5, 7)), (30, (5, 7)), (60, (5, 7)), (90, (5, 7)), (90, (11, 15)),
+@pytest.mark.parametrize("n,win", [(17, (5, 7)), (30, (5, 7)), (57, (5, 7)), (60, (5, 7)), (61, (5, 7)),
+                                   (64, (5, 7)), (90, (5, 7)), (90, (11, 15)),
                                    (120, (11, 15)), (150, (11, 15)), (160, (11, 15)),
                                    (200, (11, 15))])
 def test_emi_ds_vs_fp64(n, win):
