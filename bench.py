@@ -7,8 +7,10 @@ pixel taken through the whole path.  One step = one pass over the tile:
 
     rmli = |rslc|^2
     p, is_shp, shp_num = ks_test + select_shp              (p materialised, as the drop-in returns it)
-    per batch of pixels: coh = emperical_co_pc -> (ph, quality, t) = emi then ds_temp_coh
-                         (the fused driver's inner step: one kernel for 32 < N <= 64)
+    per batch of pixels: (ph, quality, t) = emperical_co_pc -> emi -> ds_temp_coh, the batch
+                         driver's step (for 32 < N <= 64 one launch computes each point's
+                         coherence into LDS and runs EMI and the temporal coherence from there;
+                         batches alternate over two streams)
 
 Inputs are generated on the device before timing (synthetic, seeded).  Prints
 one JSON line.  Algorithmic units per pixel: KS 4N + 5W + 4 bytes, coherence 4K N(N+1)
@@ -51,6 +53,9 @@ def parse():
     ap.add_argument("--r", type=int, default=10)
     ap.add_argument("--ps", type=float, default=0.10)
     ap.add_argument("--batch", type=int, default=1 << 18, help="pixels per coherence/EMI batch")
+    ap.add_argument("--batch-streams", type=int, default=2, choices=(1, 2),
+                    help="streams the plain pass's batches alternate over (1: one after the other, for "
+                         "per-kernel profiles without overlap)")
     ap.add_argument("--cpu-sample", type=int, default=256, help="CPU baseline: interior sub-tile edge")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--err-sample", type=int, default=2048)
@@ -264,14 +269,13 @@ def _n60_window_batch(args, torch):
     """The batched form on the plain workload's own pixels: N = 60, SHP sets from the KS test over
     15x21 windows of a 384x384 DS stack; each pixel's sample matrix is its in-bounds SHP stacks in
     window order, gathered on the device.  Fused kernel (autotuned, HIP-graph replay) against the
-    plain drop-in on the same pixels (emperical_co_pc on the raster, then EMI + DS temporal
-    coherence): both compute the same outputs (bit for bit, tests/test_gpu_batched.py)."""
+    plain drop-in on the same pixels (emperical_co_emi_temp_coh_pc on the raster: coherence, EMI and
+    DS temporal coherence): both compute the same outputs (bit for bit, tests/test_gpu_batched.py)."""
     import numpy as np
 
     import moraine_amd as mr
     from moraine_amd import synth
     from moraine_amd.batched import AutoTuner, DSBatch, DSBatchRunner
-    from moraine_amd.pl import emi_ds_temp_coh
 
     S, N, az, r = 384, 60, 7, 10
     aw, rw = 2 * az + 1, 2 * r + 1
@@ -308,9 +312,7 @@ def _n60_window_batch(args, torch):
         return min(ts)
 
     def plain():
-        for s0 in range(0, npx, 1 << 16):
-            c = mr.emperical_co_pc(z, idx[s0:s0 + (1 << 16)], pcs[s0:s0 + (1 << 16)])
-            emi_ds_temp_coh(c)
+        mr.emperical_co_emi_temp_coh_pc(z, idx, pcs)
 
     runner = DSBatchRunner(b, tuner=AutoTuner())
     runner.capture()
@@ -578,14 +580,13 @@ def main():
 
     import moraine_amd as mr
     from moraine_amd import synth
-    from moraine_amd.pl import _emi_tcoh_dev as emi_tcoh
+    from moraine_amd.pl import co_emi_tcoh_batches
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     S, N, az, r = args.size, args.n, args.az, args.r
     aw, rw = 2 * az + 1, 2 * r + 1
     npx = S * S
-    P = N * (N - 1) // 2
 
     rslc = synth.ds_stack_torch(S, S, N, seed=0, ps_frac=args.ps, device=dev)
     ii, jj = torch.meshgrid(torch.arange(S, device=dev, dtype=torch.int32),
@@ -622,14 +623,13 @@ def main():
         p, is_shp, shp_num = mr.ks_test_select_shp(rmli, az, r, 0.05, return_p=True)
         mark("ks")
         pcs = is_shp.view(npx, aw, rw)
-        for s, e in batches:
-            mark("co0")
-            coh = mr.emperical_co_pc(rslc, idx[s:e], pcs[s:e])
-            mark("co1")
-            # emperical_co_emi_temp_coh_pc's inner step, writing straight into ph, q, tc
-            emi_tcoh(coh, N, 0, ph[s:e], q[s:e], tc[s:e])
-            mark("emi1")
-        state["p"], state["shp_num"], state["coh_last"] = p, shp_num, coh
+        mark("co0")
+        # emperical_co_emi_temp_coh_pc's batches (coherence, EMI, temporal coherence; one launch
+        # per batch for 32 < N <= 64, batches alternating over two streams so one batch's
+        # Householder pass and re-solve overlap the next launch), writing straight into ph, q, tc
+        co_emi_tcoh_batches(rslc, idx, pcs, 0, ph, q, tc, args.batch, args.batch_streams)
+        mark("emi1")
+        state["p"], state["shp_num"], state["pcs"] = p, shp_num, pcs
         if mr.pl.last_householder is not None:
             state["householder_last"] = mr.pl.last_householder
             state["resolved_last"] = mr.pl.last_resolved
@@ -657,8 +657,7 @@ def main():
 
     t_rmli = span("start", "rmli")
     t_ks = span("rmli", "ks")
-    t_co = span("co0", "co1")
-    t_emi = span("co1", "emi1")
+    t_cet = span("co0", "emi1")
     nb = len(batches)
 
     shp_num = state["shp_num"].float()
@@ -666,46 +665,50 @@ def main():
     # algorithmic units per pixel: bytes for ks/tcoh, flops for coherence/EMI
     ks_bytes = 4 * N + 4 * aw * rw + aw * rw + 4  # rmli in, p out, mask out, count out
     co_flops = 4.0 * kmean * N * (N + 1)  # Hermitian rank-K update (unique entries)
-    co_bytes = 8 * P
     emi_flops = emi_algorithmic_flops(N)
-    tc_bytes = 8 * P + 8 * N + 4
     stages = {
         "rmli": {"s": t_rmli},
         "ks_test+select_shp": {"s": t_ks, "GB/s": ks_bytes * npx / t_ks / 1e9,
                                "hbm_frac": ks_bytes * npx / t_ks / 1e9 / HBM_PEAK_GBS},
-        "emperical_co_pc": {"s": t_co, "TFLOP/s": co_flops * npx / t_co / 1e12,
-                            "coh_write_GB/s": co_bytes * npx / t_co / 1e9},
-        "emi+ds_temp_coh": {"s": t_emi, "TFLOP/s": emi_flops * npx / t_emi / 1e12,
-                            "coh_read_GB/s": tc_bytes * npx / t_emi / 1e9},
+        "emperical_co_pc+emi+ds_temp_coh": {
+            "s": t_cet, "emi_TFLOP/s": emi_flops * npx / t_cet / 1e12, "coh_TFLOP/s": co_flops * npx / t_cet / 1e12,
+            "note": "one launch per batch computes each point's coherence into LDS, the shift-invert EMI "
+                    "pass and the temporal coherence; the Householder pass and FP64 re-solve follow"},
     }
-    dom = max(["ks_test+select_shp", "emperical_co_pc", "emi+ds_temp_coh"], key=lambda k: stages[k]["s"])
-    if dom == "emi+ds_temp_coh":
-        ach = emi_flops * (npx / nb) / (t_emi / nb) / 1e12
-        ek = ["emi_fast_kernel<64, true, false, 1>", "emi_fast_kernel<64, true, true, 2>",
+    dom = max(["ks_test+select_shp", "emperical_co_pc+emi+ds_temp_coh"], key=lambda k: stages[k]["s"])
+    if dom == "emperical_co_pc+emi+ds_temp_coh":
+        ach = emi_flops * (npx / nb) / (t_cet / nb) / 1e12
+        # the fused launch keeps 60 EMI columns per lane up to N = 60, else 64
+        ek = [f"co_emi_pc_kernel<{60 if N <= 60 else 64}>", "emi_fast_kernel<64, true, true, 2>",
               "emi_general_kernel<true, float2 >"]
-        roof = {"kernel": " + ".join(ek) if 32 < N <= 64 else "emi_wide_kernel", "bound": "valu", "achieved": ach,
+        # the same launches also do the coherence's 4KN(N+1) flops (MFMA work): the share of the
+        # stage time an ideal machine would need for both at their peaks
+        ideal = (emi_flops / (FP64_PEAK_TFLOPS * 1e12) + co_flops / 2.5e15) * npx
+        roof = {"kernel": " + ".join(ek) if 32 < N <= 64 else "co + emi kernels", "bound": "valu", "achieved": ach,
                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
+                "frac_with_coherence": ideal / t_cet,
                 "traffic": measured_traffic(ek, args),
-                "note": "EMI with the DS temporal coherence fused, VALU-bound (no MFMA): the shift-invert pass over "
-                        "every point, the Householder pass over the points it leaves and the FP64 re-solve of those "
-                        "the Householder pass flags, timed together with HIP events per batch; achieved = algorithmic flops N^3 + (4/3)N^3 + 16N^2 per point; the "
-                        "passes execute more (an FP64 sweep, 2N^3, and a complex FP32 sweep, 8N^3, in place of "
-                        "Cholesky factors); peak = MI355X FP64 = FP32 non-packed vector rate"}
+                "note": "coherence, EMI and the DS temporal coherence: the fused launch (coherence on MFMA into LDS, "
+                        "the shift-invert EMI pass, temporal coherence) over every point, the Householder pass over "
+                        "the points it leaves and the FP64 re-solve of those the Householder pass flags, timed "
+                        "together with HIP events around the batch loop (batches alternate over two streams, "
+                        "so a batch's tail overlaps the next launch; per batch = stage time / batches); "
+                        "VALU-bound (EMI); achieved = EMI algorithmic flops "
+                        "N^3 + (4/3)N^3 + 16N^2 per point only (the coherence's MFMA flops are in "
+                        "frac_with_coherence); the passes execute more (an FP64 sweep, 2N^3, and a complex FP32 "
+                        "sweep, 8N^3, in place of Cholesky factors); peak = MI355X FP64 = FP32 non-packed vector rate"}
     elif dom == "ks_test+select_shp":
         ach = ks_bytes * npx / t_ks / 1e9
         ks = ["ks_sort_kernel<float, 1, false>", "ks_pairs_kernel<float, 4, unsigned char>",
               "ks_emit_kernel<float, unsigned char>"]
         roof = {"kernel": " + ".join(ks), "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic(ks, args)}
-    else:
-        ach = co_flops * (npx / nb) / (t_co / nb) / 1e12
-        roof = {"kernel": "co_mfma_kernel<false, true>", "bound": "mfma", "achieved": ach, "peak": 2500.0,
-                "unit": "TFLOP/s", "frac": ach / 2500.0,
-                "traffic": measured_traffic(["co_mfma_kernel<false, true>"], args)}
 
-    # phase error of the last batch vs an FP64 solve (torch/rocSOLVER, complex128) on a sample
-    coh = state["coh_last"]
-    s0 = batches[-1][0]
+    # phase error of the last batch vs an FP64 solve (torch/rocSOLVER, complex128) on a sample;
+    # the last batch's coherence rows (the fused launch keeps them on chip) computed again here,
+    # outside the timing
+    s0, s1 = batches[-1]
+    coh = mr.emperical_co_pc(rslc, idx[s0:s1], state["pcs"][s0:s1])
     m = min(args.err_sample, coh.shape[0])
     G = mr.uncompress_coh(coh[:m]).to(torch.complex128)
     A = G.abs()
@@ -798,7 +801,7 @@ def main():
         "dtype": "f32 (KS: int keys; EMI: f64 inverse + f32 eigen)", "data": "synthetic (seeded DS/PS model)",
         "config": {"workload": f"plain drop-in pipeline, N={N}, {S}x{S} tile, {aw}x{rw} window, "
                                f"{int(args.ps*100)}% PS, p_max=0.05, all pixels", "pixels": npx,
-                   "n_images": N, "window": [aw, rw], "batch": args.batch},
+                   "n_images": N, "window": [aw, rw], "batch": args.batch, "batch_streams": args.batch_streams},
         "latency_ms": {"p50": float(np.percentile(times, 50) * 1e3), "p99": float(np.percentile(times, 99) * 1e3),
                        "of": "plain-form tile steps; request latency of the serving form is under forms.serving"},
         "phase_err_vs_fp64": phase,

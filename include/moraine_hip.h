@@ -257,6 +257,25 @@ int mr_emi_ds_temp_coh(const float* coh, long long n_pts, int n, int ref, const 
                        float* ph, float* quality, float* t_coh, void* workspace, size_t workspace_bytes,
                        mr_stream_t stream);
 
+/* One batch of moraine.emperical_co_emi_temp_coh_pc (moraine/pl.py:148-168):
+ * emperical_co_pc on the full pair network (moraine/co.py:230-266), emi
+ * (moraine/pl.py:68-77) and ds_temp_coh (moraine/pl.py:126-145) for n_pc
+ * points.  Arguments as mr_emperical_co_pc (complex64 rslc (nl, w, n), idx
+ * (n_pc, 2) int32, pc_is_shp (n_pc, aw, rw) bytes) and mr_emi_ds_temp_coh
+ * (ph (n_pc, n) complex64, quality and t_coh (n_pc,) float32).  Results equal
+ * mr_emperical_co_pc -> mr_emi_ds_temp_coh bit for bit.  For 32 < n <= 64 one
+ * kernel computes each point's coherence into on-chip memory and runs the
+ * shift-invert EMI pass and the temporal coherence from there; only the rows
+ * of the points that pass leaves to the Householder pass are written to the
+ * workspace.  Other n run the two calls through the workspace.
+ * workspace >= mr_co_emi_ds_temp_coh_pc_workspace_bytes(n_pc, n): the EMI
+ * lists (as mr_emi_workspace_bytes), the pair network, and one coherence row
+ * per point. */
+size_t mr_co_emi_ds_temp_coh_pc_workspace_bytes(long long n_pc, int n);
+int mr_co_emi_ds_temp_coh_pc(const float* rslc, int nl, int w, int n, const int32_t* idx, const uint8_t* pc_is_shp,
+                             long long n_pc, int aw, int rw, int ref, float* ph, float* quality, float* t_coh,
+                             void* workspace, size_t workspace_bytes, mr_stream_t stream);
+
 /* Precision form of emi / emi + ds_temp_coh for 2 <= n <= 200 (above, mr_emi
  * is FP64 throughout), no reference counterpart (moraine/pl.py:18-77 computes
  * in the array library's precision): the same solve as mr_emi with the checks

@@ -37,6 +37,8 @@ SIGNATURES = {
     "mr_regularize_spectral": (_i, [_vp, _ll, _i, _vp, _i, _vp, _vp]),
     "mr_ds_temp_coh": (_i, [_vp, _vp, _ll, _i, _vp, _i, _vp, _vp]),
     "mr_emi_ds_temp_coh": (_i, [_vp, _ll, _i, _i, _vp, _i, _vp, _vp, _vp, _vp, _sz, _vp]),
+    "mr_co_emi_ds_temp_coh_pc_workspace_bytes": (_sz, [_ll, _i]),
+    "mr_co_emi_ds_temp_coh_pc": (_i, [_vp, _i, _i, _i, _vp, _vp, _ll, _i, _i, _i, _vp, _vp, _vp, _vp, _sz, _vp]),
     "mr_temp_coh": (_i, [_vp, _vp, _ll, _i, _vp, _i, _vp, _vp]),
     "mr_emi_refined_workspace_bytes": (_sz, [_ll, _i]),
     "mr_emi_refined": (_i, [_vp, _ll, _i, _i, _d, _vp, _i, _vp, _vp, _vp, _vp, _sz, _vp]),

@@ -16,52 +16,6 @@ __host__ __device__ inline size_t com_wave_bytes(int win, bool) {
   return (list > tile ? list : tile) + 2 * 64 * 4;
 }
 
-// Point order inside an XCD's range (point-cloud form).  When the range is a
-// run of consecutive raster pixels (its first and last points `len` - 1 pixels
-// apart in raster order: the all-pixel batches of the batch driver), slot s
-// walks strips of at most CO_STRIP raster rows column by column, so the points
-// in flight on the XCD form a compact patch (about 16 x 16 pixels) whose SHP
-// windows share its L2; otherwise slot s is point s.  Either way every point
-// gets exactly one slot (a bijection on [0, len) plus empty slots), so the
-// output does not depend on it.
-constexpr int CO_STRIP = 16;
-struct CoStrips {
-  int w = 0, c0 = 0, h = 0;  // h = 0: identity order
-  long long strip = 0;       // h * w slots per strip
-  int nstrips = 0;
-  __device__ __forceinline__ long long slots(long long len) const { return h ? strip * nstrips : len; }
-  // point offset of slot s, -1 for an empty slot
-  __device__ __forceinline__ long long point(long long s, long long len) const {
-    if (!h) return s;
-    const long long k = s / strip, rem = s - k * strip;
-    const long long col = rem / h, lr = k * h + (rem - col * h);
-    const long long u = lr * w + col - c0;
-    return u >= 0 && u < len ? u : -1;
-  }
-};
-// Uniform over the block: from the first and last points of the range.
-__device__ __forceinline__ CoStrips co_strips(const int32_t* __restrict__ idx, long long beg, long long end, int w) {
-  CoStrips st;
-  if (end - beg < 2 * (long long)w) return st;  // shorter than two rows: identity order
-  const int r0 = idx[2 * beg], c0 = idx[2 * beg + 1];
-  const int r1 = idx[2 * (end - 1)], c1 = idx[2 * (end - 1) + 1];
-  const long long len = end - beg;
-  if ((long long)(r1 - r0) * w + (c1 - c0) + 1 != len || c0 < 0 || c0 >= w) return st;
-  const int rows = (int)((c0 + len + w - 1) / w);  // raster rows the run touches
-  st.nstrips = (rows + CO_STRIP - 1) / CO_STRIP;
-  st.h = (rows + st.nstrips - 1) / st.nstrips;
-  st.w = w;
-  st.c0 = c0;
-  st.strip = (long long)st.h * w;
-  return st;
-}
-// Slots per XCD range of `per` points on a raster of width w, at most.
-__host__ __device__ inline long long co_max_slots(long long per, int w) {
-  const long long rows = (w - 1 + per + w - 1) / w;
-  const long long ns = (rows + CO_STRIP - 1) / CO_STRIP;
-  return std::max(per, ((rows + ns - 1) / ns) * ns * (long long)w);
-}
-
 template <bool RASTER, bool TWO>
 __global__ __launch_bounds__(64 * COM_WAVES) void co_mfma_kernel(
     const float2* __restrict__ rslc, int nl, int w, int n, const int32_t* __restrict__ idx,

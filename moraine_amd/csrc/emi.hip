@@ -26,6 +26,7 @@
 // two lanes per row (emi_wide.hip; a single wave for n <= 32, where one lane
 // per row would leave half the wave idle).  General path (n > 200): one
 // 256-thread block per point, everything FP64 in a per-block global workspace.
+#include "co_mfma.hpp"
 #include "emi_fast.hpp"
 #include "emi_general.hpp"
 #include "tcoh.hpp"
@@ -208,6 +209,10 @@ inline size_t emi_slots_bytes(long long npts, int n) {
   return (size_t)std::min<long long>(npts > 0 ? npts : 1, EMI_SI_SLOTS) * emi_slot_doubles(n) * 8;
 }
 template <bool TC>
+static int emi_hh_passes(const float2* c, long long n_pts, int n, int ref, float2* p, float* quality,
+                         const int32_t* pairs, float* t_coh, unsigned* count, int* list, unsigned* res_count,
+                         int* res_list, double tau2, hipStream_t st);
+template <bool TC>
 static int emi_two_pass(const float2* c, long long n_pts, int n, int ref, float2* p, float* quality,
                         const int32_t* pairs, float* t_coh, unsigned* count, int* list, unsigned* res_count,
                         int* res_list, double* xslots, double tau1, double tau2, hipStream_t st) {
@@ -234,6 +239,17 @@ static int emi_two_pass(const float2* c, long long n_pts, int n, int ref, float2
   }
 #undef MR_EMI_SI
   if (int rc = launch_status("emi_fast shift-invert")) return rc;
+  return emi_hh_passes<TC>(c, n_pts, n, ref, p, quality, pairs, t_coh, count, list, res_count, res_list, tau2, st);
+}
+
+// Passes 2 and 3 of the two-pass path: the Householder pass over the points
+// the shift-invert pass listed (list[0 .. *count)), then the FP64 re-solve of
+// those its conditioning test (tau2) flags (res_list[0 .. *res_count)).
+template <bool TC>
+static int emi_hh_passes(const float2* c, long long n_pts, int n, int ref, float2* p, float* quality,
+                         const int32_t* pairs, float* t_coh, unsigned* count, int* list, unsigned* res_count,
+                         int* res_list, double tau2, hipStream_t st) {
+  const size_t lds = emi_fast_lds(n, 64);
   const int nb2 = (int)std::min<long long>(n_pts, EMI_FALLBACK_BLOCKS);
   hipLaunchKernelGGL((emi_fast_kernel<64, TC, true, 2>), dim3(nb2), dim3(EMI_FAST_THREADS), lds, st, c, n_pts, n,
                      ref, p, quality, pairs, t_coh, tau2, count, list, res_count, res_list, nullptr, 0);
@@ -254,6 +270,104 @@ static int emi_two_pass_plain(const float2* c, long long n_pts, int n, int ref, 
   return emi_two_pass<TC>(c, n_pts, n, ref, p, quality, pairs, t_coh, count, reinterpret_cast<int*>(ws + 256),
                           count + 1, reinterpret_cast<int*>(ws + 256 + emi_list_span(n_pts)), nullptr, 0.0, EMI_HH_TOL,
                           st);
+}
+
+// ----------------------------------------------------------------------------
+// The batch driver's step in one launch (32 < n <= 64; moraine/pl.py:148-168:
+// emperical_co_pc -> emi -> ds_temp_coh on the full pair network): one wave
+// per point computes the point's coherence on MFMA (the point-cloud coherence
+// kernel's body, co_mfma.hpp) straight into an LDS row in triu order -- the
+// values that kernel stores in HBM -- then runs the shift-invert EMI pass and
+// the DS temporal coherence on that row, as emi_fast_kernel<NM, true, false, 1>
+// does after staging the row from HBM.  Points the pass declines get their row
+// written to coh (the only coherence that reaches HBM) and go to the
+// Householder pass and FP64 re-solve (emi_hh_passes), which read it there.
+// The outputs equal emperical_co_pc -> emi -> ds_temp_coh bit for bit.  The
+// waves of a CU interleave the coherence's gathers with other points' EMI
+// arithmetic.  Point order: the coherence kernel's XCD ranges and strips, so
+// neighbouring windows share an XCD's L2.  LDS per wave: the coherence row
+// (which holds the SHP list while the coherence accumulates), scales, EMI
+// scratch.
+// ----------------------------------------------------------------------------
+__host__ __device__ inline size_t coe_row_bytes(int n, int win) {
+  const size_t row = (size_t)(n * (n - 1) / 2) * 8, lst = (size_t)win * 4;
+  return ((row > lst ? row : lst) + 15) & ~(size_t)15;
+}
+inline size_t coe_lds(int n, int win) { return coe_row_bytes(n, win) + 128 * 4 + EMI_FAST_LDS_BYTES; }
+
+template <int NM>
+__global__ __launch_bounds__(EMI_FAST_THREADS) void co_emi_pc_kernel(
+    const float2* __restrict__ rslc, int nl, int w, int n, const int32_t* __restrict__ idx,
+    const uint8_t* __restrict__ shp, long long npts, int aw, int rw, int ref, const int32_t* __restrict__ pairs,
+    float2* __restrict__ coh, float2* __restrict__ ph, float* __restrict__ quality, float* __restrict__ t_coh,
+    unsigned* __restrict__ count, int* __restrict__ list) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int P = n * (n - 1) / 2;
+  const int win = aw * rw, az = (aw - 1) / 2, r = (rw - 1) / 2;
+  float2* cs = reinterpret_cast<float2*>(smem);  // coherence row (the SHP list before it)
+  int* shl = reinterpret_cast<int*>(smem);
+  float* sc = reinterpret_cast<float*>(smem + coe_row_bytes(n, win));  // scales, 1/sqrt(diag)
+  unsigned char* work = reinterpret_cast<unsigned char*>(sc + 128);     // EMI_FAST_LDS_BYTES
+  // XCD x (block b runs on XCD b % 8) takes the x-th of eight contiguous
+  // ranges of points, in the coherence kernel's strip order (grid: a multiple of 8)
+  const unsigned nbx = gridDim.x / 8, xcd = blockIdx.x % 8, jb = blockIdx.x / 8;
+  const long long per = (npts + 7) / 8;
+  const long long beg = (long long)xcd * per, end = beg + per < npts ? beg + per : npts;
+  const CoStrips st = co_strips(idx, beg, end, w);
+  for (long long sl = jb; sl < st.slots(end - beg); sl += nbx) {
+    const long long u = st.point(sl, end - beg);
+    if (u < 0) continue;  // a slot of the strip grid past the run
+    const long long pt = beg + u;
+    int lane = threadIdx.x;
+    __asm__ volatile("" : "+v"(lane));  // per-point opaque lane (see emi_fast_kernel)
+    const int ia = idx[2 * pt], ir = idx[2 * pt + 1];
+    const int K = co_compact_shp(shp + pt * win, win, rw, ia, ir, az, r, nl, w, lane, shl);
+    __builtin_amdgcn_wave_barrier();
+    const CoAcc acc = co_mfma_accumulate<true>(rslc, RowList{shl}, K, n, lane);
+    __builtin_amdgcn_wave_barrier();  // the row below overwrites the list
+    co_store_scales<true>(acc, n, lane, sc);
+    {
+      int ln = lane;
+      __asm__ volatile("" : "+v"(ln));
+      co_write_tile_direct(acc.r00, acc.i00, 0, 0, n, ln, sc + 64, cs);
+      co_write_tile_direct(acc.r01, acc.i01, 0, 1, n, ln, sc + 64, cs);
+      co_write_tile_direct(acc.r11, acc.i11, 1, 1, n, ln, sc + 64, cs);
+    }
+    __builtin_amdgcn_wave_barrier();
+    float vr, vi;
+    double rq;
+    if (!emi_inv_point<NM>(CohPacked{cs, n}, n, lane, work, vr, vi, rq)) {
+      // declined: the row goes to HBM for the Householder pass
+      float2* dst = coh + pt * (long long)P;
+      for (int t = lane; t < P; t += 64) dst[t] = cs[t];
+      if (lane == 0) list[atomicAdd(count, 1u)] = (int)pt;
+      __builtin_amdgcn_wave_barrier();
+      continue;
+    }
+    const float2 pv = emi_link_phase(vr, vi, ref);
+    if (lane < n) ph[pt * n + lane] = pv;
+    if (lane == 0) quality[pt] = (float)rq;
+    float2* phb = reinterpret_cast<float2*>(work);  // the EMI scratch is free again
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    phb[lane] = pv;
+    __builtin_amdgcn_wave_barrier();
+    const float tc = tcoh_point<true>([&](int q, int, int) { return cs[q]; }, [&](int i) { return phb[i]; }, pairs, P,
+                                      lane);
+    if (lane == 0) t_coh[pt] = tc;
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// the full pair network of n images in triu order (moraine/tnet.py:15-20)
+__global__ void full_pairs_kernel(int n, int32_t* __restrict__ pairs) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n) return;
+  int p = a * (2 * n - a - 1) / 2;
+  for (int b = a + 1; b < n; ++b, ++p) {
+    pairs[2 * p] = a;
+    pairs[2 * p + 1] = b;
+  }
 }
 
 }  // namespace mr
@@ -339,6 +453,67 @@ int mr_emi_ds_temp_coh(const float* coh, long long n_pts, int n, int ref, const 
   const int rc = mr_emi(coh, n_pts, n, ref, ph, quality, workspace, workspace_bytes, stream);
   if (rc != MR_OK) return rc;
   return mr_ds_temp_coh(coh, ph, n_pts, n, pairs, n_pairs, t_coh, stream);
+}
+
+static size_t coe_round(size_t b) { return (b + 255) & ~(size_t)255; }
+
+size_t mr_co_emi_ds_temp_coh_pc_workspace_bytes(long long n_pc, int n) {
+  if (n < 2 || n_pc < 0) return 0;
+  const size_t P = (size_t)n * (n - 1) / 2;
+  return coe_round(mr_emi_workspace_bytes(n_pc, n)) + coe_round(P * 8) + (size_t)n_pc * P * 8;
+}
+
+int mr_co_emi_ds_temp_coh_pc(const float* rslc, int nl, int w, int n, const int32_t* idx, const uint8_t* pc_is_shp,
+                             long long n_pc, int aw, int rw, int ref, float* ph, float* quality, float* t_coh,
+                             void* workspace, size_t workspace_bytes, mr_stream_t stream) {
+  MR_CHECK_ARG(n >= 2, "co_emi_ds_temp_coh_pc: need at least 2 images (n=%d)", n);
+  MR_CHECK_ARG(ref >= 0 && ref < n, "co_emi_ds_temp_coh_pc: ref=%d outside [0,%d)", ref, n);
+  MR_CHECK_ARG(n_pc >= 0 && n_pc < (1ll << 31), "co_emi_ds_temp_coh_pc: point count %lld outside [0, 2^31)", n_pc);
+  MR_CHECK_ARG(nl >= 0 && w >= 0, "co_emi_ds_temp_coh_pc: negative raster size");
+  MR_CHECK_ARG(aw >= 1 && rw >= 1 && aw * rw <= 4096, "co_emi_ds_temp_coh_pc: window %dx%d unsupported (<= 4096 entries)",
+               aw, rw);
+  if (n_pc == 0) return MR_OK;
+  MR_CHECK_ARG(rslc && idx && pc_is_shp && ph && quality && t_coh, "co_emi_ds_temp_coh_pc: null pointer");
+  const size_t need = mr_co_emi_ds_temp_coh_pc_workspace_bytes(n_pc, n);
+  MR_CHECK_ARG(workspace != nullptr && workspace_bytes >= need, "co_emi_ds_temp_coh_pc: workspace %zu < %zu bytes",
+               workspace_bytes, need);
+  hipStream_t st = mr::as_stream(stream);
+  const int P = n * (n - 1) / 2;
+  unsigned char* ws = reinterpret_cast<unsigned char*>(workspace);
+  const size_t ew = mr_emi_workspace_bytes(n_pc, n);
+  int32_t* pairs = reinterpret_cast<int32_t*>(ws + coe_round(ew));
+  float2* coh = reinterpret_cast<float2*>(ws + coe_round(ew) + coe_round((size_t)P * 8));
+  hipLaunchKernelGGL(mr::full_pairs_kernel, dim3((n + 63) / 64), dim3(64), 0, st, n, pairs);
+  if (int rc = mr::launch_status("co_emi_ds_temp_coh_pc pairs")) return rc;
+  if (!(n > 32 && n <= 64)) {
+    // the stages one after the other through the coherence buffer
+    if (int rc = mr_emperical_co_pc(rslc, nl, w, n, idx, pc_is_shp, (int)n_pc, aw, rw, pairs, P,
+                                    reinterpret_cast<float*>(coh), stream))
+      return rc;
+    return mr_emi_ds_temp_coh(reinterpret_cast<const float*>(coh), n_pc, n, ref, pairs, P, ph, quality, t_coh, ws, ew,
+                              stream);
+  }
+  unsigned* count = reinterpret_cast<unsigned*>(ws);
+  int* list = reinterpret_cast<int*>(ws + 256);
+  int* res_list = reinterpret_cast<int*>(ws + 256 + mr::emi_list_span(n_pc));
+  hipLaunchKernelGGL(mr::emi_zero_count_kernel, dim3(1), dim3(1), 0, st, count);
+  if (int rc = mr::launch_status("co_emi_ds_temp_coh_pc count init")) return rc;
+  // one wave per slot of the coherence kernel's strip order (a multiple of 8 blocks)
+  const int win = aw * rw;
+  const long long per = (n_pc + 7) / 8;
+  const long long waves = per < 2 * (long long)w ? n_pc : 8 * mr::co_max_slots(per, w);
+  const long long nb = (std::min<long long>(waves, 1 << 20) + 7) / 8 * 8;
+  const float2* z = reinterpret_cast<const float2*>(rslc);
+  float2* p = reinterpret_cast<float2*>(ph);
+  if (n <= 60)
+    hipLaunchKernelGGL((mr::co_emi_pc_kernel<60>), dim3((unsigned)nb), dim3(mr::EMI_FAST_THREADS), mr::coe_lds(n, win),
+                       st, z, nl, w, n, idx, pc_is_shp, n_pc, aw, rw, ref, pairs, coh, p, quality, t_coh, count, list);
+  else
+    hipLaunchKernelGGL((mr::co_emi_pc_kernel<64>), dim3((unsigned)nb), dim3(mr::EMI_FAST_THREADS), mr::coe_lds(n, win),
+                       st, z, nl, w, n, idx, pc_is_shp, n_pc, aw, rw, ref, pairs, coh, p, quality, t_coh, count, list);
+  if (int rc = mr::launch_status("co_emi_pc")) return rc;
+  return mr::emi_hh_passes<true>(coh, n_pc, n, ref, p, quality, pairs, t_coh, count, list, count + 1, res_list,
+                                 mr::EMI_HH_TOL, st);
 }
 
 size_t mr_emi_refined_workspace_bytes(long long n_pts, int n) {

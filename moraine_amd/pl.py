@@ -206,8 +206,68 @@ def emperical_co_emi_temp_coh_pc(rslc, idx, pc_is_shp, batch_size: int = 1000, *
     q = torch.empty(n_pc, dtype=torch.float32, device=z.device)
     t = torch.empty(n_pc, dtype=torch.float32, device=z.device)
     bs = batch_points(batch_size, n, n_pc, torch.cuda.mem_get_info(z.device)[0])
-    for s in range(0, n_pc, bs):
-        e = min(n_pc, s + bs)
-        c = emperical_co_pc(z, ix[s:e], m[s:e])
-        _emi_tcoh_dev(c, n, 0, ph[s:e], q[s:e], t[s:e], tol)
+    if tol is None and not _dev.is_c128(z):
+        co_emi_tcoh_batches(z, ix, m, 0, ph, q, t, bs)
+    else:
+        for s in range(0, n_pc, bs):
+            e = min(n_pc, s + bs)
+            c = emperical_co_pc(z, ix[s:e], m[s:e])
+            _emi_tcoh_dev(c, n, 0, ph[s:e], q[s:e], t[s:e], tol)
     return _dev.back(ph, as_torch), _dev.back(q, as_torch), _dev.back(t, as_torch)
+
+
+_SIDE_STREAMS: dict = {}
+
+
+def _side_streams(device) -> tuple:
+    """Two streams of default priority per device, created once and reused (so
+    their cached workspace blocks are too)."""
+    key = str(device)
+    if key not in _SIDE_STREAMS:
+        _SIDE_STREAMS[key] = (torch.cuda.Stream(device=device), torch.cuda.Stream(device=device))
+    return _SIDE_STREAMS[key]
+
+
+def co_emi_tcoh_batches(z: torch.Tensor, ix: torch.Tensor, m: torch.Tensor, ref: int, ph: torch.Tensor,
+                        q: torch.Tensor, t: torch.Tensor, bs: int, streams: int = 2):
+    """co_emi_tcoh_dev over batches of bs points.  Batches alternate over two
+    side streams (each waits for the caller's stream first; the caller's stream
+    waits for both at the end), so one batch's Householder pass and FP64
+    re-solve -- a small tail of low occupancy -- overlap the next batch's fused
+    launch.  streams=1 runs every batch on the caller's stream, one after the
+    other.  Every point is independent: the outputs do not depend on bs or on
+    the overlap."""
+    n_pc = ix.shape[0]
+    if streams == 1:
+        for s in range(0, n_pc, bs):
+            e = min(n_pc, s + bs)
+            co_emi_tcoh_dev(z, ix[s:e], m[s:e], ref, ph[s:e], q[s:e], t[s:e])
+        return
+    if streams != 2:
+        raise ValueError(f"streams must be 1 or 2, got {streams}")
+    cur = torch.cuda.current_stream(z.device)
+    sides = _side_streams(z.device)
+    for st in sides:
+        st.wait_stream(cur)
+    for b, s in enumerate(range(0, n_pc, bs)):
+        e = min(n_pc, s + bs)
+        with torch.cuda.stream(sides[b % 2]):
+            co_emi_tcoh_dev(z, ix[s:e], m[s:e], ref, ph[s:e], q[s:e], t[s:e])
+    for st in sides:
+        cur.wait_stream(st)
+
+
+def co_emi_tcoh_dev(z: torch.Tensor, ix: torch.Tensor, m: torch.Tensor, ref: int, ph: torch.Tensor, q: torch.Tensor,
+                    t: torch.Tensor):
+    """One batch of emperical_co_emi_temp_coh_pc for complex64 device inputs
+    (z (nl, w, N) complex64, ix (n_pc, 2) int32, m (n_pc, aw, rw) bool, all
+    contiguous) into ph, q, t: coherence, EMI and DS temporal coherence in one
+    launch for 32 < N <= 64 (mr_co_emi_ds_temp_coh_pc); equal bit for bit to
+    emperical_co_pc followed by emi and ds_temp_coh."""
+    nl, w, n = z.shape
+    aw, rw = m.shape[-2:]
+    n_pc = ix.shape[0]
+    ws = _dev.workspace(_lib.query("mr_co_emi_ds_temp_coh_pc_workspace_bytes", n_pc, n), z.device)
+    _lib.call("mr_co_emi_ds_temp_coh_pc", _dev.ptr(z), nl, w, n, _dev.ptr(ix), _dev.ptr(m), n_pc, aw, rw, int(ref),
+              _dev.ptr(ph), _dev.ptr(q), _dev.ptr(t), _dev.ptr(ws), ws.numel(), _dev.stream())
+    _note_householder(ws, n)
