@@ -6,12 +6,12 @@
 // caller (one call per image count).
 //
 // Fused launches (blocks >= 0), one per stage chain, for every n <= 200:
-//  * 32 < n <= 64: one wave per pixel accumulates the coherence on MFMA into
-//    LDS tiles (co_mfma.hpp), runs EMI reading the tiles (emi_fast.hpp), and
-//    forms the temporal coherence from the same tiles (tcoh.hpp).  As in the
-//    plain EMI, a shift-invert pass, a Householder pass over the pixels it
-//    declines (with the conditioning test) and an FP64 re-solve of the pixels
-//    that test flags, each rebuilding the pixel's tiles.
+//  * 32 < n <= 64: as in the plain EMI, a shift-invert pass, a Householder
+//    pass over the pixels it declines (with the conditioning test) and an FP64
+//    re-solve of the pixels that test flags.  Each pass computes the pixel's
+//    coherence on MFMA (co_mfma.hpp) into LDS -- the shift-invert pass as the
+//    packed triu row the plain EMI reads, the others as tiles -- and runs EMI
+//    (emi_fast.hpp) and the temporal coherence (tcoh.hpp) from it.
 //  * n <= 32: one wave per pixel, coherence into one LDS tile, EMI by the
 //    two-lanes-per-row body (emi_wide.hpp, the drop-in's EMI for this n) reading
 //    the tile, temporal coherence from the tile.
@@ -61,11 +61,11 @@ __host__ __device__ inline size_t batch_fused_lds(bool two) {
   return (size_t)com_tile_floats(two) * 4 + 128 * 4 /* scales */ + EMI_FAST_LDS_BYTES;
 }
 
-// MODE 1: EMI by the shift-invert formulation; a pixel it declines is
-// appended to list[*count] and left unwritten.  MODE 2: the Householder
-// formulation for the pixels of that list, appending those its conditioning
-// test flags to res_list[*res_count] (the plain kernels' passes, so the
-// outputs equal the staged path's bit for bit).
+// MODE 2 (the only mode): the Householder formulation for the pixels the
+// shift-invert pass (ds_batch_si_kernel) listed, rebuilding their coherence
+// tiles, appending those its conditioning test flags to res_list[*res_count]
+// (the plain kernels' passes, so the outputs equal the staged path's bit for
+// bit).
 template <int NM, bool TWO, int MODE>
 __global__ __launch_bounds__(64) void ds_batch_fused_kernel(const float2* __restrict__ samples,
                                                             const long long* __restrict__ row_off,
@@ -77,14 +77,15 @@ __global__ __launch_bounds__(64) void ds_batch_fused_kernel(const float2* __rest
                                                             float* __restrict__ tcoh, unsigned* __restrict__ count,
                                                             int* __restrict__ list, unsigned* __restrict__ res_count,
                                                             int* __restrict__ res_list) {
+  static_assert(MODE == 2, "the shift-invert pass is ds_batch_si_kernel");
   extern __shared__ __align__(16) unsigned char smem[];
   float2* tiles = reinterpret_cast<float2*>(smem);
   float* sc = reinterpret_cast<float*>(smem + (size_t)com_tile_floats(TWO) * 4);  // scales, 1/sqrt(diag)
   unsigned char* work = reinterpret_cast<unsigned char*>(sc + 128);  // EMI_FAST_LDS_BYTES
   float2* ub = reinterpret_cast<float2*>(work + 512);               // linked phases after EMI
-  const long long nit = MODE == 2 ? (long long)*count : npix;
+  const long long nit = (long long)*count;
   for (long long it = blockIdx.x; it < nit; it += gridDim.x) {
-    const long long t = MODE == 2 ? (long long)list[it] : it;
+    const long long t = (long long)list[it];
     int lane = threadIdx.x;
     __asm__ volatile("" : "+v"(lane));  // per-pixel opaque lane (see emi.hip)
     const int id = pix[t];
@@ -92,17 +93,9 @@ __global__ __launch_bounds__(64) void ds_batch_fused_kernel(const float2* __rest
     const CohTiles<TWO> ct{tiles, sc + 64};
     float vr, vi;
     double rq;
-    if constexpr (MODE == 1) {
-      if (!emi_inv_point<NM>(ct, n, lane, work, vr, vi, rq)) {
-        if (lane == 0) list[atomicAdd(count, 1u)] = (int)t;
-        __builtin_amdgcn_wave_barrier();
-        continue;
-      }
-    } else {
-      bool flag = false;
-      emi_fast_point<NM, true>(ct, n, lane, work, vr, vi, rq, BATCH_HH_TOL, &flag);
-      if (flag && lane == 0) res_list[atomicAdd(res_count, 1u)] = (int)t;
-    }
+    bool flag = false;
+    emi_fast_point<NM, true>(ct, n, lane, work, vr, vi, rq, BATCH_HH_TOL, &flag);
+    if (flag && lane == 0) res_list[atomicAdd(res_count, 1u)] = (int)t;
     const float2 pv = emi_link_phase(vr, vi, ref);
     __builtin_amdgcn_wave_barrier();
     ub[lane] = pv;  // linked phases for the temporal coherence
@@ -111,6 +104,71 @@ __global__ __launch_bounds__(64) void ds_batch_fused_kernel(const float2* __rest
     __builtin_amdgcn_wave_barrier();
     const float tc = tcoh_point<true>([&](int, int a, int b) { return ct(a, b); }, [&](int i) { return ub[i]; },
                                       pairs, npairs, lane);
+    if (lane == 0) tcoh[id] = tc;
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// The shift-invert pass of the fused launch (32 < n <= 64), every pixel: the
+// coherence accumulates on MFMA and its output pass writes the pixel's triu
+// row straight from the accumulators into LDS -- the values the tiles give,
+// in the layout the plain EMI reads (CohPacked) -- then the shift-invert EMI
+// and the temporal coherence read that row.  A pixel the pass declines goes to
+// list[*count] for the Householder pass (ds_batch_fused_kernel<.., 2>, which
+// rebuilds its tiles).  LDS per wave: the row (16 KB at n = 64), scales, EMI
+// scratch, against the three 8 KB tiles of the Householder pass: four waves
+// per SIMD's worth of LDS per CU instead of three.
+__host__ __device__ inline size_t batch_si_lds(int n) {
+  return (((size_t)(n * (n - 1) / 2) * 8 + 15) & ~(size_t)15) + 128 * 4 + EMI_FAST_LDS_BYTES;
+}
+
+template <int NM>
+__global__ __launch_bounds__(64) void ds_batch_si_kernel(const float2* __restrict__ samples,
+                                                         const long long* __restrict__ row_off,
+                                                         const int32_t* __restrict__ n_samp,
+                                                         const long long* __restrict__ ph_off,
+                                                         const int32_t* __restrict__ pix, long long npix, int n,
+                                                         int ref, const int32_t* __restrict__ pairs, int npairs,
+                                                         float2* __restrict__ ph, float* __restrict__ quality,
+                                                         float* __restrict__ tcoh, unsigned* __restrict__ count,
+                                                         int* __restrict__ list) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int P = n * (n - 1) / 2;
+  float2* cs = reinterpret_cast<float2*>(smem);                                    // the pixel's triu row
+  float* sc = reinterpret_cast<float*>(smem + ((P * 8 + 15) & ~15));               // scales, 1/sqrt(diag)
+  unsigned char* work = reinterpret_cast<unsigned char*>(sc + 128);                // EMI_FAST_LDS_BYTES
+  float2* ub = reinterpret_cast<float2*>(work + 512);                              // linked phases after EMI
+  for (long long t = blockIdx.x; t < npix; t += gridDim.x) {
+    int lane = threadIdx.x;
+    __asm__ volatile("" : "+v"(lane));  // per-pixel opaque lane (see emi.hip)
+    const int id = pix[t];
+    const CoAcc acc = co_mfma_accumulate<true>(samples + row_off[id], RowIdentity{}, n_samp[id], n, lane);
+    __builtin_amdgcn_wave_barrier();
+    co_store_scales<true>(acc, n, lane, sc);
+    {
+      int ln = lane;
+      __asm__ volatile("" : "+v"(ln));
+      co_write_tile_direct(acc.r00, acc.i00, 0, 0, n, ln, sc + 64, cs);
+      co_write_tile_direct(acc.r01, acc.i01, 0, 1, n, ln, sc + 64, cs);
+      co_write_tile_direct(acc.r11, acc.i11, 1, 1, n, ln, sc + 64, cs);
+    }
+    __builtin_amdgcn_wave_barrier();
+    float vr, vi;
+    double rq;
+    if (!emi_inv_point<NM>(CohPacked{cs, n}, n, lane, work, vr, vi, rq)) {
+      if (lane == 0) list[atomicAdd(count, 1u)] = (int)t;
+      __builtin_amdgcn_wave_barrier();
+      continue;
+    }
+    const float2 pv = emi_link_phase(vr, vi, ref);
+    __builtin_amdgcn_wave_barrier();
+    ub[lane] = pv;  // linked phases for the temporal coherence
+    if (lane < n) ph[ph_off[id] + lane] = pv;
+    if (lane == 0) quality[id] = (float)rq;
+    __builtin_amdgcn_wave_barrier();
+    // pairs is the full triu network (mr_ds_batch's contract): row entry q is pair q
+    const float tc = tcoh_point<true>([&](int q, int, int) { return cs[q]; }, [&](int i) { return ub[i]; }, pairs,
+                                      npairs, lane);
     if (lane == 0) tcoh[id] = tc;
     __builtin_amdgcn_wave_barrier();
   }
@@ -355,13 +413,11 @@ int mr_ds_batch(const float* samples, const long long* row_off, const int32_t* n
     const int nb = (int)std::min<long long>(n_pix, blocks > 0 ? blocks : mr::BATCH_DEFAULT_BLOCKS);
     // the shift-invert pass keeps 60 columns per lane up to n = 60 (as the plain EMI)
     if (n <= 60)
-      hipLaunchKernelGGL((mr::ds_batch_fused_kernel<60, true, 1>), dim3(nb), dim3(64), mr::batch_fused_lds(true), st,
-                         smp, row_off, n_samples, ph_off, pix, n_pix, n, ref, pairs, n_pairs, phc, quality, t_coh,
-                         count, list, res_count, res_list);
+      hipLaunchKernelGGL((mr::ds_batch_si_kernel<60>), dim3(nb), dim3(64), mr::batch_si_lds(n), st, smp, row_off,
+                         n_samples, ph_off, pix, n_pix, n, ref, pairs, n_pairs, phc, quality, t_coh, count, list);
     else
-      hipLaunchKernelGGL((mr::ds_batch_fused_kernel<64, true, 1>), dim3(nb), dim3(64), mr::batch_fused_lds(true), st,
-                         smp, row_off, n_samples, ph_off, pix, n_pix, n, ref, pairs, n_pairs, phc, quality, t_coh,
-                         count, list, res_count, res_list);
+      hipLaunchKernelGGL((mr::ds_batch_si_kernel<64>), dim3(nb), dim3(64), mr::batch_si_lds(n), st, smp, row_off,
+                         n_samples, ph_off, pix, n_pix, n, ref, pairs, n_pairs, phc, quality, t_coh, count, list);
     if (int rc = mr::launch_status("ds_batch shift-invert")) return rc;
     const int nb2 = (int)std::min<long long>(n_pix, mr::BATCH_FALLBACK_BLOCKS);
     hipLaunchKernelGGL((mr::ds_batch_fused_kernel<64, true, 2>), dim3(nb2), dim3(64), mr::batch_fused_lds(true), st,

@@ -17,7 +17,9 @@ The loop (:class:`DSServer`):
     first) up to a pixel budget and a coherence-buffer budget, computes their
     coherence into one shared buffer (one launch per request chunk), then one
     EMI and one temporal-coherence launch per image count over the whole tick,
-    and scatters the results into the requests' outputs;
+    and scatters the results into the requests' outputs; for 32 < N <= 64 each
+    request chunk instead runs coherence, EMI and temporal coherence in one
+    launch (mr_co_emi_ds_temp_coh_pc) straight into the request's outputs;
   * keeps up to ``max_inflight`` ticks queued on the device and frees a request
     (and its HBM) as soon as the tick holding its last pixels has finished.
 Every kernel is per-pixel and deterministic, so a request's result does not
@@ -170,6 +172,7 @@ class DSServer:
         self.admit_stream = torch.cuda.Stream(self.dev)  # default priority
         self._pairs: dict[int, torch.Tensor] = {}
         self._emi_ws = None
+        self._fused_ws = None
         self.t0 = time.perf_counter()
         self.max_concurrent = 0
         self.peak_used = 0
@@ -231,6 +234,12 @@ class DSServer:
             self._emi_ws = _dev.workspace(need, self.dev)
         return self._emi_ws
 
+    def _fused_workspace(self, rows: int, n: int):
+        need = _lib.query("mr_co_emi_ds_temp_coh_pc_workspace_bytes", rows, n)
+        if self._fused_ws is None or self._fused_ws.numel() < need:
+            self._fused_ws = _dev.workspace(need, self.dev)
+        return self._fused_ws
+
     def _run_tick(self, plan):
         with torch.cuda.stream(self.compute):
             self._run_tick_on_stream(plan)
@@ -245,6 +254,20 @@ class DSServer:
             self.compute.wait_event(a.ready)
         off_bytes = 0
         for n, chunks in by_n.items():
+            if 32 < n <= 64:
+                # coherence, EMI and the DS temporal coherence of each request chunk in one
+                # launch (mr_co_emi_ds_temp_coh_pc), straight into the request's outputs
+                for a, take in chunks:
+                    s = a.spec
+                    p0 = a.cursor
+                    ws = self._fused_workspace(take, n)
+                    _lib.call("mr_co_emi_ds_temp_coh_pc", _dev.ptr(a.rslc), s.h, s.w, n,
+                              _dev.ptr(a.idx[p0:p0 + take]), _dev.ptr(a.mask[p0:p0 + take]), take, 2 * s.az + 1,
+                              2 * s.r + 1, 0, _dev.ptr(a.ph[p0:p0 + take]), _dev.ptr(a.q[p0:p0 + take]),
+                              _dev.ptr(a.t[p0:p0 + take]), _dev.ptr(ws), ws.numel(), st)
+                    a.cursor += take
+                    a.last_tick = self.tick_id
+                continue
             P = n * (n - 1) // 2
             rows = sum(t for _, t in chunks)
             base = off_bytes // 8
@@ -367,11 +390,11 @@ def run_stream(specs, server: DSServer):
     return sorted(server.results, key=lambda r: r.rid)
 
 
-def run_serial(specs, coh_budget: int = 2 << 30, ps_frac: float = 0.1, keep_outputs: bool = False):
-    """The same stream through the plain drop-in functions, one request at a
-    time in arrival order (the request waits while the previous one runs)."""
+def run_serial(specs, ps_frac: float = 0.1, keep_outputs: bool = False):
+    """The same stream through the plain drop-in functions (ks_test_select_shp,
+    then emperical_co_emi_temp_coh_pc), one request at a time in arrival order
+    (the request waits while the previous one runs)."""
     from . import pl, shp, synth
-    from .co import emperical_co_pc
 
     dev = _dev.device()
     out = []
@@ -391,15 +414,7 @@ def run_serial(specs, coh_budget: int = 2 << 30, ps_frac: float = 0.1, keep_outp
                                 torch.arange(s.w, device=dev, dtype=torch.int32), indexing="ij")
         idx = torch.stack([ii.reshape(-1), jj.reshape(-1)], -1).contiguous()
         m = mask.view(s.npx, 2 * s.az + 1, 2 * s.r + 1)
-        P = s.n * (s.n - 1) // 2
-        bs = max(1, min(s.npx, coh_budget // (8 * P)))
-        ph = torch.empty((s.npx, s.n), dtype=torch.complex64, device=dev)
-        q = torch.empty(s.npx, dtype=torch.float32, device=dev)
-        t = torch.empty(s.npx, dtype=torch.float32, device=dev)
-        for b0 in range(0, s.npx, bs):
-            b1 = min(s.npx, b0 + bs)
-            c = emperical_co_pc(z, idx[b0:b1], m[b0:b1])
-            pl._emi_tcoh_dev(c, s.n, 0, ph[b0:b1], q[b0:b1], t[b0:b1])
+        ph, q, t = pl.emperical_co_emi_temp_coh_pc(z, idx, m)
         torch.cuda.synchronize()
         done = time.perf_counter() - t0
         out.append(PatchResult(s.rid, ph.view(s.h, s.w, s.n) if keep_outputs else None,
