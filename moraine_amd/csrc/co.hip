@@ -500,6 +500,7 @@ int mr_emperical_co_pc(const float* rslc, int nl, int w, int n, const int32_t* i
                        int aw, int rw, const int32_t* pairs, int n_pairs, float* coh, mr_stream_t stream) {
   MR_CHECK_ARG(n_pc >= 0 && n_pairs >= 0, "emperical_co_pc: negative sizes");
   if (n_pc == 0 || n_pairs == 0) return MR_OK;
+  MR_CHECK_ARG(nl > 0 && w > 0, "emperical_co_pc: empty raster (%d x %d) with %d points", nl, w, n_pc);
   MR_CHECK_ARG(rslc && idx && pc_is_shp && pairs && coh, "emperical_co_pc: null pointer");
   if (int rc = mr::co_check(n, aw, rw)) return rc;
   if (n <= mr::CO_MFMA_MAX_N)

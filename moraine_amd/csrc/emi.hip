@@ -473,6 +473,7 @@ int mr_co_emi_ds_temp_coh_pc(const float* rslc, int nl, int w, int n, const int3
   MR_CHECK_ARG(aw >= 1 && rw >= 1 && aw * rw <= 4096, "co_emi_ds_temp_coh_pc: window %dx%d unsupported (<= 4096 entries)",
                aw, rw);
   if (n_pc == 0) return MR_OK;
+  MR_CHECK_ARG(nl > 0 && w > 0, "co_emi_ds_temp_coh_pc: empty raster (%d x %d) with %lld points", nl, w, n_pc);
   MR_CHECK_ARG(rslc && idx && pc_is_shp && ph && quality && t_coh, "co_emi_ds_temp_coh_pc: null pointer");
   const size_t need = mr_co_emi_ds_temp_coh_pc_workspace_bytes(n_pc, n);
   MR_CHECK_ARG(workspace != nullptr && workspace_bytes >= need, "co_emi_ds_temp_coh_pc: workspace %zu < %zu bytes",

@@ -172,3 +172,23 @@ def test_batch_points_respects_hbm_budget():
                 assert bs >= min(max(bsz, MIN_BATCH_POINTS), int(BATCH_HBM_SHARE * free) // per, 10 ** 7) or bs == 1
     assert batch_points(1000, 60, 5000, 250 * gib) == 5000  # never more than the point count
     assert batch_points(1000, 1000, 10 ** 6, 16 * gib) == 1 * 16 * gib // 4 // (16 * 499500 + 16000)
+
+
+def test_fused_driver_step_argument_checks():
+    """mr_co_emi_ds_temp_coh_pc validates its arguments before touching the GPU."""
+    call = lambda *a: _lib.call("mr_co_emi_ds_temp_coh_pc", *a)  # noqa: E731
+    # (rslc, nl, w, n, idx, mask, n_pc, aw, rw, ref, ph, q, t, ws, ws_bytes, stream)
+    with pytest.raises(ValueError, match="ref"):
+        call(None, 4, 4, 60, None, None, 10, 3, 3, 60, None, None, None, None, 0, None)
+    with pytest.raises(ValueError, match="window"):
+        call(None, 4, 4, 60, None, None, 10, 65, 65, 0, None, None, None, None, 0, None)
+    with pytest.raises(ValueError, match="empty raster"):
+        call(None, 0, 4, 60, None, None, 10, 3, 3, 0, None, None, None, None, 0, None)
+    with pytest.raises(ValueError, match="workspace"):
+        x = ctypes.c_void_p(16)  # never dereferenced: the workspace check fails first
+        call(x, 4, 4, 60, x, x, 10, 3, 3, 0, x, x, x, None, 0, None)
+    assert _lib.call("mr_co_emi_ds_temp_coh_pc", None, 4, 4, 60, None, None, 0, 3, 3, 0, None, None, None, None, 0,
+                     None) == 0  # no points: nothing to do
+    P = 60 * 59 // 2
+    need = _lib.query("mr_co_emi_ds_temp_coh_pc_workspace_bytes", 1000, 60)
+    assert need >= 1000 * P * 8 + P * 8 + 1000 * 8
