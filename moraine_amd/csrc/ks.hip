@@ -8,18 +8,24 @@
 //     written once to a workspace with a per-pixel "has NaN" flag (the
 //     reference's "last sorted element is NaN" test).
 //   * ks_pairs_kernel: one 256-thread block per (azimuth line, column tile).
-//     The tile's reference stacks stay in LDS; for each window row offset the
+//     The tile's reference stacks stay in LDS (the sorted keys turned back
+//     into values, a NaN after each stack); for each window row offset the
 //     secondary line segment (tile + range halo) is staged in LDS and every
 //     (pixel, range offset) pair runs the merge walk of moraine/shp.py:35-44
-//     on integer keys.  The walk yields the integer step count k = N*D; p,
+//     with the reference's float compares; at a NaN neither compare holds, so
+//     a walk stops where moraine's loop does with no index bound per step.
+//     The walk yields the integer step count k = N*D; p,
 //     dist and the SHP bit are table lookups of k, so they are bit-exact with
 //     the reference CPU path (p = dtype(f64 series), dist = dtype(k/N)).
 //     Several walks are interleaved per thread (ILP) to hide LDS latency.
 //     The walk is symmetric in its two stacks, so only the window offsets
 //     with l > 0, or l = 0 and m >= 0, are walked (half the window plus the
-//     pixel itself); each result is also written, mirrored, to the partner
-//     pixel's opposite offset, and SHP counts of mirrored entries accumulate
-//     with global atomics.
+//     pixel itself); each walk stores its step count k (one byte for n < 255)
+//     in a workspace.
+//   * ks_emit_kernel: one block per (line, column tile) writes each pixel's
+//     whole p, dist and mask rows and its SHP count, reading its own walks
+//     and, mirrored, those of the neighbours that walked its other offsets
+//     (no atomics, full-line writes).
 #include <mutex>
 
 #include "common.hpp"
@@ -156,6 +162,10 @@ struct KeyOf<float> {
     if (b == 0x80000000u) b = 0u;  // -0 == +0 under the reference's float compares
     return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
   }
+  // the value a key stands for (-0 comes back as +0, which compares the same)
+  __device__ static float value(uint32_t k) {
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+  }
 };
 template <>
 struct KeyOf<double> {
@@ -164,6 +174,9 @@ struct KeyOf<double> {
     uint64_t b = (uint64_t)__double_as_longlong(x);
     if (b == 0x8000000000000000ull) b = 0ull;
     return (b & 0x8000000000000000ull) ? ~b : (b | 0x8000000000000000ull);
+  }
+  __device__ static double value(uint64_t k) {
+    return __longlong_as_double((long long)((k & 0x8000000000000000ull) ? (k & 0x7fffffffffffffffull) : ~k));
   }
 };
 
@@ -238,9 +251,10 @@ __global__ __launch_bounds__(256) void ks_sort_kernel(const T* __restrict__ rmli
 constexpr int KS_THREADS = 256;
 // walks interleaved per thread: 4 or 6, picked per launch (ks_pick_ilp)
 
-template <typename Key>
-__device__ __forceinline__ void stage_stacks(Key* __restrict__ dst, const Key* __restrict__ src, int nstk, int n,
-                                             int ns) {
+template <typename T>
+__device__ __forceinline__ void stage_stacks(T* __restrict__ dst, const typename KeyOf<T>::type* __restrict__ src,
+                                             int nstk, int n, int ns) {
+  using Key = typename KeyOf<T>::type;
   const int tot = nstk * n;
   // eight loads per thread are in flight before their LDS stores (a plain
   // copy loop waits for every load before issuing the next)
@@ -256,11 +270,13 @@ __device__ __forceinline__ void stage_stacks(Key* __restrict__ dst, const Key* _
       const int i = i0 + u * KS_THREADS + threadIdx.x;
       if (i < tot) {
         const int s = i / n;
-        dst[s * ns + (i - s * n)] = v[u];
+        dst[s * ns + (i - s * n)] = KeyOf<T>::value(v[u]);
       }
     }
   }
-  for (int s = threadIdx.x; s < nstk; s += KS_THREADS) dst[s * ns + n] = ~(Key)0;  // sentinel
+  // sentinel: NaN compares neither <= nor >= anything, so a walk that reaches
+  // the end of either stack stops there (as moraine's loop does)
+  for (int s = threadIdx.x; s < nstk; s += KS_THREADS) dst[s * ns + n] = (T)NAN;
 }
 
 // Walk results: the integer step count k = N*D of each walked (pixel, offset)
@@ -300,7 +316,6 @@ __global__ __launch_bounds__(KS_THREADS) void ks_pairs_kernel(const typename Key
                                                               const uint8_t* __restrict__ valid, int nl, int w,
                                                               int n, int az, int r, int tw, int ntx,
                                                               long long chunk, KB* __restrict__ kbuf) {
-  using Key = typename KeyOf<T>::type;
   extern __shared__ __align__(16) unsigned char smem[];
   int tile, a;
   if (!ks_tile_of(ntx, nl, chunk, tile, a)) return;
@@ -309,8 +324,8 @@ __global__ __launch_bounds__(KS_THREADS) void ks_pairs_kernel(const typename Key
   const int ns = n + 1;
   const int rw = 2 * r + 1;
   const int scols = tcols + 2 * r;
-  Key* ref = reinterpret_cast<Key*>(smem);
-  Key* sec = ref + tw * ns;
+  T* ref = reinterpret_cast<T*>(smem);  // sorted stacks as values, NaN after each
+  T* sec = ref + tw * ns;
   uint8_t* refv = reinterpret_cast<uint8_t*>(sec + (tw + 2 * r) * ns);
   uint8_t* secv = refv + tw;
 
@@ -344,7 +359,7 @@ __global__ __launch_bounds__(KS_THREADS) void ks_pairs_kernel(const typename Key
     for (int t0 = threadIdx.x; t0 < npairs; t0 += KS_THREADS * KS_ILP) {
       int ia[KS_ILP], ib[KS_ILP], ra[KS_ILP], rb[KS_ILP], km[KS_ILP], cc[KS_ILP], mm[KS_ILP];
       bool ok[KS_ILP], in[KS_ILP];
-      Key ka[KS_ILP], kb_[KS_ILP];
+      T ka[KS_ILP], kb_[KS_ILP];
 #pragma unroll
       for (int q = 0; q < KS_ILP; ++q) {
         const int t = t0 + q * KS_THREADS;
@@ -362,7 +377,11 @@ __global__ __launch_bounds__(KS_THREADS) void ks_pairs_kernel(const typename Key
         ka[q] = ref[ra[q]];
         kb_[q] = ref[rb[q]];
       }
-      // merge walk of moraine/shp.py:35-44 on keys: '<=' advances ref, '>=' advances sec
+      // merge walk of moraine/shp.py:35-44 on the values: '<=' advances ref,
+      // '>=' advances sec.  At the NaN after either stack neither compare
+      // holds, so the walk stays where moraine's loop ends (the walks of a
+      // thread run until the slowest one is done; |ib - ia| can only shrink
+      // after either stack is exhausted)
       for (int it = 0; it < 2 * n; it += 4) {
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -370,8 +389,11 @@ __global__ __launch_bounds__(KS_THREADS) void ks_pairs_kernel(const typename Key
           for (int q = 0; q < KS_ILP; ++q) {
             const int le = ka[q] <= kb_[q];
             const int ge = ka[q] >= kb_[q];
-            ia[q] = min(ia[q] + le, n);
-            ib[q] = min(ib[q] + ge, n);
+            ia[q] += le;
+            ib[q] += ge;
+            // opaque: keeps the compiler from turning the index updates into
+            // running address sums (more instructions per step, not fewer)
+            __asm__ volatile("" : "+v"(ia[q]), "+v"(ib[q]));
             // |ib - ia| in one v_sad_u16 (both in [0, n], n < 2^16: the high halves are zero)
             km[q] = max(km[q], (int)__builtin_amdgcn_sad_u16((unsigned)ib[q], (unsigned)ia[q], 0u));
             ka[q] = ref[ra[q] + ia[q]];

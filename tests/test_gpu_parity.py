@@ -90,6 +90,22 @@ def test_ks_sizes_and_ties(port, n):
     _bits_equal(d, d_ref)
 
 
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_ks_denormal_and_signed_values(port, dt):
+    """The walk compares the values as the reference does: denormals stay
+    distinct (no flush to zero), -0 ties +0, negatives order below them."""
+    mr = _mr()
+    rng = np.random.default_rng(11)
+    tiny = np.finfo(dt).smallest_subnormal
+    vals = np.array([0.0, -0.0, tiny, -tiny, 2 * tiny, -3 * tiny, 1e3 * tiny, np.finfo(dt).tiny, -1.0, 1.0], dt)
+    rmli = vals[rng.integers(0, vals.size, (6, 8, 40))]
+    _case(f"ks denormal and signed values, {np.dtype(dt).name}")
+    d, p = mr.ks_test(rmli, 2, 3, return_dist=True)
+    d_ref, p_ref = R.ks_test(rmli, 2, 3, return_dist=True)
+    _bits_equal(p, p_ref)
+    _bits_equal(d, d_ref)
+
+
 def test_ks_edge_windows_and_f64(port):
     mr = _mr()
     rng = np.random.default_rng(7)
