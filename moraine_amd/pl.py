@@ -264,9 +264,18 @@ def co_emi_tcoh_dev(z: torch.Tensor, ix: torch.Tensor, m: torch.Tensor, ref: int
     contiguous) into ph, q, t: coherence, EMI and DS temporal coherence in one
     launch for 32 < N <= 64 (mr_co_emi_ds_temp_coh_pc); equal bit for bit to
     emperical_co_pc followed by emi and ds_temp_coh."""
+    if z.dtype != torch.complex64 or ix.dtype != torch.int32 or m.dtype not in (torch.bool, torch.uint8):
+        raise TypeError("co_emi_tcoh_dev: complex64 stack, int32 points and a bool/uint8 mask expected")
+    if not (z.is_contiguous() and ix.is_contiguous() and m.is_contiguous()):
+        raise ValueError("co_emi_tcoh_dev: inputs must be contiguous")
     nl, w, n = z.shape
     aw, rw = m.shape[-2:]
     n_pc = ix.shape[0]
+    if (tuple(ph.shape) != (n_pc, n) or tuple(q.shape) != (n_pc,) or tuple(t.shape) != (n_pc,)
+            or ph.dtype != torch.complex64 or q.dtype != torch.float32 or t.dtype != torch.float32
+            or not (ph.is_contiguous() and q.is_contiguous() and t.is_contiguous())):
+        raise ValueError("co_emi_tcoh_dev: outputs must be contiguous ph (n_pc, N) complex64, q and t (n_pc,) "
+                         "float32")
     ws = _dev.workspace(_lib.query("mr_co_emi_ds_temp_coh_pc_workspace_bytes", n_pc, n), z.device)
     _lib.call("mr_co_emi_ds_temp_coh_pc", _dev.ptr(z), nl, w, n, _dev.ptr(ix), _dev.ptr(m), n_pc, aw, rw, int(ref),
               _dev.ptr(ph), _dev.ptr(q), _dev.ptr(t), _dev.ptr(ws), ws.numel(), _dev.stream())
