@@ -798,7 +798,7 @@ def main():
         "metric": "DS pixels/s (KS+coh+EMI+tcoh) at N=60; p50/p99 latency; phase err vs FP64",
         "value": value, "unit": "DS pixels/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 (KS: int keys; EMI: f64 inverse + f32 eigen)", "data": "synthetic (seeded DS/PS model)",
+        "dtype": "f32 (KS: sorted on int keys, integer step counts; coherence: fp16 hi+lo MFMA, f32 sums; EMI: f64 inverse + f32 eigen)", "data": "synthetic (seeded DS/PS model)",
         "config": {"workload": f"plain drop-in pipeline, N={N}, {S}x{S} tile, {aw}x{rw} window, "
                                f"{int(args.ps*100)}% PS, p_max=0.05, all pixels", "pixels": npx,
                    "n_images": N, "window": [aw, rw], "batch": args.batch, "batch_streams": args.batch_streams},

@@ -7,7 +7,8 @@ host memory: the raster is cut into row bands; each band is copied with its
 KS half-window of halo rows above and below through pinned buffers on a copy
 stream while the previous band computes on the compute stream, and its results
 come back through pinned buffers, one band behind.  Per band the chain is the
-drop-in one (ks_test_select_shp -> emperical_co_pc -> emi + ds_temp_coh), so
+drop-in one (ks_test_select_shp, then emperical_co_emi_temp_coh_pc's batches:
+coherence -> emi -> ds_temp_coh), so
 every pixel's outputs equal the whole-tile computation bit for bit: the halo
 gives each interior pixel its full window, and the clipped bands at the raster's
 first and last rows meet the raster's own edges.
@@ -17,8 +18,7 @@ from __future__ import annotations
 import numpy as np
 import torch
 
-from .co import emperical_co_pc
-from .pl import _emi_tcoh_dev
+from .pl import co_emi_tcoh_batches
 from .shp import ks_test_select_shp
 
 __all__ = ["ds_pipeline_streamed"]
@@ -112,10 +112,8 @@ def ds_pipeline_streamed(rslc, az_half_win: int, r_half_win: int, p_max: float =
         npt = idx.shape[0]
         ph_b = torch.empty((npt, n), dtype=torch.complex64, device=dev)
         qtc = torch.empty((3, npt), dtype=torch.float32, device=dev)
-        for p0 in range(0, npt, batch):
-            p1 = min(npt, p0 + batch)
-            coh = emperical_co_pc(zb, idx[p0:p1], pcs[p0:p1])
-            _emi_tcoh_dev(coh, n, 0, ph_b[p0:p1], qtc[0, p0:p1], qtc[1, p0:p1])
+        # emperical_co_emi_temp_coh_pc's batches (joined back to this stream before it returns)
+        co_emi_tcoh_batches(zb, idx, pcs, 0, ph_b, qtc[0], qtc[1], batch)
         qtc[2].copy_(c[i0:i0 + m].reshape(-1).view(torch.float32))
         done_in[s].record(comp)  # this slot's input may be overwritten from here on
         # the next band's host copy and upload, and the host copy of band k-2's results,
