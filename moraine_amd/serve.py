@@ -19,7 +19,8 @@ The loop (:class:`DSServer`):
     EMI and one temporal-coherence launch per image count over the whole tick,
     and scatters the results into the requests' outputs; for 32 < N <= 64 each
     request chunk instead runs coherence, EMI and temporal coherence in one
-    launch (mr_co_emi_ds_temp_coh_pc) straight into the request's outputs;
+    launch (mr_co_emi_ds_temp_coh_pc) straight into the request's outputs,
+    chunks alternating over two side streams;
   * keeps up to ``max_inflight`` ticks queued on the device and frees a request
     (and its HBM) as soon as the tick holding its last pixels has finished.
 Every kernel is per-pixel and deterministic, so a request's result does not
@@ -37,6 +38,7 @@ import torch
 
 from . import _dev, _lib
 from .co import _pairs
+from .pl import _side_streams
 
 
 @dataclass
@@ -172,7 +174,7 @@ class DSServer:
         self.admit_stream = torch.cuda.Stream(self.dev)  # default priority
         self._pairs: dict[int, torch.Tensor] = {}
         self._emi_ws = None
-        self._fused_ws = None
+        self._fused_ws = [None, None]
         self.t0 = time.perf_counter()
         self.max_concurrent = 0
         self.peak_used = 0
@@ -234,11 +236,12 @@ class DSServer:
             self._emi_ws = _dev.workspace(need, self.dev)
         return self._emi_ws
 
-    def _fused_workspace(self, rows: int, n: int):
+    def _fused_workspace(self, rows: int, n: int, slot: int):
+        """Workspace of side stream `slot` (allocated on that stream, which is current)."""
         need = _lib.query("mr_co_emi_ds_temp_coh_pc_workspace_bytes", rows, n)
-        if self._fused_ws is None or self._fused_ws.numel() < need:
-            self._fused_ws = _dev.workspace(need, self.dev)
-        return self._fused_ws
+        if self._fused_ws[slot] is None or self._fused_ws[slot].numel() < need:
+            self._fused_ws[slot] = _dev.workspace(need, self.dev)
+        return self._fused_ws[slot]
 
     def _run_tick(self, plan):
         with torch.cuda.stream(self.compute):
@@ -256,17 +259,28 @@ class DSServer:
         for n, chunks in by_n.items():
             if 32 < n <= 64:
                 # coherence, EMI and the DS temporal coherence of each request chunk in one
-                # launch (mr_co_emi_ds_temp_coh_pc), straight into the request's outputs
-                for a, take in chunks:
+                # launch (mr_co_emi_ds_temp_coh_pc), straight into the request's outputs; the
+                # chunks alternate over the two side streams (forked from and joined back to
+                # the compute stream), so one chunk's Householder pass and re-solve overlap the
+                # next chunk's launch, as in the drop-in driver
+                sides = _side_streams(self.dev)
+                cur = torch.cuda.current_stream(self.dev)
+                for sd in sides:
+                    sd.wait_stream(cur)
+                for i, (a, take) in enumerate(chunks):
                     s = a.spec
                     p0 = a.cursor
-                    ws = self._fused_workspace(take, n)
-                    _lib.call("mr_co_emi_ds_temp_coh_pc", _dev.ptr(a.rslc), s.h, s.w, n,
-                              _dev.ptr(a.idx[p0:p0 + take]), _dev.ptr(a.mask[p0:p0 + take]), take, 2 * s.az + 1,
-                              2 * s.r + 1, 0, _dev.ptr(a.ph[p0:p0 + take]), _dev.ptr(a.q[p0:p0 + take]),
-                              _dev.ptr(a.t[p0:p0 + take]), _dev.ptr(ws), ws.numel(), st)
+                    with torch.cuda.stream(sides[i % 2]):
+                        ws = self._fused_workspace(take, n, i % 2)
+                        _lib.call("mr_co_emi_ds_temp_coh_pc", _dev.ptr(a.rslc), s.h, s.w, n,
+                                  _dev.ptr(a.idx[p0:p0 + take]), _dev.ptr(a.mask[p0:p0 + take]), take,
+                                  2 * s.az + 1, 2 * s.r + 1, 0, _dev.ptr(a.ph[p0:p0 + take]),
+                                  _dev.ptr(a.q[p0:p0 + take]), _dev.ptr(a.t[p0:p0 + take]), _dev.ptr(ws),
+                                  ws.numel(), _dev.stream())
                     a.cursor += take
                     a.last_tick = self.tick_id
+                for sd in sides:
+                    cur.wait_stream(sd)
                 continue
             P = n * (n - 1) // 2
             rows = sum(t for _, t in chunks)
