@@ -25,7 +25,7 @@ __all__ = ["ds_pipeline_streamed"]
 
 
 def ds_pipeline_streamed(rslc, az_half_win: int, r_half_win: int, p_max: float = 0.05, band_rows: int = 256,
-                         batch: int = 1 << 18, device="cuda"):
+                         batch: int = 1 << 20, device="cuda"):
     """DS phase linking of every pixel of a host (nlines, width, nimages)
     complex64 stack (numpy, memmap welcome), streamed through the GPU in bands.
 
