@@ -148,8 +148,8 @@ class _Active:
 
 
 class DSServer:
-    def __init__(self, device=None, hbm_budget: int | None = None, coh_budget: int = 2 << 30,
-                 max_tick_px: int = 1 << 18, max_inflight: int = 2, max_active: int = 64, keep_outputs: bool = True,
+    def __init__(self, device=None, hbm_budget: int | None = None, coh_budget: int = 8 << 30,
+                 max_tick_px: int = 1 << 20, max_inflight: int = 2, max_active: int = 64, keep_outputs: bool = True,
                  ps_frac: float = 0.1):
         self.dev = torch.device(device) if device is not None else _dev.device()
         self.coh_budget = int(coh_budget)
