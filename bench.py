@@ -52,7 +52,7 @@ def parse():
     ap.add_argument("--az", type=int, default=7)
     ap.add_argument("--r", type=int, default=10)
     ap.add_argument("--ps", type=float, default=0.10)
-    ap.add_argument("--batch", type=int, default=1 << 20, help="pixels per coherence/EMI batch")
+    ap.add_argument("--batch", type=int, default=1 << 21, help="pixels per coherence/EMI batch")
     ap.add_argument("--batch-streams", type=int, default=2, choices=(1, 2),
                     help="streams the plain pass's batches alternate over (1: one after the other, for "
                          "per-kernel profiles without overlap)")
@@ -78,7 +78,7 @@ def progress(msg):
 def measured_traffic(kernels, args):
     """HBM bytes per launch of the given kernels (summed) from the committed FETCH_SIZE/WRITE_SIZE
     counter summary (copy-kernel calibrated), valid only for the workload it was measured on."""
-    if (args.size, args.n, args.az, args.r, args.batch) != (2500, 60, 7, 10, 1 << 20):
+    if (args.size, args.n, args.az, args.r, args.batch) != (2500, 60, 7, 10, 1 << 21):
         return None
     try:
         with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:

@@ -171,7 +171,7 @@ def emi_ds_temp_coh(coh, ref: int = 0, *, refine=None):
 
 #: a batch of the batch driver holds at least this many points (fewer only
 #: lengthen the run: every point is independent) ...
-MIN_BATCH_POINTS = 1 << 20
+MIN_BATCH_POINTS = 1 << 21
 #: ... and its coherence buffer takes at most this share of the free HBM
 BATCH_HBM_SHARE = 0.25
 
@@ -191,9 +191,10 @@ def emperical_co_emi_temp_coh_pc(rslc, idx, pc_is_shp, batch_size: int = 1000, *
 
     Every point is independent and every kernel deterministic, so the result
     does not depend on the batch size.  The batch is batch_points(batch_size,
-    ...): at least 2^20 points (smaller batches run slower: 65536-point batches
-    take 1.2x as long as 2^18-point ones at N = 60, 2^20-point ones 0.97x), at
-    most what a quarter of the free HBM holds.
+    ...): at least 2^21 points (smaller batches run slower: at N = 60, 15x21
+    windows, 65536-point batches take 1.2x as long as 2^18-point ones, 2^20-point
+    ones 0.97x, 2^21-point ones 0.95x), at most what a quarter of the free HBM
+    holds.
     refine: as for emi (not in moraine).  complex128 input runs in FP64 and
     returns complex128 phases.
     """
